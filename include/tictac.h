@@ -1,0 +1,245 @@
+/*
+ * tictac.h -- C-ABI of the MI355X (gfx950) AGS/WPCD solve path.
+ *
+ * Drop-in boundary for the three solve-path entry points of seanbsm/Tic-tac
+ * (all called from CPP/main.cpp) plus the state-space builders whose
+ * conventions they consume.  Plain C: pointers, sizes, status codes; no C++ or
+ * torch types.  Array layouts, index order (idx = alpha*Nq*Np + q*Np + p),
+ * units (MeV) and on-shell conventions are the reference's.
+ *
+ * Errors: every entry point returns tt_status; tt_last_error() gives the
+ * message of the last failure on the calling thread.  Nothing calls exit()
+ * (the reference's raise_error exits with status 0, CPP/error_management.cpp:4-14).
+ *
+ * Complex arrays are interleaved (re, im) doubles, i.e. the memory layout of
+ * std::complex<double> / cdouble in the reference.
+ */
+#ifndef TICTAC_H
+#define TICTAC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TT_OK = 0,
+    TT_ERR_ARG = 1,          /* invalid argument or inconsistent sizes          */
+    TT_ERR_HIP = 2,          /* HIP runtime error                               */
+    TT_ERR_ALLOC = 3,        /* host or device allocation failed                */
+    TT_ERR_PHYSICS = 4,      /* e.g. unphysical bound states, T_lab not in space */
+    TT_ERR_NOT_CONVERGED = 5,/* iterative solve hit its iteration limit          */
+    TT_ERR_UNSUPPORTED = 6   /* option combination not implemented               */
+} tt_status;
+
+const char *tt_last_error(void);
+const char *tt_version(void);
+
+/* ------------------------------------------------------------------ context */
+typedef struct tt_ctx tt_ctx;
+/* Binds a device and creates one default-priority HIP stream. */
+tt_status tt_ctx_create(int device, tt_ctx **out);
+void tt_ctx_destroy(tt_ctx *ctx);
+/* The stream all work of this context is issued on (a hipStream_t). */
+void *tt_ctx_stream(tt_ctx *ctx);
+tt_status tt_ctx_synchronize(tt_ctx *ctx);
+
+/* ------------------------------------------------------------ run parameters
+ * Subset of run_params (CPP/type_defs.h:87-123) used by the path. */
+typedef struct {
+    int two_J_3N_max;
+    int J_2N_max;
+    int Np_WP, Nq_WP;
+    int Np_per_WP;          /* quadrature points per p bin for V_WP (default 8) */
+    int Nphi, Nx;           /* P123 quadrature (default 48 x 48)               */
+    double chebyshev_s;     /* MeV                                              */
+    double chebyshev_t;
+    int tensor_force;
+    int isospin_breaking_1S0;
+    int midpoint_approx;
+    int swave_only;         /* the S-wave filter of CPP/make_pw_symm_states.cpp:176-179 */
+} tt_run_params;
+
+void tt_run_params_default(tt_run_params *rp);   /* CPP/set_run_parameters.cpp:577-615 */
+
+/* ------------------------------------------------------------- state spaces */
+/* Mirrors pw_3N_statespace (CPP/type_defs.h:15-29).  Arrays owned by the library. */
+typedef struct {
+    int Nalpha;
+    int J_2N_max;
+    int *L_2N, *S_2N, *J_2N, *T_2N, *L_1N, *two_J_1N, *two_J_3N, *two_T_3N, *P_3N;
+    int *chn_3N_idx;        /* N_chn_3N + 1 entries */
+    int N_chn_3N;
+} tt_pw_states;
+
+/* Replaces construct_symmetric_pw_states (CPP/make_pw_symm_states.cpp:62-284). */
+tt_status tt_construct_pw_states(const tt_run_params *rp, tt_pw_states *out);
+void tt_pw_states_free(tt_pw_states *pw);
+/* View of one 3N channel (pointers into pw; nothing to free), as CPP/main.cpp:177-192. */
+tt_status tt_pw_channel(const tt_pw_states *pw, int chn, tt_pw_states *view);
+
+/* Mirrors fwp_statespace (CPP/type_defs.h:32-47); momentum WPs. */
+typedef struct {
+    int Np_WP, Nq_WP, Np_per_WP, Nq_per_WP;
+    double *p_WP, *q_WP;        /* bin edges, N+1 each             */
+    double *p, *wp;             /* per-bin quadrature (Np*Np_per_WP) or midpoints */
+    double *q, *wq;
+    double *norm_p, *norm_q;    /* sqrt(bin width)                 */
+} tt_fwp_states;
+
+/* Replaces make_fwp_statespace (CPP/make_wp_states.cpp:116-211), Chebyshev grids. */
+tt_status tt_make_fwp_statespace(const tt_run_params *rp, tt_fwp_states *out);
+void tt_fwp_states_free(tt_fwp_states *f);
+
+/* ------------------------------------------------------ interaction plugin
+ * Replaces the potential_model factory (CPP/Interactions/potential_model.cpp:18-62)
+ * for the models of this path: "malfliet_tjon" and "synthetic".
+ * V(...) follows CPP/Interactions/potential_model.h:20: six slots
+ * [0] S=0, [1] S=1 L=J, [2] L=L'=J-1, [3] (+-), [4] (-+), [5] L=L'=J+1, in MeV^-2. */
+typedef struct tt_potential tt_potential;
+tt_status tt_potential_create(const char *name, tt_potential **out);
+void tt_potential_destroy(tt_potential *pot);
+/* update_parameters (potential_model.h:16).  "synthetic": 12 strengths theta
+ * (3 Yukawa [MeV fm], C_1S0, C_3S1 [fm^2], C_3P0, C_1P1, C_3P1, C_SD [fm^4],
+ * 3 separable [fm^2]) followed by the 3 separable ranges beta [fm^-1]. */
+tt_status tt_potential_update_parameters(tt_potential *pot, const double *params, int n);
+tt_status tt_potential_V(tt_potential *pot, double p_in, double p_out, int coupled,
+                         int S, int J, int T, int Tz, double V[6]);
+
+/* Number of uncoupled / coupled 2N channels (CPP/main.cpp:273-289). */
+void tt_num_2N_states(const tt_run_params *rp, int *num_unco, int *num_coup);
+
+/* Replaces calculate_potential_matrices_array_in_WP_basis (CPP/make_potential_matrix.cpp:51-430).
+ * V_unco: num_unco*Np*Np, V_coup: num_coup*4*Np*Np (row-major), caller-allocated.
+ * pw_full is the full state space (as the reference passes it, CPP/main.cpp:308-313). */
+tt_status tt_calculate_potential_matrices(const tt_run_params *rp, const tt_fwp_states *fwp,
+                                          const tt_pw_states *pw_full, tt_potential *pot,
+                                          double *V_unco, double *V_coup);
+
+/* Mirrors swp_statespace (CPP/type_defs.h:50-61); arrays caller-owned. */
+typedef struct {
+    int Np_WP, Nq_WP;
+    int num_2N_unco_states, num_2N_coup_states;
+    double E_bound;             /* deuteron energy, negative, MeV   */
+    double *e_SWP_unco;         /* num_unco*(Np+1)                  */
+    double *e_SWP_coup;         /* num_coup*2*(Np+1)                */
+    double *C_SWP_unco;         /* num_unco*Np*Np, C[basis][state]  */
+    double *C_SWP_coup;         /* num_coup*4*Np*Np                 */
+    double *q_WP;               /* Nq+1 edges                       */
+} tt_swp_states;
+
+/* Replaces make_swp_states (CPP/make_swp_states.cpp:221-471); fills the caller's
+ * arrays in swp (sizes as above) and E_bound.  TT_ERR_PHYSICS unless exactly one
+ * 3S1 bound state and no other bound state. */
+tt_status tt_make_swp_states(const tt_run_params *rp, const tt_fwp_states *fwp,
+                             const tt_pw_states *pw_full, const double *V_unco,
+                             const double *V_coup, tt_swp_states *swp);
+
+/* Mirrors channel_os_indexing (CPP/type_defs.h:76-85), elastic part. */
+typedef struct {
+    int num_T_lab;              /* number of on-shell q bins        */
+    int *q_com_idx;             /* num_T_lab                        */
+    double *T_lab, *q_com, *E_com;  /* num_T_lab                    */
+    int num_deuteron_states;
+    int *deuteron_idx;          /* indices into the channel's alphas */
+} tt_channel_os;
+
+/* Replaces find_on_shell_bins (CPP/run_organizer.cpp:4-131, elastic) and
+ * find_deuteron_channels (CPP/run_organizer.cpp:342-378) for one channel. */
+tt_status tt_find_on_shell(const tt_swp_states *swp, const tt_pw_states *chn,
+                           const double *T_lab_in, int n_T_lab_in, tt_channel_os *out);
+void tt_channel_os_free(tt_channel_os *os);
+
+/* ---------------------------------------------------------------- P123 CSC
+ * The CSC of CPP/main.cpp:226-244: values (double), row indices (int32, rows
+ * ascending inside a column), column pointers (uint64, dim+1).  Host arrays are
+ * optional; the device mirror is owned by the library. */
+typedef struct {
+    int64_t dim, nnz;
+    double *val;                /* host, may be NULL */
+    int32_t *row;
+    uint64_t *colptr;
+    void *dev;                  /* library-owned device mirror (opaque) */
+} tt_csc;
+
+/* Wraps caller host arrays (copied to the device; caller keeps ownership of its arrays). */
+tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val,
+                        const int32_t *row, const uint64_t *colptr, tt_csc *out);
+/* Copies the device CSC to newly malloc'ed host arrays in m (freed by tt_csc_free). */
+tt_status tt_csc_download(tt_ctx *ctx, tt_csc *m);
+void tt_csc_free(tt_csc *m);
+
+/* Replaces fill_P123_arrays (CPP/make_permutation_matrix.h:79-87,
+ * CPP/make_permutation_matrix.cpp:1297-1403) followed by the column-major sort and
+ * CSC conversion of CPP/main.cpp:226-244.  Built on the GPU; out->val/row/colptr are
+ * left NULL (call tt_csc_download for a host copy).  production_run = 0 gives the
+ * placeholder values cos(row)*cos(col) of CPP/make_permutation_matrix.cpp:817-822. */
+typedef struct {
+    double mask_density;        /* fraction of (q',p',q,p) cells with bin overlap */
+    int64_t n_cells;            /* overlapping cells                               */
+    int n_tuples;               /* distinct angular functions                      */
+    double ms_total;            /* wall time of the build                          */
+} tt_p123_stats;
+tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_states *fwp,
+                              const tt_pw_states *chn, int production_run, tt_csc *out,
+                              tt_p123_stats *stats);
+
+/* ------------------------------------------------------------- resolvent G
+ * Replaces calculate_resolvent_array_in_SWP_basis (CPP/make_resolvent.h:36-40),
+ * batched over energies: G[j*dim + idx] at total energy E[j] (= E_com + E_bound,
+ * CPP/main.cpp:391).  G is host memory (interleaved complex, nE*dim). */
+tt_status tt_calculate_resolvent(tt_ctx *ctx, const tt_run_params *rp, const tt_swp_states *swp,
+                                 const tt_pw_states *chn, const double *E, int nE, double *G);
+
+/* ------------------------------------------------------------ Faddeev solve */
+typedef enum {
+    TT_SOLVE_PADE = 0,          /* reference semantics: Neumann terms + [N/N] Pade (default) */
+    TT_SOLVE_DENSE = 1,         /* solve_dense=true: (1 - A G) U = A, dense LU (small dims)   */
+    TT_SOLVE_GMRES = 2,         /* converged FP64 GMRES on y (1 - G A) = e_r^T A              */
+    TT_SOLVE_MIXED = 3          /* FP32 products + FP64 iterative refinement                 */
+} tt_solve_method;
+
+typedef struct {
+    int method;                 /* tt_solve_method                         */
+    double tol;                 /* relative residual target (GMRES/MIXED)   */
+    int max_iter;               /* total Krylov iterations per system       */
+    int restart;                /* GMRES restart length                     */
+    int NM_max;                 /* Pade order limit (reference: 14)         */
+} tt_solve_opts;
+void tt_solve_opts_default(tt_solve_opts *o);
+
+typedef struct {
+    int n_apply;                /* A applications (FP64 + FP32)             */
+    int n_apply_fp32;
+    int iterations;             /* Krylov iterations (max over systems) or Neumann terms */
+    double residual;            /* final relative residual (max over systems) */
+    double ms_total;
+    double ms_apply;            /* device time inside A applications        */
+} tt_solve_stats;
+
+/* Replaces solve_faddeev_equations (CPP/solve_faddeev.h:25-38).
+ * U: host, interleaved complex, U[d_row*nd*nE + d_col*nE + j] (CPP/solve_faddeev.cpp:424).
+ * G: host, nE*dim complex (from tt_calculate_resolvent), nE = os->num_T_lab.
+ * V_unco/V_coup: as filled by tt_calculate_potential_matrices. */
+tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, double *U,
+                                     const double *G, const tt_csc *P123,
+                                     const double *V_unco, const double *V_coup,
+                                     const tt_swp_states *swp, const tt_channel_os *os,
+                                     const tt_pw_states *chn, const tt_solve_opts *opts,
+                                     tt_solve_stats *stats);
+
+/* Raw kernel-matrix application for tests and profiling: Y = (X .* G_sel) A in
+ * the reference's row form (CPP/solve_faddeev.cpp:956-1088), r rows.
+ * X, Y: host, r*dim complex (row-major).  g_index[k] selects the energy of row k
+ * (-1: no G factor).  G: host nE*dim complex. */
+tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123,
+                        const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                        const tt_pw_states *chn, const double *G, int nE, const int *g_index,
+                        int r, const double *X, double *Y, int fp32);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TICTAC_H */

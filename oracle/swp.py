@@ -1,0 +1,278 @@
+"""Oracle: V in the WP basis, SWP states, on-shell bins (test infrastructure only).
+
+Restates:
+  CPP/make_potential_matrix.cpp:4-430   V_WP blocks (uncoupled Np x Np, coupled 2Np x 2Np),
+                                        isoscalar nn/np mixing, isospin-breaking 1S0 block
+  CPP/make_swp_states.cpp:28-183,221-471 H = H0 + V, eigenvectors C, SWP energy-bin edges
+  CPP/run_organizer.cpp:4-131,342-378   on-shell q-bins and deuteron channels
+
+The reference diagonalises with LAPACKE_dspevd; here numpy.linalg.eigh.  The
+eigenvector sign is arbitrary in both, and every on-shell U element is
+invariant under it (C enters as C ... C^T with the diagonal G in between).
+
+Pins (tests/test_oracle_swp.py): C^T C = 1 and C^T H C diagonal; MT-III deuteron
+near -2.23 MeV at large N (CPP/Interactions/malfliet_tjon.cpp:11); V_WP of the
+MT potential against scipy.integrate.dblquad over one bin pair.
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import states as st
+
+
+def extract_potential_element(L, Lp, J, S, coupled, v):
+    """CPP/make_potential_matrix.cpp:4-48."""
+    if J < abs(S - L) or J > S + L or J < abs(S - Lp) or J > S + Lp:
+        raise ValueError("unphysical state")
+    sgn = -1.0
+    if coupled:
+        if L == Lp and L < J:
+            return v[2]
+        if L == Lp and L > J:
+            return v[5]
+        if L < Lp:
+            return sgn * v[4]
+        return sgn * v[3]
+    if L == J + 1:
+        return v[5]
+    if L == J - 1:
+        return v[2]
+    if S == 0:
+        return v[0]
+    if S == 1 and L == J:
+        return v[1]
+    return 0.0
+
+
+def potential_matrices(fwp, pw_full, pot, tensor_force, isospin_breaking_1S0, J_2N_max):
+    """CPP/make_potential_matrix.cpp:51-430 (quadrature branch, midpoint_approx=false).
+
+    Returns (V_unco[nu, Np, Np], V_coup[nc, 2Np, 2Np]); blocks never touched stay NaN.
+    """
+    nu, nc = st.num_2N_states(J_2N_max, tensor_force, isospin_breaking_1S0)
+    Np, npw = fwp.Np, fwp.Np_per_WP
+    Vu = np.full((nu, Np, Np), np.nan)
+    Vc = np.full((nc, 2 * Np, 2 * Np), np.nan)
+    done_u, done_c = set(), set()
+    pts, wts = fwp.p_pts, fwp.p_wts
+    for ar in range(pw_full.Nalpha):
+        Lr, Sr, Jr, Tr, t3r = (int(pw_full.L[ar]), int(pw_full.S[ar]), int(pw_full.J[ar]),
+                               int(pw_full.T[ar]), int(pw_full.two_T3[ar]))
+        for ac in range(pw_full.Nalpha):
+            Lc, Sc, Jc, Tc, t3c = (int(pw_full.L[ac]), int(pw_full.S[ac]), int(pw_full.J[ac]),
+                                   int(pw_full.T[ac]), int(pw_full.two_T3[ac]))
+            checkL = (tensor_force and abs(Lr - Lc) <= 2) or Lr == Lc
+            if not (Tr == Tc and Jr == Jc and Sr == Sc and checkL):
+                continue
+            s1S0 = (Sr == 0 and Jr == 0 and Lr == 0)
+            # isoscalar type, lines 138-158 (operator precedence of line 147 kept)
+            if s1S0 and t3r == 3 and t3c == 3:
+                iso = 2
+            elif (s1S0 and (t3r == 3 and t3c == 1)) or (t3r == 1 and t3c == 3):
+                iso = 3
+            elif Tr == 0 and t3r == t3c and t3r == 1:
+                iso = 0
+            elif Tr == 1 and t3r == t3c and t3r == 1:
+                iso = 1
+            else:
+                raise ValueError("unknown isoscalar type")
+            via_L, via_T = st.coupling_kind(Lr, Sr, Jr, Tr, Lc, tensor_force, isospin_breaking_1S0)
+            coupled_matrix = via_L or via_T
+            coupled_model = (Lr != Lc) or (Lr == Lc and Lr != Jr)
+            idx = st.unique_2N_idx(Lr, Sr, Jr, Tr, coupled_matrix, tensor_force, isospin_breaking_1S0)
+            if coupled_matrix:
+                if idx in done_c:
+                    continue
+                done_c.add(idx)
+            else:
+                if idx in done_u:
+                    continue
+                done_u.add(idx)
+            for br in range(Np):
+                Nr = fwp.norm_p[br]
+                for bc in range(Np):
+                    Nc = fwp.norm_p[bc]
+                    acc = np.zeros(6)
+                    for ir in range(npw):
+                        pr = pts[br * npw + ir]
+                        fr = wts[br * npw + ir] * pr
+                        for ic in range(npw):
+                            pc = pts[bc * npw + ic]
+                            fc = wts[bc * npw + ic] * pc
+                            fac = fr * fc
+                            if not via_T and iso == 0:
+                                vis = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0))
+                            elif not via_T and iso == 1:
+                                vnn = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, -1))
+                                vnp = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0))
+                                vis = (2. / 3) * vnn + (1. / 3) * vnp
+                            elif via_T:
+                                vnn = pot.V(pc, pr, coupled_model, Sr, Jr, Tr, -1)
+                                vnp = pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0)
+                                enn = extract_potential_element(Lr, Lc, Jr, Sr, False, vnn)
+                                enp = extract_potential_element(Lr, Lc, Jr, Sr, False, vnp)
+                                vis = np.zeros(6)
+                                vis[2] = (2. / 3) * enn + (1. / 3) * enp
+                                vis[3] = (math.sqrt(2) / 3.) * (enn - enp)
+                                vis[4] = (math.sqrt(2) / 3.) * (enn - enp)
+                                vis[5] = (1. / 3) * enn + (2. / 3) * enp
+                            else:
+                                raise ValueError("unknown isoscalar potential")
+                            acc += fac * vis / (Nr * Nc)
+                    if coupled_matrix:
+                        if via_L:
+                            Vc[idx, br, bc] = extract_potential_element(Jr - 1, Jr - 1, Jr, Sr, True, acc)
+                            Vc[idx, br, bc + Np] = extract_potential_element(Jr - 1, Jr + 1, Jr, Sr, True, acc)
+                            Vc[idx, br + Np, bc] = extract_potential_element(Jr + 1, Jr - 1, Jr, Sr, True, acc)
+                            Vc[idx, br + Np, bc + Np] = extract_potential_element(Jr + 1, Jr + 1, Jr, Sr, True, acc)
+                        else:
+                            Vc[idx, br, bc] = acc[2]
+                            Vc[idx, br, bc + Np] = acc[3]
+                            Vc[idx, br + Np, bc] = acc[4]
+                            Vc[idx, br + Np, bc + Np] = acc[5]
+                    else:
+                        Vu[idx, br, bc] = extract_potential_element(Lr, Lc, Jr, Sr, False, acc)
+    return Vu, Vc
+
+
+@dataclass
+class SWP:
+    Np: int
+    Nq: int
+    nu: int
+    nc: int
+    E_bound: float
+    e_unco: np.ndarray     # (nu, Np+1)
+    e_coup: np.ndarray     # (nc, 2(Np+1))
+    C_unco: np.ndarray     # (nu, Np, Np)    C[basis, eigenstate]
+    C_coup: np.ndarray     # (nc, 2Np, 2Np)
+    q_edges: np.ndarray
+
+
+def _swp_edges(ev, Np, coupled, chn_3S1):
+    """CPP/make_swp_states.cpp:146-183."""
+    e = np.zeros(2 * (Np + 1) if coupled else Np + 1)
+    for i in range(Np - 1):
+        e[i + 1] = 0.5 * (ev[i + 1] + ev[i])
+        if coupled:
+            e[i + 1 + Np + 1] = 0.5 * (ev[i + 1 + Np] + ev[i + Np])
+    e[Np] = ev[Np - 1] + 0.5 * (e[Np - 1] - e[Np - 2])
+    if coupled:
+        e[2 * Np + 1] = ev[2 * Np - 1] + 0.5 * (e[2 * Np] - e[2 * Np - 1])
+    if chn_3S1:
+        e[0] = ev[0]
+        e[1] = 0.0
+    return e
+
+
+def make_swp_states(fwp, pw_full, Vu, Vc, tensor_force, isospin_breaking_1S0):
+    """CPP/make_swp_states.cpp:221-471; raises unless exactly one 3S1 bound state."""
+    Np = fwp.Np
+    pe = fwp.p_edges
+    H0 = (pe[1:] ** 2 + pe[1:] * pe[:-1] + pe[:-1] ** 2) / (6 * st.mu23)
+    nu, nc = Vu.shape[0], Vc.shape[0]
+    Cu = np.full_like(Vu, np.nan)
+    Cc = np.full_like(Vc, np.nan)
+    eu = np.zeros((nu, Np + 1))
+    ec = np.zeros((nc, 2 * (Np + 1)))
+    done_u, done_c = set(), set()
+    E_bound = 0.0
+    for ar in range(pw_full.Nalpha):
+        Lr, Sr, Jr, Tr = (int(pw_full.L[ar]), int(pw_full.S[ar]), int(pw_full.J[ar]), int(pw_full.T[ar]))
+        for ac in range(pw_full.Nalpha):
+            Lc, Sc, Jc, Tc = (int(pw_full.L[ac]), int(pw_full.S[ac]), int(pw_full.J[ac]), int(pw_full.T[ac]))
+            if not (Tr == Tc and Jr == Jc and Sr == Sc and abs(Lr - Lc) <= 2):
+                continue
+            via_L, via_T = st.coupling_kind(Lr, Sr, Jr, Tr, Lc, tensor_force, isospin_breaking_1S0)
+            coupled = via_L or via_T
+            idx = st.unique_2N_idx(Lr, Sr, Jr, Tr, coupled, tensor_force, isospin_breaking_1S0)
+            if coupled:
+                if idx in done_c:
+                    continue
+                done_c.add(idx)
+                V = Vc[idx]
+                h0 = np.concatenate([H0, H0])
+            else:
+                if idx in done_u:
+                    continue
+                done_u.add(idx)
+                V = Vu[idx]
+                h0 = H0
+            # H from the upper triangle of V (construct_full_hamiltonian, lines 45-81)
+            Vs = np.triu(V) + np.triu(V, 1).T
+            H = Vs + np.diag(h0)
+            ev, C = np.linalg.eigh(H)
+            chn_3S1 = (Lr == 0 and Sr == 1 and Jr == 1 and Tr == 0)
+            nb = int(np.sum(ev < 0))
+            if nb == 1:
+                E_bound = float(ev[ev < 0][0])
+            if (nb == 0 and not chn_3S1) or (nb == 1 and chn_3S1):
+                pass
+            elif nb != 0:
+                raise ValueError("unphysical bound states in NN Hamiltonian (L=%d S=%d J=%d T=%d)" % (Lr, Sr, Jr, Tr))
+            else:
+                raise ValueError("no bound state in the 3S1 Hamiltonian")
+            if coupled:
+                # reorder_coupled_eigenspectrum, lines 85-112
+                order = np.concatenate([np.arange(0, 2 * Np, 2), np.arange(1, 2 * Np, 2)])
+                ev = ev[order]
+                C = C[:, order]
+                Cc[idx] = C
+                ec[idx] = _swp_edges(ev, Np, True, chn_3S1)
+            else:
+                Cu[idx] = C
+                eu[idx] = _swp_edges(ev, Np, False, chn_3S1)
+    return SWP(Np, fwp.Nq, nu, nc, E_bound, eu, ec, Cu, Cc, fwp.q_edges.copy())
+
+
+@dataclass
+class OnShell:
+    T_lab: np.ndarray
+    E_com: np.ndarray
+    q_com: np.ndarray
+    q_idx: np.ndarray
+
+
+def find_on_shell_bins(q_edges, E_bound, T_lab_input):
+    """CPP/run_organizer.cpp:4-131 (elastic part)."""
+    Nq = len(q_edges) - 1
+    m1 = st.mu1(E_bound)
+    Tmid = np.zeros(Nq)
+    for k in range(Nq):
+        Eql = 0.5 * q_edges[k] ** 2 / m1
+        Equ = 0.5 * q_edges[k + 1] ** 2 / m1
+        Ec = 0.5 * (Equ + Eql)
+        Tmid[k] = st.com_momentum_to_lab_energy(st.com_energy_to_com_q_momentum(Ec), E_bound)
+    mid = [False] * (Nq - 1)
+    for k in range(Nq - 1):
+        for T in T_lab_input:
+            if Tmid[k] <= T <= Tmid[k + 1]:
+                mid[k] = True
+    onbin = [False] * Nq
+    for k in range(Nq - 1):
+        if mid[k]:
+            onbin[k] = onbin[k + 1] = True
+    bins = [k for k in range(Nq) if onbin[k]]
+    T_lab = np.array([Tmid[k] for k in bins])
+    q_com = np.array([st.lab_energy_to_com_momentum(T, E_bound) for T in T_lab])
+    E_com = np.array([st.com_q_momentum_to_com_energy(q) for q in q_com])
+    q_idx = []
+    for q in q_com:
+        found = -1
+        for k in range(Nq):
+            if q_edges[k] < q < q_edges[k + 1]:
+                found = k
+                break
+        if found < 0:
+            raise ValueError("on-shell T_lab outside the WP state space")
+        q_idx.append(found)
+    return OnShell(T_lab, E_com, q_com, np.array(q_idx, dtype=np.int64))
+
+
+def find_deuteron_channels(pw_chn):
+    """CPP/run_organizer.cpp:342-378 for one 3N channel."""
+    return np.array([a for a in range(pw_chn.Nalpha)
+                     if pw_chn.L[a] == st.deuteron_L and pw_chn.S[a] == st.deuteron_S
+                     and pw_chn.J[a] == st.deuteron_J and pw_chn.T[a] == st.deuteron_T], dtype=np.int64)

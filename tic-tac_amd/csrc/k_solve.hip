@@ -1,0 +1,1115 @@
+// gfx950 kernels and drivers of the solve path:
+//   resolvent G                 CPP/make_resolvent.cpp:4-211
+//   A = C^T (2 P123) V C        CPP/solve_faddeev.cpp:4-269 (applied matrix-free, row form)
+//   Neumann rows + Pade         CPP/solve_faddeev.cpp:271-296,440-1444
+//   dense solve                 CPP/solve_faddeev.cpp:305-438
+//   converged GMRES / mixed     no reference counterpart (the reference's Pade stops at ~1e-6)
+//
+// Vectors of r rows are stored RHS-fastest: X[idx][k], idx = alpha*Nq*Np + q*Np + p,
+// complex interleaved.  The reference's rows x A (cblas_dgemm, lines 1072-1088) becomes
+//   S = C_op (X .* G)      block op over the 2N-coupled alphas, q-diagonal
+//   U = 2 P123^T S         gather SpMM over the reference's own CSC arrays (CSC of P = CSR of P^T)
+//   Y = VC_op^T U          block op
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "tt_internal.h"
+
+namespace tt {
+
+// ------------------------------------------------------------------ device memory
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) TT_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    void upload(const T *h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        if (count) TT_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+};
+
+DeviceCSC::~DeviceCSC() {
+    if (val) (void)hipFree(val);
+    if (val32) (void)hipFree(val32);
+    if (row) (void)hipFree(row);
+    if (colptr) (void)hipFree(colptr);
+}
+
+// ------------------------------------------------------------------ resolvent
+__device__ inline double heaviside(double v) { return v < 0 ? 0.0 : 1.0; }
+
+// One thread per (energy j, alpha, q, p); formulas of CPP/make_resolvent.cpp:14-80.
+__global__ void k_resolvent(double2 *__restrict__ G, const double *__restrict__ e_unco,
+                            const double *__restrict__ e_coup, const int *__restrict__ which,
+                            const int *__restrict__ offset, const double *__restrict__ qe,
+                            const double *__restrict__ E, int Na, int Nq, int Np, int nE) {
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * nE) return;
+    const int j = (int)(t / dim);
+    const int64_t idx = t - (int64_t)j * dim;
+    const int a = (int)(idx / (Nq * Np));
+    const int q = (int)((idx / Np) % Nq);
+    const int p = (int)(idx % Np);
+    const double *e = (which[a] ? e_coup : e_unco) + offset[a];
+    const double el = e[p], eu = e[p + 1];
+    const double Ej = E[j];
+    const double ql = qe[q], qu = qe[q + 1];
+    double re, im;
+    if (el < 0) {
+        const double Eb = el;
+        const double m1 = 939.565 * (939.565 + 938.272 + Eb) / (939.565 + 939.565 + 938.272 + Eb);
+        const double Eql = ql * ql / (2 * m1), Equ = qu * qu / (2 * m1);
+        const double Dq = Equ - Eql;
+        re = log(fabs((Eql + Eb - Ej) / (Equ + Eb - Ej))) / Dq;
+        im = (heaviside(Equ + Eb - Ej) - heaviside(Eql + Eb - Ej)) * (-M_PI) / Dq;
+    } else {
+        const double m1 = 939.565 * (939.565 + 938.272) / (939.565 + 939.565 + 938.272);
+        const double Eql = ql * ql / (2 * m1), Equ = qu * qu / (2 * m1);
+        const double Ep = 0.5 * (el + eu), Eq = 0.5 * (Eql + Equ);
+        const double Dq = Equ - Eql, Dp = eu - el;
+        const double DM = 0.5 * (Dp - Dq), DP = 0.5 * (Dp + Dq);
+        const double D = Ep + Eq - Ej;
+        re = ((D + DM) * log(fabs(D + DM)) + (D - DM) * log(fabs(D - DM)) - (D + DP) * log(fabs(D + DP)) -
+              (D - DP) * log(fabs(D - DP))) /
+             (Dp * Dq);
+        im = ((D + DM) * heaviside(D + DM) + (D - DM) * heaviside(D - DM) - (D + DP) * heaviside(D + DP) -
+              (D - DP) * heaviside(D - DP)) *
+             M_PI / (Dp * Dq);
+    }
+    G[t] = make_double2(re, im);
+}
+
+// ------------------------------------------------------------------ block ops
+// Y[(a,q,p)][k] = sum_{e in part(a)} sum_p' B_e[p][p'] * Xg[(alpha_e,q,p')][k]
+// with Xg = X .* G_{gidx[k]} when G != nullptr.  One thread per output element.
+template <typename TI, typename TO>
+__global__ void k_block(const TI *__restrict__ X, TO *__restrict__ Y, const int *__restrict__ part_ptr,
+                        const int *__restrict__ part_alpha, const double *__restrict__ blocks,
+                        const double2 *__restrict__ G, const int *__restrict__ gidx, int Na, int Nq, int Np,
+                        int r) {
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * r) return;
+    const int k = (int)(t % r);
+    const int64_t idx = t / r;
+    const int a = (int)(idx / (Nq * Np));
+    const int q = (int)((idx / Np) % Nq);
+    const int p = (int)(idx % Np);
+    const int g = G ? gidx[k] : -1;
+    double sr = 0.0, si = 0.0;
+    for (int e = part_ptr[a]; e < part_ptr[a + 1]; ++e) {
+        const int ap = part_alpha[e];
+        const double *B = blocks + ((size_t)e * Np + p) * Np;
+        const int64_t base = ((int64_t)ap * Nq + q) * Np;
+        for (int pp = 0; pp < Np; ++pp) {
+            const int64_t src = base + pp;
+            double xr = (double)X[src * r + k].x, xi = (double)X[src * r + k].y;
+            if (g >= 0) {
+                const double2 gv = G[(int64_t)g * dim + src];
+                const double tr = xr * gv.x - xi * gv.y;
+                xi = xr * gv.y + xi * gv.x;
+                xr = tr;
+            }
+            sr += B[pp] * xr;
+            si += B[pp] * xi;
+        }
+    }
+    Y[t].x = sr;
+    Y[t].y = si;
+}
+
+// ------------------------------------------------------------------ gather SpMM
+// U[c][v] = scale * sum_{nz in column c} val[nz] * S[row[nz]][v], v < R2 (real lanes).
+// One wavefront per column.  The wave loads 64 consecutive (val, row) pairs with one
+// coalesced access, then the lanes (slot, v) walk them through cross-lane shuffles.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmm_gather(const uint64_t *__restrict__ colptr,
+                                                     const int32_t *__restrict__ row, const T *__restrict__ val,
+                                                     const T *__restrict__ S, T *__restrict__ U, int64_t dim,
+                                                     int R2, int R2p, T scale) {
+    const int lane = threadIdx.x & 63;
+    const int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (col >= dim) return;  // uniform per wavefront
+    const uint64_t beg = colptr[col], end = colptr[col + 1];
+    if (R2p <= 64) {
+        const int slots = 64 / R2p;
+        const int slot = lane / R2p, v = lane % R2p;
+        const bool active = v < R2;
+        T acc = 0;
+        for (uint64_t base = beg; base < end; base += 64) {
+            const int cnt = (int)min<uint64_t>(64, end - base);
+            T myval = 0;
+            int myrow = 0;
+            if (lane < cnt) {
+                myval = val[base + lane];
+                myrow = row[base + lane];
+            }
+            for (int j0 = 0; j0 < cnt; j0 += slots) {
+                const int src = j0 + slot;
+                const T pv = __shfl(myval, src < 64 ? src : 0);
+                const int pr = __shfl(myrow, src < 64 ? src : 0);
+                if (active && src < cnt) acc += pv * S[(int64_t)pr * R2 + v];
+            }
+        }
+        for (int off = R2p; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
+        if (slot == 0 && active) U[col * R2 + v] = scale * acc;
+    } else {
+        for (int v0 = 0; v0 < R2; v0 += 64) {
+            const int v = v0 + lane;
+            T acc = 0;
+            for (uint64_t base = beg; base < end; base += 64) {
+                const int cnt = (int)min<uint64_t>(64, end - base);
+                T myval = 0;
+                int myrow = 0;
+                if (lane < cnt) {
+                    myval = val[base + lane];
+                    myrow = row[base + lane];
+                }
+                for (int j = 0; j < cnt; ++j) {
+                    const T pv = __shfl(myval, j);
+                    const int pr = __shfl(myrow, j);
+                    if (v < R2) acc += pv * S[(int64_t)pr * R2 + v];
+                }
+            }
+            if (v < R2) U[col * R2 + v] = scale * acc;
+        }
+    }
+}
+
+__global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (float)a[t];
+}
+
+// ------------------------------------------------------------------ Krylov vector ops
+// partial[(i*nb + b)*r + k] = sum over block b's slice of conj(V_i[e]) * W[e] for column k.
+__global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, int nvec, const double2 *__restrict__ W,
+                       double2 *__restrict__ partial, int64_t n_elem, int r) {
+    extern __shared__ double2 sh[];
+    const int i = blockIdx.y;
+    const int b = blockIdx.x, nb = gridDim.x;
+    const double2 *Vi = V + (int64_t)i * vstride;
+    const int tpr = blockDim.x / r;  // threads per column k
+    const int k = threadIdx.x % r;
+    const int sub = threadIdx.x / r;
+    double sr = 0.0, si = 0.0;
+    if (sub < tpr) {
+        const int64_t rows = n_elem / r;
+        for (int64_t e = (int64_t)b * tpr + sub; e < rows; e += (int64_t)nb * tpr) {
+            const double2 v = Vi[e * r + k], w = W[e * r + k];
+            sr += v.x * w.x + v.y * w.y;
+            si += v.x * w.y - v.y * w.x;
+        }
+    }
+    sh[threadIdx.x] = make_double2(sr, si);
+    __syncthreads();
+    if (threadIdx.x < r) {
+        double ar = 0.0, ai = 0.0;
+        for (int s = 0; s < tpr; ++s) {
+            ar += sh[s * r + threadIdx.x].x;
+            ai += sh[s * r + threadIdx.x].y;
+        }
+        partial[((int64_t)i * nb + b) * r + threadIdx.x] = make_double2(ar, ai);
+    }
+    (void)nvec;
+}
+
+// W[e] -= sum_i h[i][k] V_i[e]
+__global__ void k_sub_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
+                            const double2 *__restrict__ h, int64_t n_elem, int r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
+    const int k = (int)(t % r);
+    double2 w = W[t];
+    for (int i = 0; i < nvec; ++i) {
+        const double2 c = h[i * r + k], v = V[(int64_t)i * vstride + t];
+        w.x -= c.x * v.x - c.y * v.y;
+        w.y -= c.x * v.y + c.y * v.x;
+    }
+    W[t] = w;
+}
+
+// X[e] += sum_i z[i][k] V_i[e]
+__global__ void k_add_combo(double2 *__restrict__ X, const double2 *__restrict__ V, int64_t vstride, int nvec,
+                            const double2 *__restrict__ z, int64_t n_elem, int r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
+    const int k = (int)(t % r);
+    double2 x = X[t];
+    for (int i = 0; i < nvec; ++i) {
+        const double2 c = z[i * r + k], v = V[(int64_t)i * vstride + t];
+        x.x += c.x * v.x - c.y * v.y;
+        x.y += c.x * v.y + c.y * v.x;
+    }
+    X[t] = x;
+}
+
+// Y[e] = alpha_k * X[e] (complex per column), or Y = X - Z when Z != nullptr and alpha == nullptr
+__global__ void k_scale(double2 *__restrict__ Y, const double2 *__restrict__ X, const double2 *__restrict__ alpha,
+                        int64_t n_elem, int r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
+    const double2 a = alpha[t % r], x = X[t];
+    Y[t] = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
+}
+
+__global__ void k_axpby(double2 *__restrict__ Y, const double2 *__restrict__ A, const double2 *__restrict__ B,
+                        double alpha, double beta, int64_t n_elem) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
+    const double2 a = A[t], b = B[t];
+    Y[t] = make_double2(alpha * a.x + beta * b.x, alpha * a.y + beta * b.y);
+}
+
+// out[m] = X[src_idx[m]][src_k[m]]
+__global__ void k_gather(const double2 *__restrict__ X, const int64_t *__restrict__ src_idx,
+                         const int *__restrict__ src_k, double2 *__restrict__ out, int n, int r) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = X[src_idx[t] * r + src_k[t]];
+}
+
+// X[dst_idx[m]][m] = 1 (unit rows), rest zero beforehand
+__global__ void k_unit_rows(double2 *__restrict__ X, const int64_t *__restrict__ dst_idx, int r) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < r) X[dst_idx[t] * r + t] = make_double2(1.0, 0.0);
+}
+
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------ the operator
+// Matrix-free row-form application Y = (X .* G_sel) A for r rows at once.
+struct RowOperator {
+    hipStream_t s;
+    int Na, Nq, Np, nE;
+    int64_t dim;
+    const DeviceCSC *P;
+    DevBuf<int> c_ptr, c_alpha, vc_ptr, vc_alpha;
+    DevBuf<double> c_blocks, vc_blocks;
+    DevBuf<double2> G;
+    DevBuf<double2> S, U;      // FP64 scratch (dim * r)
+    DevBuf<float2> S32, U32;   // FP32 scratch
+    DevBuf<int> gidx;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double ms_apply = 0.0;
+    int n_apply = 0, n_apply32 = 0;
+    int r_cap = 0;
+
+    ~RowOperator() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+
+    static void block_list(const std::vector<double> &B, int Na, int Np, bool transpose, std::vector<int> &ptr,
+                           std::vector<int> &alpha, std::vector<double> &blocks) {
+        const int64_t n = (int64_t)Na * Np;
+        ptr.assign(Na + 1, 0);
+        alpha.clear();
+        blocks.clear();
+        for (int a = 0; a < Na; ++a) {
+            for (int ap = 0; ap < Na; ++ap) {
+                bool nz = false;
+                for (int p = 0; p < Np && !nz; ++p)
+                    for (int pp = 0; pp < Np && !nz; ++pp) {
+                        double v = transpose ? B[(size_t)(ap * Np + pp) * n + a * Np + p]
+                                             : B[(size_t)(a * Np + p) * n + ap * Np + pp];
+                        nz = v != 0.0;
+                    }
+                if (!nz) continue;
+                alpha.push_back(ap);
+                for (int p = 0; p < Np; ++p)
+                    for (int pp = 0; pp < Np; ++pp)
+                        blocks.push_back(transpose ? B[(size_t)(ap * Np + pp) * n + a * Np + p]
+                                                   : B[(size_t)(a * Np + p) * n + ap * Np + pp]);
+            }
+            ptr[a + 1] = (int)alpha.size();
+        }
+    }
+
+    void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
+               const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, const double *Gh, int nE_) {
+        s = stream;
+        Na = chn.Nalpha;
+        Nq = swp.Nq_WP;
+        Np = swp.Np_WP;
+        nE = nE_;
+        dim = (int64_t)Na * Nq * Np;
+        P = csc;
+        TT_REQUIRE(csc && csc->dim == dim, "P123 dimension does not match Nalpha*Nq*Np");
+        int nu, nc;
+        num_2N_states(rp, nu, nc);
+        // V.C per 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array, lines 1828-1855)
+        std::vector<double> VCu((size_t)nu * Np * Np, 0.0), VCc((size_t)nc * 4 * Np * Np, 0.0);
+        auto mm = [](const double *A, const double *B, double *C, int n) {
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    double acc = 0.0;
+                    for (int k = 0; k < n; ++k) acc += A[(size_t)i * n + k] * B[(size_t)k * n + j];
+                    C[(size_t)i * n + j] = acc;
+                }
+        };
+        for (int c = 0; c < nu; ++c)
+            mm(V_unco + (size_t)c * Np * Np, swp.C_SWP_unco + (size_t)c * Np * Np, &VCu[(size_t)c * Np * Np], Np);
+        for (int c = 0; c < nc; ++c)
+            mm(V_coup + (size_t)c * 4 * Np * Np, swp.C_SWP_coup + (size_t)c * 4 * Np * Np,
+               &VCc[(size_t)c * 4 * Np * Np], 2 * Np);
+        std::vector<double> Cop = embed_2N(chn, Np, swp.C_SWP_unco, swp.C_SWP_coup, rp);
+        std::vector<double> VCop = embed_2N(chn, Np, VCu.data(), VCc.data(), rp);
+        std::vector<int> p1, a1, p2, a2;
+        std::vector<double> b1, b2;
+        block_list(Cop, Na, Np, false, p1, a1, b1);   // S = C_op (X G)
+        block_list(VCop, Na, Np, true, p2, a2, b2);   // Y = VC_op^T U
+        c_ptr.upload(p1.data(), p1.size(), s);
+        c_alpha.upload(a1.data(), std::max<size_t>(a1.size(), 1), s);
+        c_blocks.upload(b1.data(), std::max<size_t>(b1.size(), 1), s);
+        vc_ptr.upload(p2.data(), p2.size(), s);
+        vc_alpha.upload(a2.data(), std::max<size_t>(a2.size(), 1), s);
+        vc_blocks.upload(b2.data(), std::max<size_t>(b2.size(), 1), s);
+        if (nE > 0 && Gh) G.upload(reinterpret_cast<const double2 *>(Gh), (size_t)nE * dim, s);
+        if (!ev0) {
+            TT_HIP(hipEventCreate(&ev0));
+            TT_HIP(hipEventCreate(&ev1));
+        }
+    }
+
+    void reserve(int r, bool fp32) {
+        if (r > r_cap) {
+            S.alloc((size_t)dim * r);
+            U.alloc((size_t)dim * r);
+            gidx.alloc(r);
+            r_cap = r;
+            S32.release();
+            U32.release();
+        }
+        if (fp32 && !S32.p) {
+            S32.alloc((size_t)dim * r_cap);
+            U32.alloc((size_t)dim * r_cap);
+        }
+    }
+
+    void set_gidx(const std::vector<int> &g) {
+        reserve((int)g.size(), false);
+        gidx.upload(g.data(), g.size(), s);
+    }
+
+    void ensure_f32() {
+        DeviceCSC *m = const_cast<DeviceCSC *>(P);
+        if (!m->val32) {
+            TT_HIP(hipMalloc(&m->val32, sizeof(float) * std::max<int64_t>(m->nnz, 1)));
+            if (m->nnz) hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz, 256)), dim3(256), 0, s, m->val, m->val32, m->nnz);
+        }
+    }
+
+    // Y = (X .* G_{gidx}) A ; X, Y device, dim*r complex; use_G selects whether G multiplies X.
+    void apply(const double2 *X, double2 *Y, int r, bool use_G, bool fp32, bool timed = true) {
+        reserve(r, fp32);
+        if (timed) TT_HIP(hipEventRecord(ev0, s));
+        const int64_t ne = dim * r;
+        int R2 = 2 * r, R2p = 1;
+        while (R2p < R2) R2p <<= 1;
+        const unsigned spmm_grid = nblk(dim, 4);
+        if (!fp32) {
+            hipLaunchKernelGGL((k_block<double2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, X, S.p, c_ptr.p,
+                               c_alpha.p, c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r);
+            hipLaunchKernelGGL((k_spmm_gather<double>), dim3(spmm_grid), dim3(256), 0, s, P->colptr, P->row, P->val,
+                               reinterpret_cast<const double *>(S.p), reinterpret_cast<double *>(U.p), dim, R2, R2p,
+                               2.0);
+            hipLaunchKernelGGL((k_block<double2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, U.p, Y, vc_ptr.p,
+                               vc_alpha.p, vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r);
+            ++n_apply;
+        } else {
+            ensure_f32();
+            hipLaunchKernelGGL((k_block<double2, float2>), dim3(nblk(ne, 256)), dim3(256), 0, s, X, S32.p, c_ptr.p,
+                               c_alpha.p, c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r);
+            hipLaunchKernelGGL((k_spmm_gather<float>), dim3(spmm_grid), dim3(256), 0, s, P->colptr, P->row, P->val32,
+                               reinterpret_cast<const float *>(S32.p), reinterpret_cast<float *>(U32.p), dim, R2, R2p,
+                               2.0f);
+            hipLaunchKernelGGL((k_block<float2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, U32.p, Y, vc_ptr.p,
+                               vc_alpha.p, vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r);
+            ++n_apply;
+            ++n_apply32;
+        }
+        if (timed) {
+            TT_HIP(hipEventRecord(ev1, s));
+            TT_HIP(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            TT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            ms_apply += ms;
+        }
+    }
+};
+
+// ------------------------------------------------------------------ Pade (host)
+static cdouble det_lu(std::vector<cdouble> A, int n) {
+    // LU with partial pivoting; det = prod(diag) * (-1)^swaps (matrix_routines.cpp:82-101)
+    cdouble det = 1.0;
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double best = std::abs(A[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(A[(size_t)i * n + k]) > best) {
+                best = std::abs(A[(size_t)i * n + k]);
+                piv = i;
+            }
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)piv * n + j]);
+            det = -det;
+        }
+        cdouble d = A[(size_t)k * n + k];
+        det *= d;
+        if (d == cdouble(0.0)) return 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            cdouble f = A[(size_t)i * n + k] / d;
+            for (int j = k + 1; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+        }
+    }
+    return det;
+}
+
+// CPP/solve_faddeev.cpp:271-296
+static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
+    std::vector<cdouble> P((size_t)(M + 1) * (M + 1)), Q((size_t)(M + 1) * (M + 1));
+    for (int r = 0; r < M; ++r)
+        for (int c = 0; c <= M; ++c) P[(size_t)r * (M + 1) + c] = Q[(size_t)r * (M + 1) + c] = a[N - M + 1 + r + c];
+    for (int c = 0; c <= M; ++c) {
+        Q[(size_t)M * (M + 1) + c] = std::pow(z, M - c);
+        cdouble s = 0.0;
+        for (int j = M - c; j <= N; ++j) s += a[j - (M - c)] * std::pow(z, j);
+        P[(size_t)M * (M + 1) + c] = s;
+    }
+    return det_lu(P, M + 1) / det_lu(Q, M + 1);
+}
+
+// ------------------------------------------------------------------ GMRES (batched)
+struct Gmres {
+    RowOperator &op;
+    int r, m;
+    int64_t ne;
+    DevBuf<double2> V, W, Xs, R, tmp, hdev;
+    DevBuf<double2> partial;
+    int nb = 64;
+    std::vector<double2> hpart;
+
+    Gmres(RowOperator &o, int r_, int m_) : op(o), r(r_), m(m_) {
+        ne = o.dim * r;
+        V.alloc((size_t)(m + 1) * ne);
+        W.alloc(ne);
+        Xs.alloc(ne);
+        R.alloc(ne);
+        tmp.alloc(ne);
+        hdev.alloc((size_t)(m + 1) * r);
+        partial.alloc((size_t)(m + 1) * nb * r);
+    }
+
+    // H[i][k] = <V_i, Wv> for i < nvec (host result)
+    void dots(const double2 *Vb, int nvec, const double2 *Wv, std::vector<cdouble> &H) {
+        int bs = 256 - 256 % r;
+        if (bs < r) bs = r;
+        hipLaunchKernelGGL(k_dots, dim3(nb, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, nvec, Wv, partial.p,
+                           ne, r);
+        hpart.resize((size_t)nvec * nb * r);
+        TT_HIP(hipMemcpyAsync(hpart.data(), partial.p, sizeof(double2) * hpart.size(), hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipStreamSynchronize(op.s));
+        H.assign((size_t)nvec * r, 0.0);
+        for (int i = 0; i < nvec; ++i)
+            for (int b = 0; b < nb; ++b)
+                for (int k = 0; k < r; ++k) {
+                    const double2 v = hpart[((size_t)i * nb + b) * r + k];
+                    H[(size_t)i * r + k] += cdouble(v.x, v.y);
+                }
+    }
+
+    std::vector<double> norms(const double2 *Wv) {
+        std::vector<cdouble> H;
+        dots(Wv, 1, Wv, H);
+        std::vector<double> n(r);
+        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, H[k].real()));
+        return n;
+    }
+
+    void upload_coeffs(const std::vector<cdouble> &c) {
+        hdev.upload(reinterpret_cast<const double2 *>(c.data()), c.size(), op.s);
+    }
+
+    // M(y) = y - (y .* G) A  into out
+    void apply_M(const double2 *y, double2 *out, bool fp32) {
+        op.apply(y, tmp.p, r, true, fp32);
+        hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, out, y, tmp.p, 1.0, -1.0, ne);
+    }
+
+    // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).
+    // Returns iterations; res = final true relative residual per system (max).
+    int solve(const double2 *b, double2 *x, double tol, int max_iter, bool fp32, double &res_out,
+              const std::vector<double> &bnorm) {
+        int it_total = 0;
+        std::vector<cdouble> H((size_t)(m + 1) * m * r), cs((size_t)m * r), sn((size_t)m * r), g((size_t)(m + 1) * r);
+        std::vector<char> done(r, 0);
+        double res_max = 0.0;
+        while (true) {
+            // true residual R = b - M x
+            apply_M(x, R.p, fp32);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, R.p, b, R.p, 1.0, -1.0, ne);
+            std::vector<double> beta = norms(R.p);
+            res_max = 0.0;
+            bool all = true;
+            for (int k = 0; k < r; ++k) {
+                double rel = beta[k] / std::max(bnorm[k], 1e-300);
+                res_max = std::max(res_max, rel);
+                if (rel > tol) all = false;
+            }
+            if (all || it_total >= max_iter) break;
+            // v0 = R / beta
+            std::vector<cdouble> inv(r);
+            for (int k = 0; k < r; ++k) inv[k] = beta[k] > 0 ? 1.0 / beta[k] : 0.0;
+            upload_coeffs(inv);
+            hipLaunchKernelGGL(k_scale, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R.p, hdev.p, ne, r);
+            std::fill(g.begin(), g.end(), 0.0);
+            for (int k = 0; k < r; ++k) g[k] = beta[k];
+            int j = 0;
+            for (; j < m && it_total < max_iter; ++j, ++it_total) {
+                double2 *vj = V.p + (size_t)j * ne;
+                double2 *vj1 = V.p + (size_t)(j + 1) * ne;
+                apply_M(vj, W.p, fp32);
+                // CGS2
+                std::vector<cdouble> h1, h2;
+                dots(V.p, j + 1, W.p, h1);
+                upload_coeffs(h1);
+                hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, hdev.p,
+                                   ne, r);
+                dots(V.p, j + 1, W.p, h2);
+                upload_coeffs(h2);
+                hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, hdev.p,
+                                   ne, r);
+                std::vector<double> hn = norms(W.p);
+                std::vector<cdouble> inv2(r);
+                for (int k = 0; k < r; ++k) inv2[k] = hn[k] > 0 ? 1.0 / hn[k] : 0.0;
+                upload_coeffs(inv2);
+                hipLaunchKernelGGL(k_scale, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, hdev.p, ne, r);
+                bool conv_all = true;
+                for (int k = 0; k < r; ++k) {
+                    auto Hk = [&](int i, int jj) -> cdouble & { return H[((size_t)i * m + jj) * r + k]; };
+                    for (int i = 0; i <= j; ++i) Hk(i, j) = h1[(size_t)i * r + k] + h2[(size_t)i * r + k];
+                    Hk(j + 1, j) = hn[k];
+                    for (int i = 0; i < j; ++i) {
+                        cdouble c = cs[(size_t)i * r + k], sv = sn[(size_t)i * r + k];
+                        cdouble t0 = std::conj(c) * Hk(i, j) + std::conj(sv) * Hk(i + 1, j);
+                        cdouble t1 = -sv * Hk(i, j) + c * Hk(i + 1, j);
+                        Hk(i, j) = t0;
+                        Hk(i + 1, j) = t1;
+                    }
+                    cdouble a = Hk(j, j), bb = Hk(j + 1, j);
+                    double den = std::sqrt(std::norm(a) + std::norm(bb));
+                    cdouble c = den > 0 ? a / den : 1.0, sv = den > 0 ? bb / den : 0.0;
+                    cs[(size_t)j * r + k] = c;
+                    sn[(size_t)j * r + k] = sv;
+                    Hk(j, j) = den;
+                    Hk(j + 1, j) = 0.0;
+                    cdouble gj = g[(size_t)j * r + k];
+                    g[(size_t)j * r + k] = std::conj(c) * gj;
+                    g[(size_t)(j + 1) * r + k] = -sv * gj;
+                    double rel = std::abs(g[(size_t)(j + 1) * r + k]) / std::max(bnorm[k], 1e-300);
+                    if (rel > 0.5 * tol) conv_all = false;
+                }
+                if (conv_all) {
+                    ++j;
+                    ++it_total;
+                    break;
+                }
+            }
+            // back substitution and update x += V z
+            std::vector<cdouble> z((size_t)(m + 1) * r, 0.0);
+            for (int k = 0; k < r; ++k) {
+                auto Hk = [&](int i, int jj) { return H[((size_t)i * m + jj) * r + k]; };
+                for (int i = j - 1; i >= 0; --i) {
+                    cdouble sacc = g[(size_t)i * r + k];
+                    for (int l = i + 1; l < j; ++l) sacc -= Hk(i, l) * z[(size_t)l * r + k];
+                    z[(size_t)i * r + k] = Hk(i, i) != cdouble(0.0) ? sacc / Hk(i, i) : 0.0;
+                }
+            }
+            upload_coeffs(z);
+            hipLaunchKernelGGL(k_add_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, j, hdev.p, ne, r);
+            (void)done;
+        }
+        res_out = res_max;
+        return it_total;
+    }
+};
+
+// ------------------------------------------------------------------ solve driver
+struct OnshellLayout {
+    int nd, nE, r;
+    std::vector<int64_t> row_idx;   // r rows: (d_row, j) -> idx of (alpha_d, q_j, 0)
+    std::vector<int> row_g;         // energy of each row
+};
+
+static OnshellLayout onshell_layout(const tt_channel_os &os, int Nq, int Np) {
+    OnshellLayout L;
+    L.nd = os.num_deuteron_states;
+    L.nE = os.num_T_lab;
+    L.r = L.nd * L.nE;
+    for (int d = 0; d < L.nd; ++d)
+        for (int j = 0; j < L.nE; ++j) {
+            L.row_idx.push_back(((int64_t)os.deuteron_idx[d] * Nq + os.q_com_idx[j]) * Np);
+            L.row_g.push_back(j);
+        }
+    return L;
+}
+
+// values rows[(dr, j)][(alpha_dc, q_j, 0)] -> out[dr][dc][j]
+static void gather_onshell(RowOperator &op, const double2 *rows, const OnshellLayout &L, const tt_channel_os &os,
+                           std::vector<cdouble> &out) {
+    std::vector<int64_t> idx;
+    std::vector<int> kk;
+    for (int dr = 0; dr < L.nd; ++dr)
+        for (int dc = 0; dc < L.nd; ++dc)
+            for (int j = 0; j < L.nE; ++j) {
+                idx.push_back(((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np);
+                kk.push_back(dr * L.nE + j);
+            }
+    int n = (int)idx.size();
+    DevBuf<int64_t> di;
+    DevBuf<int> dk;
+    DevBuf<double2> dout(n);
+    di.upload(idx.data(), n, op.s);
+    dk.upload(kk.data(), n, op.s);
+    hipLaunchKernelGGL(k_gather, dim3(nblk(n, 64)), dim3(64), 0, op.s, rows, di.p, dk.p, dout.p, n, L.r);
+    std::vector<double2> h(n);
+    TT_HIP(hipMemcpyAsync(h.data(), dout.p, sizeof(double2) * n, hipMemcpyDeviceToHost, op.s));
+    TT_HIP(hipStreamSynchronize(op.s));
+    out.resize(n);
+    for (int i = 0; i < n; ++i) out[i] = cdouble(h[i].x, h[i].y);
+}
+
+static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 *Bout) {
+    DevBuf<double2> E((size_t)op.dim * L.r);
+    DevBuf<int64_t> di;
+    TT_HIP(hipMemsetAsync(E.p, 0, sizeof(double2) * E.n, op.s));
+    di.upload(L.row_idx.data(), L.r, op.s);
+    hipLaunchKernelGGL(k_unit_rows, dim3(nblk(L.r, 64)), dim3(64), 0, op.s, E.p, di.p, L.r);
+    op.apply(E.p, Bout, L.r, false, false);
+}
+
+static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
+                       tt_solve_stats &st) {
+    OnshellLayout L = onshell_layout(os, op.Nq, op.Np);
+    const int nel = L.nd * L.nd * L.nE;
+    const int NM_max = o.NM_max > 0 ? o.NM_max : 14;
+    const int nterms = 2 * NM_max + 1;
+    op.set_gidx(L.row_g);
+    DevBuf<double2> rows((size_t)op.dim * L.r), rows2((size_t)op.dim * L.r);
+    on_shell_rows_of_A(op, L, rows.p);
+    std::vector<cdouble> a((size_t)nel * nterms, 0.0), an;
+    gather_onshell(op, rows.p, L, os, an);
+    for (int e = 0; e < nel; ++e) a[(size_t)e * nterms] = an[e];
+    std::vector<cdouble> PA((size_t)nel * (NM_max + 1), 0.0);
+    std::vector<int> best_idx(nel, 0);
+    std::vector<char> conv(nel, 0);
+    int nconv = 0, n_done = 0;
+    for (int NM = 0; NM <= NM_max; ++NM) {
+        if (nconv == nel) break;
+        for (int n = 2 * NM - 1; n < 2 * NM + 1; ++n) {
+            if (n <= 0) continue;
+            op.apply(rows.p, rows2.p, L.r, true, false);
+            std::swap(rows.p, rows2.p);
+            gather_onshell(op, rows.p, L, os, an);
+            for (int e = 0; e < nel; ++e)
+                if (!conv[e]) a[(size_t)e * nterms + n] = an[e];
+            n_done = n;
+        }
+        for (int e = 0; e < nel; ++e) {
+            if (conv[e]) continue;
+            cdouble *PAe = &PA[(size_t)e * (NM_max + 1)];
+            PAe[NM] = pade_approximant(&a[(size_t)e * nterms], NM, NM, 1.0);
+            // selection rules of lines 1255-1295; the approximant "before" NM = 0 is taken as 0
+            auto prev = [&](int k) { return k >= 0 ? PAe[k] : cdouble(0.0); };
+            int best = 0;
+            double mind = 1.0;
+            for (int k = 0; k < NM; ++k) {
+                double d = std::abs(PAe[k] - prev(k - 1));
+                if (d < mind && d > 1e-15) {
+                    best = k;
+                    mind = d;
+                }
+            }
+            double d = std::abs(PAe[NM] - prev(NM - 1));
+            if (d < mind && d > 1e-15) {
+                best = NM;
+                mind = d;
+            }
+            bool c0 = NM == NM_max, c1 = NM - best > 4, c2 = mind < 1e-6 * std::abs(PAe[best]), c3 = mind < 1e-7;
+            if (c0 || c1 || c2 || c3) {
+                conv[e] = 1;
+                best_idx[e] = best;
+                ++nconv;
+            }
+        }
+    }
+    for (int e = 0; e < nel; ++e) U[e] = PA[(size_t)e * (NM_max + 1) + best_idx[e]];
+    st.iterations = n_done + 1;
+    std::swap(rows.p, rows2.p);  // both buffers freed by their owners
+    std::swap(rows.p, rows2.p);
+}
+
+static void solve_krylov(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
+                         tt_solve_stats &st, bool mixed) {
+    OnshellLayout L = onshell_layout(os, op.Nq, op.Np);
+    op.set_gidx(L.row_g);
+    const int64_t ne = op.dim * L.r;
+    DevBuf<double2> b(ne), x(ne);
+    on_shell_rows_of_A(op, L, b.p);
+    const int m = o.restart > 0 ? o.restart : 40;
+    Gmres gm(op, L.r, m);
+    std::vector<double> bnorm = gm.norms(b.p);
+    TT_HIP(hipMemsetAsync(x.p, 0, sizeof(double2) * ne, op.s));
+    double res = 0.0;
+    int iters = 0;
+    if (!mixed) {
+        iters = gm.solve(b.p, x.p, o.tol, o.max_iter, false, res, bnorm);
+    } else {
+        // iterative refinement: FP64 residual, FP32-product GMRES corrections
+        DevBuf<double2> rres(ne), d(ne);
+        for (int outer = 0; outer < 20; ++outer) {
+            gm.apply_M(x.p, rres.p, false);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, rres.p, b.p, rres.p, 1.0, -1.0, ne);
+            std::vector<double> rn = gm.norms(rres.p);
+            res = 0.0;
+            for (int k = 0; k < L.r; ++k) res = std::max(res, rn[k] / std::max(bnorm[k], 1e-300));
+            if (res <= o.tol || iters >= o.max_iter) break;
+            TT_HIP(hipMemsetAsync(d.p, 0, sizeof(double2) * ne, op.s));
+            double inner_res = 0.0;
+            // inner tolerance relative to this residual
+            std::vector<double> rnorm = rn;
+            double inner_tol = std::max(1e-6, o.tol / std::max(res, 1e-300));
+            inner_tol = std::min(inner_tol, 1e-4);
+            iters += gm.solve(rres.p, d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x.p, x.p, d.p, 1.0, 1.0, ne);
+        }
+    }
+    std::vector<cdouble> out;
+    gather_onshell(op, x.p, L, os, out);
+    for (size_t e = 0; e < out.size(); ++e) U[e] = out[e];
+    st.iterations = iters;
+    st.residual = res;
+    if (res > o.tol) fail(TT_ERR_NOT_CONVERGED, "Krylov solve did not reach the requested tolerance");
+}
+
+// Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
+static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt_solve_stats &st) {
+    const int64_t dim = op.dim;
+    TT_REQUIRE(dim <= 4096, "dense solve is a debug path for dim <= 4096");
+    std::vector<cdouble> A((size_t)dim * dim);
+    const int chunk = 64;
+    DevBuf<double2> E((size_t)dim * chunk), Y((size_t)dim * chunk);
+    DevBuf<int64_t> di;
+    std::vector<double2> h((size_t)dim * chunk);
+    op.set_gidx(std::vector<int>(chunk, -1));
+    for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
+        int rc = (int)std::min<int64_t>(chunk, dim - r0);
+        std::vector<int64_t> idx(rc);
+        for (int k = 0; k < rc; ++k) idx[k] = r0 + k;
+        TT_HIP(hipMemsetAsync(E.p, 0, sizeof(double2) * dim * rc, op.s));
+        di.upload(idx.data(), rc, op.s);
+        hipLaunchKernelGGL(k_unit_rows, dim3(nblk(rc, 64)), dim3(64), 0, op.s, E.p, di.p, rc);
+        op.apply(E.p, Y.p, rc, false, false);
+        TT_HIP(hipMemcpyAsync(h.data(), Y.p, sizeof(double2) * dim * rc, hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipStreamSynchronize(op.s));
+        for (int k = 0; k < rc; ++k)
+            for (int64_t c = 0; c < dim; ++c) A[(size_t)(r0 + k) * dim + c] = cdouble(h[c * rc + k].x, h[c * rc + k].y);
+    }
+    std::vector<double2> Gh((size_t)op.nE * dim);
+    TT_HIP(hipMemcpyAsync(Gh.data(), op.G.p, sizeof(double2) * Gh.size(), hipMemcpyDeviceToHost, op.s));
+    TT_HIP(hipStreamSynchronize(op.s));
+    const int nd = os.num_deuteron_states, nE = os.num_T_lab;
+    for (int j = 0; j < nE; ++j) {
+        // L = 1 - A G_j (columns scaled), RHS = on-shell columns of A (CPP/solve_faddeev.cpp:372-388)
+        std::vector<cdouble> M((size_t)dim * dim);
+        for (int64_t rr = 0; rr < dim; ++rr)
+            for (int64_t c = 0; c < dim; ++c) {
+                const double2 g = Gh[(size_t)j * dim + c];
+                M[(size_t)rr * dim + c] = (rr == c ? 1.0 : 0.0) - A[(size_t)rr * dim + c] * cdouble(g.x, g.y);
+            }
+        std::vector<cdouble> Bm((size_t)dim * nd);
+        for (int64_t rr = 0; rr < dim; ++rr)
+            for (int dc = 0; dc < nd; ++dc) {
+                int64_t c = ((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np;
+                Bm[(size_t)rr * nd + dc] = A[(size_t)rr * dim + c];
+            }
+        // Gaussian elimination with partial pivoting
+        for (int64_t k = 0; k < dim; ++k) {
+            int64_t piv = k;
+            double best = std::abs(M[(size_t)k * dim + k]);
+            for (int64_t i = k + 1; i < dim; ++i)
+                if (std::abs(M[(size_t)i * dim + k]) > best) {
+                    best = std::abs(M[(size_t)i * dim + k]);
+                    piv = i;
+                }
+            if (piv != k) {
+                for (int64_t c = 0; c < dim; ++c) std::swap(M[(size_t)k * dim + c], M[(size_t)piv * dim + c]);
+                for (int c = 0; c < nd; ++c) std::swap(Bm[(size_t)k * nd + c], Bm[(size_t)piv * nd + c]);
+            }
+            cdouble d = M[(size_t)k * dim + k];
+            for (int64_t i = k + 1; i < dim; ++i) {
+                cdouble f = M[(size_t)i * dim + k] / d;
+                if (f == cdouble(0.0)) continue;
+                for (int64_t c = k + 1; c < dim; ++c) M[(size_t)i * dim + c] -= f * M[(size_t)k * dim + c];
+                for (int c = 0; c < nd; ++c) Bm[(size_t)i * nd + c] -= f * Bm[(size_t)k * nd + c];
+            }
+        }
+        for (int64_t i = dim - 1; i >= 0; --i)
+            for (int c = 0; c < nd; ++c) {
+                cdouble s = Bm[(size_t)i * nd + c];
+                for (int64_t l = i + 1; l < dim; ++l) s -= M[(size_t)i * dim + l] * Bm[(size_t)l * nd + c];
+                Bm[(size_t)i * nd + c] = s / M[(size_t)i * dim + i];
+            }
+        for (int dr = 0; dr < nd; ++dr)
+            for (int dc = 0; dc < nd; ++dc) {
+                int64_t rr = ((int64_t)os.deuteron_idx[dr] * op.Nq + os.q_com_idx[j]) * op.Np;
+                U[(size_t)dr * nd * nE + (size_t)dc * nE + j] = Bm[(size_t)rr * nd + dc];
+            }
+    }
+    st.iterations = 0;
+}
+
+static DeviceCSC *dev_of(const tt_csc *m) {
+    TT_REQUIRE(m && m->dev, "P123 has no device mirror (use tt_csc_upload or tt_fill_P123_arrays)");
+    return reinterpret_cast<DeviceCSC *>(m->dev);
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+tt_status tt_ctx_create(int device, tt_ctx **out) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(out, "null argument");
+    int n = 0;
+    TT_HIP(hipGetDeviceCount(&n));
+    TT_REQUIRE(device >= 0 && device < n, "no such HIP device");
+    TT_HIP(hipSetDevice(device));
+    auto *c = new tt_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        fail(TT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    TT_GUARD_END
+}
+
+void tt_ctx_destroy(tt_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void *tt_ctx_stream(tt_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+tt_status tt_ctx_synchronize(tt_ctx *c) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(c, "null context");
+    TT_HIP(hipStreamSynchronize(c->stream));
+    TT_GUARD_END
+}
+
+void tt_solve_opts_default(tt_solve_opts *o) {
+    o->method = TT_SOLVE_PADE;
+    o->tol = 1e-12;
+    o->max_iter = 2000;
+    o->restart = 40;
+    o->NM_max = 14;
+}
+
+tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
+                        const uint64_t *colptr, tt_csc *out) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && out && colptr && (nnz == 0 || (val && row)) && dim > 0, "null argument");
+    TT_REQUIRE(colptr[0] == 0 && (int64_t)colptr[dim] == nnz, "colptr inconsistent with nnz");
+    for (int64_t c = 0; c < dim; ++c) TT_REQUIRE(colptr[c] <= colptr[c + 1], "colptr not monotone");
+    for (int64_t i = 0; i < nnz; ++i) TT_REQUIRE(row[i] >= 0 && row[i] < dim, "row index out of range");
+    TT_HIP(hipSetDevice(ctx->device));
+    auto *d = new DeviceCSC;
+    try {
+        d->dim = dim;
+        d->nnz = nnz;
+        TT_HIP(hipMalloc(&d->val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+        TT_HIP(hipMalloc(&d->row, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+        TT_HIP(hipMalloc(&d->colptr, sizeof(uint64_t) * (dim + 1)));
+        if (nnz) {
+            TT_HIP(hipMemcpyAsync(d->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
+            TT_HIP(hipMemcpyAsync(d->row, row, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
+        }
+        TT_HIP(hipMemcpyAsync(d->colptr, colptr, sizeof(uint64_t) * (dim + 1), hipMemcpyHostToDevice, ctx->stream));
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        delete d;
+        throw;
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->dim = dim;
+    out->nnz = nnz;
+    out->dev = d;
+    TT_GUARD_END
+}
+
+tt_status tt_csc_download(tt_ctx *ctx, tt_csc *m) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && m, "null argument");
+    DeviceCSC *d = dev_of(m);
+    std::free(m->val);
+    std::free(m->row);
+    std::free(m->colptr);
+    m->val = (double *)std::malloc(sizeof(double) * std::max<int64_t>(d->nnz, 1));
+    m->row = (int32_t *)std::malloc(sizeof(int32_t) * std::max<int64_t>(d->nnz, 1));
+    m->colptr = (uint64_t *)std::malloc(sizeof(uint64_t) * (d->dim + 1));
+    if (!m->val || !m->row || !m->colptr) throw std::bad_alloc();
+    if (d->nnz) {
+        TT_HIP(hipMemcpyAsync(m->val, d->val, sizeof(double) * d->nnz, hipMemcpyDeviceToHost, ctx->stream));
+        TT_HIP(hipMemcpyAsync(m->row, d->row, sizeof(int32_t) * d->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    TT_HIP(hipMemcpyAsync(m->colptr, d->colptr, sizeof(uint64_t) * (d->dim + 1), hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    m->dim = d->dim;
+    m->nnz = d->nnz;
+    TT_GUARD_END
+}
+
+void tt_csc_free(tt_csc *m) {
+    if (!m) return;
+    std::free(m->val);
+    std::free(m->row);
+    std::free(m->colptr);
+    delete reinterpret_cast<DeviceCSC *>(m->dev);
+    std::memset(m, 0, sizeof(*m));
+}
+
+tt_status tt_calculate_resolvent(tt_ctx *ctx, const tt_run_params *rp, const tt_swp_states *swp,
+                                 const tt_pw_states *chn, const double *E, int nE, double *G) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && swp && chn && E && G && nE > 0, "null argument");
+    TT_HIP(hipSetDevice(ctx->device));
+    std::vector<int> which, offset;
+    swp_edge_offsets(*chn, *swp, *rp, which, offset);
+    const int Na = chn->Nalpha, Nq = swp->Nq_WP, Np = swp->Np_WP;
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const size_t nu = (size_t)swp->num_2N_unco_states * (Np + 1), nc = (size_t)swp->num_2N_coup_states * 2 * (Np + 1);
+    DevBuf<double> eu, ec, qe, dE;
+    DevBuf<int> dw, doff;
+    DevBuf<double2> dG((size_t)dim * nE);
+    eu.upload(swp->e_SWP_unco, std::max<size_t>(nu, 1), ctx->stream);
+    if (nc) ec.upload(swp->e_SWP_coup, nc, ctx->stream);
+    qe.upload(swp->q_WP, Nq + 1, ctx->stream);
+    dE.upload(E, nE, ctx->stream);
+    dw.upload(which.data(), Na, ctx->stream);
+    doff.upload(offset.data(), Na, ctx->stream);
+    hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, ctx->stream, dG.p, eu.p, ec.p, dw.p,
+                       doff.p, qe.p, dE.p, Na, Nq, Np, nE);
+    TT_HIP(hipGetLastError());
+    TT_HIP(hipMemcpyAsync(G, dG.p, sizeof(double2) * dim * nE, hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    TT_GUARD_END
+}
+
+tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G,
+                                     const tt_csc *P123, const double *V_unco, const double *V_coup,
+                                     const tt_swp_states *swp, const tt_channel_os *os, const tt_pw_states *chn,
+                                     const tt_solve_opts *opts, tt_solve_stats *stats) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && U && G && P123 && V_unco && swp && os && chn, "null argument");
+    TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
+    TT_HIP(hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    tt_solve_opts o;
+    tt_solve_opts_default(&o);
+    if (opts) o = *opts;
+    tt_solve_stats st;
+    std::memset(&st, 0, sizeof(st));
+    RowOperator op;
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, os->num_T_lab);
+    std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
+    switch (o.method) {
+        case TT_SOLVE_PADE:
+            solve_pade(op, *os, o, Uc.data(), st);
+            break;
+        case TT_SOLVE_DENSE:
+            solve_dense(op, *os, Uc.data(), st);
+            break;
+        case TT_SOLVE_GMRES:
+            solve_krylov(op, *os, o, Uc.data(), st, false);
+            break;
+        case TT_SOLVE_MIXED:
+            solve_krylov(op, *os, o, Uc.data(), st, true);
+            break;
+        default:
+            fail(TT_ERR_ARG, "unknown solve method");
+    }
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(U, Uc.data(), sizeof(cdouble) * Uc.size());
+    st.n_apply = op.n_apply;
+    st.n_apply_fp32 = op.n_apply32;
+    st.ms_apply = op.ms_apply;
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    TT_GUARD_END
+}
+
+tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123, const double *V_unco,
+                        const double *V_coup, const tt_swp_states *swp, const tt_pw_states *chn, const double *G,
+                        int nE, const int *g_index, int r, const double *X, double *Y, int fp32) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
+    TT_HIP(hipSetDevice(ctx->device));
+    RowOperator op;
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, nE);
+    std::vector<int> g(r, -1);
+    bool useG = false;
+    if (g_index)
+        for (int k = 0; k < r; ++k) {
+            TT_REQUIRE(g_index[k] < nE, "g_index out of range");
+            g[k] = g_index[k];
+            if (g[k] >= 0) useG = true;
+        }
+    TT_REQUIRE(!useG || G, "G required when g_index selects energies");
+    op.set_gidx(g);
+    const int64_t dim = op.dim;
+    // host rows (r x dim) -> device RHS-fastest layout
+    std::vector<double2> xt((size_t)dim * r), yt((size_t)dim * r);
+    for (int k = 0; k < r; ++k)
+        for (int64_t i = 0; i < dim; ++i) xt[(size_t)i * r + k] = make_double2(X[2 * ((size_t)k * dim + i)], X[2 * ((size_t)k * dim + i) + 1]);
+    DevBuf<double2> dx, dy((size_t)dim * r);
+    dx.upload(xt.data(), xt.size(), ctx->stream);
+    op.apply(dx.p, dy.p, r, useG, fp32 != 0);
+    TT_HIP(hipMemcpyAsync(yt.data(), dy.p, sizeof(double2) * yt.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < r; ++k)
+        for (int64_t i = 0; i < dim; ++i) {
+            Y[2 * ((size_t)k * dim + i)] = yt[(size_t)i * r + k].x;
+            Y[2 * ((size_t)k * dim + i) + 1] = yt[(size_t)i * r + k].y;
+        }
+    TT_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Internal declarations of the tictac library (host C++ and HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tictac.h"
+
+namespace tt {
+
+// Physical constants, CPP/constants.h:19-35.
+constexpr double kMp = 938.272;
+constexpr double kMn = 939.565;
+constexpr double kMN = 0.5 * (kMp + kMn);
+constexpr double kMu23 = kMp * kMn / (kMn + kMp);
+constexpr double kHbarc = 197.327;
+constexpr int kDeuteronL = 0, kDeuteronS = 1, kDeuteronJ = 1, kDeuteronT = 0;
+
+using cdouble = std::complex<double>;
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+struct Error {
+    tt_status code;
+    std::string msg;
+};
+[[noreturn]] void fail(tt_status code, const std::string &msg);
+
+#define TT_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            ::tt::fail(TT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define TT_REQUIRE(cond, msg)                                  \
+    do {                                                       \
+        if (!(cond)) ::tt::fail(TT_ERR_ARG, std::string(msg)); \
+    } while (0)
+
+// -------------------------------------------------------------- kinematics
+double mu1(double Eb);
+double com_momentum_to_lab_energy(double q_com, double Eb);
+double lab_energy_to_com_momentum(double T_lab, double Eb);
+double com_energy_to_com_q_momentum(double E_com);
+double com_q_momentum_to_com_energy(double q_com);
+void gauss_legendre(int n, double *x, double *w);                 // on [-1, 1], ascending
+void gauss_legendre_ab(int n, double a, double b, double *x, double *w);
+
+// ------------------------------------------------------------ 2N channels
+int unique_2N_idx(int L, int S, int J, int T, bool coupled, const tt_run_params &rp);
+void num_2N_states(const tt_run_params &rp, int &nu, int &nc);
+// coupled_via_L_2N / coupled_via_T_3N of a (row, column) pair, CPP/solve_faddeev.cpp:1703-1713
+void coupling_kind(int Lr, int Sr, int Jr, int Lc, const tt_run_params &rp, bool &via_L, bool &via_T);
+
+// Block table of a 2N-channel operator embedded in the (alpha, p) space of one
+// 3N channel: for every coupled pair (alpha_r, alpha_c) the Np x Np block
+// M_2N[h(alpha_r) Np + p_r, h(alpha_c) Np + p_c] (see the pointer tables of
+// CPP/solve_faddeev.cpp:1588-1927).  Dense (Nalpha*Np)^2 layout, row-major.
+std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_unco,
+                             const double *M_coup, const tt_run_params &rp);
+
+// Per-alpha offset of the SWP energy edges used by the resolvent (CPP/make_resolvent.cpp:118-156):
+// returns for each alpha a pointer offset into either e_SWP_unco (flag 0) or e_SWP_coup (flag 1).
+void swp_edge_offsets(const tt_pw_states &chn, const tt_swp_states &swp, const tt_run_params &rp,
+                      std::vector<int> &which, std::vector<int> &offset);
+
+// --------------------------------------------------------- angular tables
+double wigner3j(int tj1, int tj2, int tj3, int tm1, int tm2, int tm3);
+double wigner6j(int a, int b, int c, int d, int e, int f);
+double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
+double clebsch_gordan(int tj1, int tj2, int tj3, int tm1, int tm2, int tm3);
+double atilde(const tt_pw_states &chn, int a, int ap, int Ltot, int two_J, int two_T);
+
+// Host description of the angular part of P123 for one channel:
+//   G~_{a'a}(phi, x) = sum_e coef_e * F_{tuple_e}(phi, x)
+//   F_t = sum_terms c * Y1[i1](x) * Y2[i2](cos th1) * Y3[i3](cos th2)
+struct P123Angular {
+    int lmax1 = 0, lmax2 = 0, lmax3 = 0;           // l3 (bra), L12 (ket), l3 (ket)
+    int n_tuples = 0;
+    std::vector<int> term_ptr;                      // n_tuples + 1
+    std::vector<double> term_coef;                  // includes 8 pi^2 sqrt((2L'+1)/4pi), CGs, signs
+    std::vector<int> term_i1, term_i2, term_i3;     // indices l(l+1)/2 + |m| into the tables
+    std::vector<int> pair_ptr;                      // Nalpha^2 + 1, pair = a'*Nalpha + a
+    std::vector<int> pair_tuple;                    // tuple index of each entry
+    std::vector<double> pair_atilde;                // A~(a', a, Ltot) of each entry
+};
+P123Angular build_p123_angular(const tt_pw_states &chn);
+
+// ------------------------------------------------------------ library ctx
+struct DeviceCSC {
+    int64_t dim = 0, nnz = 0;
+    double *val = nullptr;
+    float *val32 = nullptr;          // lazily made FP32 copy for the mixed path
+    int32_t *row = nullptr;
+    uint64_t *colptr = nullptr;
+    ~DeviceCSC();
+};
+
+}  // namespace tt
+
+#define TT_GUARD_BEGIN try {
+#define TT_GUARD_END                                  \
+    }                                                 \
+    catch (const Error &e) {                          \
+        set_error(e.msg);                             \
+        return e.code;                                \
+    }                                                 \
+    catch (const std::bad_alloc &) {                  \
+        set_error("host allocation failed");          \
+        return TT_ERR_ALLOC;                          \
+    }                                                 \
+    return TT_OK;
+
+struct tt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};

@@ -1,0 +1,271 @@
+"""Host-side mirror of the reference driver (CPP/main.cpp:83-505) over the C-ABI.
+
+``Problem`` builds the state spaces, potential matrices and SWP basis once
+(CPP/main.cpp:144-153, 273-347); ``Channel`` holds one 3N channel J^Pi with
+its on-shell indexing (CPP/main.cpp:177-198, 356-378) and calls the three
+solve-path entry points: P123 build, resolvent, Faddeev solve.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, dptr, iptr
+
+HBARC = 197.327
+
+
+def chebyshev_scale(N, top_fm=8.0):
+    """Scale s putting the top Chebyshev edge s*tan((2N-1)pi/4N) at top_fm fm^-1."""
+    return top_fm * HBARC / np.tan((2 * N - 1) * np.pi / (4 * N))
+
+
+class Context:
+    """One HIP device and stream (tt_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        check(L.lib().tt_ctx_create(device, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        check(L.lib().tt_ctx_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            L.lib().tt_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_params(**kw):
+    rp = L.RunParams()
+    L.lib().tt_run_params_default(C.byref(rp))
+    for k, v in kw.items():
+        if not hasattr(rp, k):
+            raise KeyError(k)
+        setattr(rp, k, v)
+    return rp
+
+
+class Potential:
+    def __init__(self, name, params=None):
+        self._h = C.c_void_p()
+        check(L.lib().tt_potential_create(name.encode(), C.byref(self._h)))
+        self.name = name
+        if params is not None:
+            self.update(params)
+
+    def update(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float64)
+        check(L.lib().tt_potential_update_parameters(self._h, dptr(a), len(a)))
+        self.params = a
+
+    def V(self, p_in, p_out, coupled, S, J, T, Tz):
+        out = np.zeros(6)
+        check(L.lib().tt_potential_V(self._h, p_in, p_out, int(coupled), S, J, T, Tz, dptr(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                L.lib().tt_potential_destroy(self._h)
+        except Exception:
+            pass
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype).copy()
+
+
+class Problem:
+    """State spaces + 2N potential + SWP basis for one parameter set."""
+
+    def __init__(self, rp, potential):
+        lib = L.lib()
+        self.rp = rp
+        self.pot = potential
+        self.pw = L.PwStates()
+        check(lib.tt_construct_pw_states(C.byref(rp), C.byref(self.pw)))
+        self.fwp = L.FwpStates()
+        check(lib.tt_make_fwp_statespace(C.byref(rp), C.byref(self.fwp)))
+        nu, nc = C.c_int(), C.c_int()
+        lib.tt_num_2N_states(C.byref(rp), C.byref(nu), C.byref(nc))
+        self.nu, self.nc = nu.value, nc.value
+        Np = rp.Np_WP
+        self.V_unco = np.full(max(self.nu * Np * Np, 1), np.nan)
+        self.V_coup = np.full(max(self.nc * 4 * Np * Np, 1), np.nan)
+        check(lib.tt_calculate_potential_matrices(C.byref(rp), C.byref(self.fwp), C.byref(self.pw), potential._h,
+                                                  dptr(self.V_unco), dptr(self.V_coup)))
+        self.e_unco = np.zeros(max(self.nu * (Np + 1), 1))
+        self.e_coup = np.zeros(max(self.nc * 2 * (Np + 1), 1))
+        self.C_unco = np.full(max(self.nu * Np * Np, 1), np.nan)
+        self.C_coup = np.full(max(self.nc * 4 * Np * Np, 1), np.nan)
+        self.swp = L.SwpStates()
+        self.swp.e_SWP_unco = dptr(self.e_unco)
+        self.swp.e_SWP_coup = dptr(self.e_coup)
+        self.swp.C_SWP_unco = dptr(self.C_unco)
+        self.swp.C_SWP_coup = dptr(self.C_coup)
+        check(lib.tt_make_swp_states(C.byref(rp), C.byref(self.fwp), C.byref(self.pw), dptr(self.V_unco),
+                                     dptr(self.V_coup), C.byref(self.swp)))
+        self.E_bound = self.swp.E_bound
+        self.p_edges = _arr(self.fwp.p_WP, Np + 1, np.float64)
+        self.q_edges = _arr(self.fwp.q_WP, rp.Nq_WP + 1, np.float64)
+
+    @property
+    def n_channels(self):
+        return self.pw.N_chn_3N
+
+    def channel(self, c, T_lab):
+        return Channel(self, c, T_lab)
+
+    def pw_arrays(self):
+        n = self.pw.Nalpha
+        names = ["L_2N", "S_2N", "J_2N", "T_2N", "L_1N", "two_J_1N", "two_J_3N", "two_T_3N", "P_3N"]
+        d = {k: _arr(getattr(self.pw, k), n, np.int64) for k in names}
+        d["chn_3N_idx"] = _arr(self.pw.chn_3N_idx, self.pw.N_chn_3N + 1, np.int64)
+        return d
+
+    def __del__(self):
+        try:
+            L.lib().tt_pw_states_free(C.byref(self.pw))
+            L.lib().tt_fwp_states_free(C.byref(self.fwp))
+        except Exception:
+            pass
+
+
+class P123:
+    """Device-resident P123 in the reference's CSC layout (tt_csc)."""
+
+    def __init__(self, csc, ctx, stats=None):
+        self.csc = csc
+        self.ctx = ctx
+        self.stats = stats
+
+    @property
+    def dim(self):
+        return self.csc.dim
+
+    @property
+    def nnz(self):
+        return self.csc.nnz
+
+    @classmethod
+    def from_host(cls, ctx, dim, val, row, colptr):
+        csc = L.CSC()
+        val = np.ascontiguousarray(val, np.float64)
+        row = np.ascontiguousarray(row, np.int32)
+        colptr = np.ascontiguousarray(colptr, np.uint64)
+        check(L.lib().tt_csc_upload(ctx.handle, dim, len(val), dptr(val), row.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    colptr.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(csc)))
+        return cls(csc, ctx)
+
+    def to_host(self):
+        check(L.lib().tt_csc_download(self.ctx.handle, C.byref(self.csc)))
+        n, d = self.csc.nnz, self.csc.dim
+        return (_arr(self.csc.val, n, np.float64), _arr(self.csc.row, n, np.int32),
+                _arr(self.csc.colptr, d + 1, np.uint64))
+
+    def __del__(self):
+        try:
+            L.lib().tt_csc_free(C.byref(self.csc))
+        except Exception:
+            pass
+
+
+class Channel:
+    """One 3N channel J^Pi with its elastic on-shell indexing."""
+
+    def __init__(self, prob, c, T_lab):
+        lib = L.lib()
+        self.prob = prob
+        self.index = c
+        self.view = L.PwStates()
+        check(lib.tt_pw_channel(C.byref(prob.pw), c, C.byref(self.view)))
+        self.Nalpha = self.view.Nalpha
+        self.dim = self.Nalpha * prob.rp.Np_WP * prob.rp.Nq_WP
+        self.os = L.ChannelOS()
+        T = np.ascontiguousarray(T_lab, np.float64)
+        check(lib.tt_find_on_shell(C.byref(prob.swp), C.byref(self.view), dptr(T), len(T), C.byref(self.os)))
+        nE = self.os.num_T_lab
+        self.nE = nE
+        self.q_idx = _arr(self.os.q_com_idx, nE, np.int64)
+        self.T_lab = _arr(self.os.T_lab, nE, np.float64)
+        self.E_com = _arr(self.os.E_com, nE, np.float64)
+        self.nd = self.os.num_deuteron_states
+        self.deut = _arr(self.os.deuteron_idx, self.nd, np.int64)
+        self.two_J = int(self.view.two_J_3N[0])
+        self.parity = int(self.view.P_3N[0])
+
+    def pw_arrays(self):
+        n = self.Nalpha
+        names = ["L_2N", "S_2N", "J_2N", "T_2N", "L_1N", "two_J_1N", "two_J_3N", "two_T_3N", "P_3N"]
+        return {k: _arr(getattr(self.view, k), n, np.int64) for k in names}
+
+    def energies(self):
+        """Total energies E_com + E_bound of the on-shell bins (CPP/main.cpp:391)."""
+        return self.E_com + self.prob.E_bound
+
+    def build_p123(self, ctx, production_run=True):
+        csc = L.CSC()
+        st = L.P123Stats()
+        check(L.lib().tt_fill_P123_arrays(ctx.handle, C.byref(self.prob.rp), C.byref(self.prob.fwp),
+                                          C.byref(self.view), int(production_run), C.byref(csc), C.byref(st)))
+        return P123(csc, ctx, st)
+
+    def resolvent(self, ctx, E=None):
+        E = np.ascontiguousarray(self.energies() if E is None else E, np.float64)
+        G = np.zeros(len(E) * self.dim * 2)
+        check(L.lib().tt_calculate_resolvent(ctx.handle, C.byref(self.prob.rp), C.byref(self.prob.swp),
+                                             C.byref(self.view), dptr(E), len(E), dptr(G)))
+        return G.view(np.complex128).reshape(len(E), self.dim)
+
+    def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14):
+        """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics."""
+        if G is None:
+            G = self.resolvent(ctx)
+        G = np.ascontiguousarray(G, np.complex128)
+        assert G.shape == (self.nE, self.dim)
+        o = L.SolveOpts()
+        L.lib().tt_solve_opts_default(C.byref(o))
+        o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
+        st = L.SolveStats()
+        U = np.zeros(self.nd * self.nd * self.nE * 2)
+        Gf = G.view(np.float64).ravel()
+        check(L.lib().tt_solve_faddeev_equations(ctx.handle, C.byref(self.prob.rp), dptr(U), dptr(Gf),
+                                                 C.byref(P.csc), dptr(self.prob.V_unco), dptr(self.prob.V_coup),
+                                                 C.byref(self.prob.swp), C.byref(self.os), C.byref(self.view),
+                                                 C.byref(o), C.byref(st)))
+        return U.view(np.complex128).reshape(self.nd, self.nd, self.nE), st
+
+    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False):
+        """Y = (X .* G_sel) A for rows X (r, dim) complex."""
+        X = np.ascontiguousarray(X, np.complex128)
+        r = X.shape[0]
+        Y = np.zeros_like(X)
+        nE = 0 if G is None else G.shape[0]
+        Gf = np.zeros(2) if G is None else np.ascontiguousarray(G, np.complex128).view(np.float64).ravel()
+        gi = None
+        if g_index is not None:
+            gi = np.ascontiguousarray(g_index, np.int32)
+        check(L.lib().tt_apply_rows(ctx.handle, C.byref(self.prob.rp), C.byref(P.csc), dptr(self.prob.V_unco),
+                                    dptr(self.prob.V_coup), C.byref(self.prob.swp), C.byref(self.view),
+                                    dptr(Gf), nE, iptr(gi) if gi is not None else None, r,
+                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)), int(fp32)))
+        return Y
+
+    def __del__(self):
+        try:
+            L.lib().tt_channel_os_free(C.byref(self.os))
+        except Exception:
+            pass
