@@ -1,0 +1,156 @@
+"""Benchmark of the AGS/WPCD solve path on one MI355X.
+
+Step = one channel-solve: the resolvent G of the on-shell energies plus the
+on-shell U-matrix solve, for the 3N channel J^Pi = 1/2+ of the seeded synthetic
+NN potential at Np = Nq = 30 (N_alpha = 27, dim = 24300), T_lab = 10 MeV
+(2 on-shell q bins x 2 deuteron channels = 4 coupled systems).  P123, V and the
+SWP basis are built once before timing and stay resident in HBM.
+
+The headline value is the mixed-precision solve (FP32 P.V.G products with FP64
+iterative refinement to |dU/U| <= 1e-8); the plain FP64 GMRES path is timed on
+the same inputs and reported beside it, with the accuracy of the mixed result
+against it.  One JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tic-tac_amd"))
+
+import numpy as np  # noqa: E402
+
+import tictac as tt  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("synthetic NN (3 Yukawas + contacts, seed 7), J^Pi=1/2+, Np=Nq=30, J2max=3, N_alpha=27, "
+            "dim=24300, T_lab=10 MeV -> 2 on-shell bins x 2 deuteron channels")
+
+
+def build(N=30, seed=7, T_lab=(10.0,)):
+    rp = tt.run_params(two_J_3N_max=1, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N))
+    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(seed)))
+    ch = prob.channel(0, list(T_lab))
+    return prob, ch
+
+
+def timed_steps(ctx, ch, P, method, steps, warmup):
+    stats = []
+    U = None
+    for _ in range(warmup):
+        G = ch.resolvent(ctx)
+        U, st = ch.solve(ctx, P, G, method=method)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        G = ch.resolvent(ctx)
+        U, st = ch.solve(ctx, P, G, method=method)
+        stats.append(st)
+    ctx.synchronize()
+    return time.perf_counter() - t0, U, stats
+
+
+def spmm_roofline(stats):
+    ms = sum(s.ms_spmm for s in stats)
+    nb = sum(s.spmm_bytes for s in stats)
+    n = sum(s.n_spmm for s in stats)
+    achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_spmm_gather", "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+            "bytes_per_launch": round(nb / max(n, 1))}
+
+
+def cpu_baseline(prob, ch, P, G, budget_s=20.0):
+    """Oracle (numpy/scipy, 1 thread) converged GMRES on the same inputs: channel-solves/s."""
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from threadpoolctl import threadpool_limits
+    import scipy.sparse.linalg as sla
+    from oracle import solve as so
+    from helpers import oracle_pw, oracle_swp, oracle_V, flags
+    val, row, ptr = P.to_host()
+    tf, ib = flags(prob)
+    Vu, Vc = oracle_V(prob)
+    N = prob.rp.Np_WP
+    with threadpool_limits(limits=1):
+        K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
+        done, t0 = 0, time.perf_counter()
+        U = np.zeros((ch.nd, ch.nd, ch.nE), complex)
+        while True:
+            for dr in range(ch.nd):
+                for j in range(ch.nE):
+                    e = np.zeros(ch.dim)
+                    e[ch.deut[dr] * N * N + ch.q_idx[j] * N] = 1.0
+                    b = K.row_apply(e[None, :])[0].astype(complex)
+                    g = G[j]
+                    op = sla.LinearOperator((ch.dim, ch.dim), dtype=complex,
+                                            matvec=lambda v, g=g: v - K.row_apply((v * g)[None, :])[0])
+                    y, _ = sla.gmres(op, b, rtol=1e-12, restart=60, maxiter=50)
+                    for dc in range(ch.nd):
+                        U[dr, dc, j] = y[ch.deut[dc] * N * N + ch.q_idx[j] * N]
+            done += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 2 or done >= 8:
+                break
+    return {"value": round(done / el, 4), "unit": "channel-solves/s", "cores": 1, "kind": "port",
+            "sample": "%d full channel-solve(s) (4 GMRES systems each, rtol 1e-12) of the same workload, "
+                      "scipy.sparse on 1 thread" % done}, U
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
+              "independent; see DESIGN.md for how the work would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    prob, ch = build()
+    ctx = tt.Context(0)
+    P = ch.build_p123(ctx)
+    ctx.synchronize()
+
+    t_mix, U_mix, st_mix = timed_steps(ctx, ch, P, tt.SOLVE_MIXED, args.steps, args.warmup)
+    t_64, U_64, st_64 = timed_steps(ctx, ch, P, tt.SOLVE_GMRES, args.steps, args.warmup)
+    rel = float(np.max(np.abs(U_mix - U_64)) / np.max(np.abs(U_64)))
+    value = args.steps / t_mix
+    fp64 = args.steps / t_64
+
+    out = {
+        "metric": "U-matrix channel-solves/s batched-mixed vs plain FP64 @|dU/U|<=1e-8; grad/fwd time",
+        "value": round(value, 3),
+        "unit": "channel-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mix / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 products + fp64 refinement (complex)",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "nnz_P123": int(P.nnz), "p123_build_ms": round(P.stats.ms_total, 2)},
+        "fp64_value": round(fp64, 3),
+        "mixed_over_fp64": round(value / fp64, 3),
+        "mixed_vs_fp64_max_rel": rel,
+        "grad_over_fwd": None,
+        "roofline": spmm_roofline(st_mix),
+        "roofline_fp64": spmm_roofline(st_64),
+    }
+    if not args.no_cpu:
+        G = ch.resolvent(ctx)
+        cb, U_cpu = cpu_baseline(prob, ch, P, G)
+        cb["max_rel_vs_gpu_fp64"] = float(np.max(np.abs(U_cpu - U_64)) / np.max(np.abs(U_64)))
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -207,6 +207,10 @@ typedef struct {
     int max_iter;               /* total Krylov iterations per system       */
     int restart;                /* GMRES restart length                     */
     int NM_max;                 /* Pade order limit (reference: 14)         */
+    /* Optional (Pade): Neumann terms a_n, interleaved complex, laid out
+     * [(d_row*nd + d_col)*nE + j][n] for n < 2*NM_max+1; terms not computed stay 0.
+     * The reference appends them to neumann_terms*.txt (CPP/solve_faddeev.cpp:907-917). */
+    double *neumann_terms;
 } tt_solve_opts;
 void tt_solve_opts_default(tt_solve_opts *o);
 
@@ -217,6 +221,13 @@ typedef struct {
     double residual;            /* final relative residual (max over systems) */
     double ms_total;
     double ms_apply;            /* device time inside A applications        */
+    /* The gather SpMM over P123 (dominant kernel): device time summed over its
+     * launches and the algorithmic bytes those launches must move:
+     * nnz*(s_v+4) + 8*(dim+1) + 2*dim*r*s_c per launch (s_v, s_c: value and
+     * complex-vector element sizes of the precision used). */
+    int n_spmm;
+    double ms_spmm;
+    double spmm_bytes;
 } tt_solve_stats;
 
 /* Replaces solve_faddeev_equations (CPP/solve_faddeev.h:25-38).

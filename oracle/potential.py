@@ -105,7 +105,7 @@ def synthetic_theta(seed=SYNTH_DEFAULT_SEED, contacts=None, separable=None):
     beta = rng.uniform(1.0, 3.0, size=3)
     lam = rng.normal(size=3) * 0.01
     if contacts is None:
-        contacts = [-0.10, -0.165, 0.20, 0.15, 0.20, 0.02]
+        contacts = [-0.10, -0.12653, 0.20, 0.15, 0.20, 0.02]
     if separable is not None:
         lam = np.asarray(separable, dtype=float)
     theta = np.concatenate([Y, np.asarray(contacts, float), lam])

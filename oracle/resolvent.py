@@ -21,6 +21,24 @@ def heaviside(v):
     return 0.0 if v < 0 else 1.0
 
 
+def resolvent_term_scale(E, Eb, q_u, q_l, e_u, e_l, bound):
+    """Sum of the magnitudes of the terms the closed forms add up, divided by their
+    denominators: the rounding of each term (a few ulp of its size) bounds how far two
+    correct evaluations of the same formula may differ.  The continuum formula is a
+    second difference of v ln|v| and loses up to ~10 digits for narrow bins far off shell."""
+    m1 = st.Mn * (st.Mn + st.Mp + Eb) / (st.Mn + st.Mn + st.Mp + Eb)
+    Eql = q_l * q_l / (2 * m1)
+    Equ = q_u * q_u / (2 * m1)
+    Dq = Equ - Eql
+    if bound:
+        return (abs(math.log(abs(Eql + Eb - E))) + abs(math.log(abs(Equ + Eb - E))) + math.pi) / Dq
+    Dp = e_u - e_l
+    Ep, Eq = 0.5 * (e_l + e_u), 0.5 * (Eql + Equ)
+    DM, DP, D = 0.5 * (Dp - Dq), 0.5 * (Dp + Dq), Ep + Eq - E
+    s = sum(abs(v) * (abs(math.log(abs(v))) + 1.0) for v in (D + DM, D - DM, D + DP, D - DP) if v != 0.0)
+    return s / (Dp * Dq)
+
+
 def resolvent_bound_continuum(E, Eb, q_u, q_l):
     """CPP/make_resolvent.cpp:14-36."""
     m1 = st.Mn * (st.Mn + st.Mp + Eb) / (st.Mn + st.Mn + st.Mp + Eb)
@@ -79,12 +97,14 @@ def swp_edge_table(pw_chn, swp, tensor_force, isospin_breaking_1S0):
     return out
 
 
-def resolvent_array(E, pw_chn, swp, tensor_force, isospin_breaking_1S0):
-    """G[(alpha, q, p)] at total energy E (CPP/make_resolvent.cpp:82-211)."""
+def resolvent_array(E, pw_chn, swp, tensor_force, isospin_breaking_1S0, with_scale=False):
+    """G[(alpha, q, p)] at total energy E (CPP/make_resolvent.cpp:82-211); with_scale also
+    returns resolvent_term_scale of every element."""
     Np, Nq = swp.Np, swp.Nq
     edges = swp_edge_table(pw_chn, swp, tensor_force, isospin_breaking_1S0)
     qe = swp.q_edges
     G = np.zeros(pw_chn.Nalpha * Nq * Np, dtype=complex)
+    scale = np.zeros(len(G))
     for a in range(pw_chn.Nalpha):
         for p in range(Np):
             el, eu = edges[a, p], edges[a, p + 1]
@@ -96,4 +116,6 @@ def resolvent_array(E, pw_chn, swp, tensor_force, isospin_breaking_1S0):
                 else:
                     g = resolvent_continuum_continuum(E, Eb, qe[q + 1], qe[q], eu, el)
                 G[a * Nq * Np + q * Np + p] = g
-    return G
+                if with_scale:
+                    scale[a * Nq * Np + q * Np + p] = resolvent_term_scale(E, Eb, qe[q + 1], qe[q], eu, el, bound)
+    return (G, scale) if with_scale else G

@@ -1,0 +1,197 @@
+"""GPU parity of the gfx950 solve path against the oracle, through the C-ABI.
+
+Tolerances (floating point; the reference itself is FP64):
+  resolvent            64 ulp of the summed term magnitudes of the reference's closed form
+                       (oracle.resolvent.resolvent_term_scale): the continuum formula is a second
+                       difference of v ln|v| and is ill-conditioned for narrow bins far off shell
+  P123 elements        structure identical; values 1e-13 relative to max|P|
+  A application        1e-13 (FP64), 1e-5 (FP32 products)
+  converged U          1e-10 relative (dense, GMRES); mixed vs FP64 1e-8 (the workload's target)
+  Pade                 Neumann terms 1e-10; U within 1e-5 of the converged solve, the accuracy of
+                       the reference's own stopping rule (1e-6 relative / 1e-7 absolute)
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+import tictac as tt
+from helpers import flags, mt_config, oracle_pw, oracle_swp, oracle_V, synthetic_config
+from oracle import p123 as pm
+from oracle import resolvent as rs
+from oracle import solve as so
+
+pytestmark = pytest.mark.gpu
+
+
+class Case:
+    pass
+
+
+def _setup(ctx, rp, pot, T, oracle_p123=None):
+    c = Case()
+    c.prob = tt.Problem(rp, pot)
+    c.ch = c.prob.channel(0, T)
+    c.P = c.ch.build_p123(ctx)
+    c.G = c.ch.resolvent(ctx)
+    c.och = oracle_pw(c.ch)
+    c.oswp = oracle_swp(c.prob)
+    c.Vu, c.Vc = oracle_V(c.prob)
+    c.tf, c.ib = flags(c.prob)
+    c.val, c.row, c.ptr = c.P.to_host()
+    c.K = so.KernelA(c.och, c.oswp, c.Vu, c.Vc, so.p_matrix(c.val, c.row, c.ptr, c.ch.dim), c.tf, c.ib)
+    return c
+
+
+@pytest.fixture(scope="module")
+def c1(ctx):
+    rp, pot, T = mt_config(N=16)
+    c = _setup(ctx, rp, pot, T)
+    c.A = c.K.dense()
+    c.Ud = so.dense_solve(c.A, c.G, c.ch.deut, c.ch.q_idx, rp.Nq_WP, rp.Np_WP)
+    return c
+
+
+@pytest.fixture(scope="module")
+def c2(ctx):
+    rp, pot, T = synthetic_config(N=30)
+    return _setup(ctx, rp, pot, T)
+
+
+# ------------------------------------------------------------------ Malfliet-Tjon, N = 16
+def _check_resolvent(c):
+    for j, E in enumerate(c.ch.energies()):
+        Go, scale = rs.resolvent_array(E, c.och, c.oswp, c.tf, c.ib, with_scale=True)
+        err = np.abs(c.G[j] - Go)
+        assert (err <= 64 * np.finfo(float).eps * scale + 1e-15 * np.abs(Go)).all(), err.max()
+        # well-conditioned elements (the bulk) agree to 1e-12 relative
+        good = scale <= 1e3 * np.abs(Go)
+        assert good.mean() > 0.2
+        assert (err[good] <= 1e-12 * np.abs(Go[good]) + 1e-15).all()
+
+
+def test_resolvent_c1(c1):
+    _check_resolvent(c1)
+
+
+def test_p123_c1_matches_oracle_structure_and_values(c1):
+    rp = c1.prob.rp
+    ent, mask = pm.p123_rows(c1.och, c1.prob.p_edges, c1.prob.q_edges, rp.Nphi, rp.Nx)
+    vo, ro, po = pm.to_csc(ent, c1.ch.dim)
+    assert c1.P.stats.mask_density == pytest.approx(mask.mean(), abs=0)
+    assert np.array_equal(po, c1.ptr) and np.array_equal(ro, c1.row)
+    assert np.abs(vo - c1.val).max() <= 1e-13 * np.abs(vo).max()
+
+
+def test_apply_c1(ctx, c1):
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(3, c1.ch.dim)) + 1j * rng.normal(size=(3, c1.ch.dim))
+    Gsel = np.stack([c1.G[0], c1.G[1], np.ones(c1.ch.dim)])
+    Yo = (X * Gsel) @ c1.A
+    Y = c1.ch.apply_rows(ctx, c1.P, X, G=c1.G, g_index=[0, 1, -1])
+    assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
+    Y32 = c1.ch.apply_rows(ctx, c1.P, X, G=c1.G, g_index=[0, 1, -1], fp32=True)
+    assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
+
+
+@pytest.mark.parametrize("method", [tt.SOLVE_DENSE, tt.SOLVE_GMRES, tt.SOLVE_MIXED])
+def test_converged_solves_c1(ctx, c1, method):
+    U, st = c1.ch.solve(ctx, c1.P, c1.G, method=method)
+    assert np.abs(U - c1.Ud).max() <= 1e-10 * np.abs(c1.Ud).max()
+    if method != tt.SOLVE_DENSE:
+        assert st.residual <= 1e-12
+
+
+def test_pade_c1(ctx, c1):
+    U, st, a = c1.ch.solve(ctx, c1.P, c1.G, method=tt.SOLVE_PADE, neumann=True)
+    ao = so.neumann_terms(c1.K, c1.G, c1.ch.deut, c1.ch.q_idx, c1.prob.rp.Nq_WP, c1.prob.rp.Np_WP, a.shape[-1])
+    done = a != 0
+    assert done[..., :3].all()
+    for n in range(a.shape[-1]):
+        m = done[..., n]
+        if m.any():
+            assert np.abs(a[..., n][m] - ao[..., n][m]).max() <= 1e-10 * np.abs(ao[..., n]).max()
+    assert np.abs(U - c1.Ud).max() <= 1e-5 * np.abs(c1.Ud).max()
+
+
+# ---------------------------------------------------------- synthetic potential, N = 30, N_alpha = 27
+def test_p123_c2_sampled_rows(c2):
+    rp = c2.prob.rp
+    bins = [(3, 4), (21, 22)]
+    ent, _ = pm.p123_rows(c2.och, c2.prob.p_edges, c2.prob.q_edges, rp.Nphi, rp.Nx, row_bins=bins)
+    Pr = so.p_matrix(c2.val, c2.row, c2.ptr, c2.ch.dim).tocsr()
+    N = rp.Np_WP
+    rows = [a * N * N + qp * N + pp for (qp, pp) in bins for a in range(c2.ch.Nalpha)]
+    assert sum(Pr[r].nnz for r in rows) == len(ent)
+    mx = max(abs(v) for v in ent.values())
+    for (r, cidx), v in ent.items():
+        assert abs(Pr[r, cidx] - v) <= 1e-13 * mx
+
+
+def test_p123_c2_placeholder_structure(ctx, c2):
+    """production_run = 0 (CPP/make_permutation_matrix.cpp:817-822): cos(row)cos(col) on every
+    overlapping cell and alpha pair; the production matrix is its exact-zero-dropped subset."""
+    Q = c2.ch.build_p123(ctx, production_run=False)
+    v, r, p = Q.to_host()
+    Na = c2.ch.Nalpha
+    assert Q.nnz == Q.stats.n_cells * Na * Na
+    cols = np.repeat(np.arange(c2.ch.dim), np.diff(p.astype(np.int64)))
+    assert np.abs(v - np.cos(r.astype(float)) * np.cos(cols.astype(float))).max() < 1e-15
+    # rows strictly ascending inside each column
+    d = np.diff(r.astype(np.int64))
+    starts = p[1:-1].astype(np.int64)
+    ok = np.ones(len(d), bool)
+    ok[starts[(starts > 0) & (starts < len(r))] - 1] = False
+    assert (d[ok] > 0).all()
+    prod_cols = np.repeat(np.arange(c2.ch.dim), np.diff(c2.ptr.astype(np.int64)))
+    key_q = set((cols * c2.ch.dim + r).tolist())
+    key_p = (prod_cols * c2.ch.dim + c2.row).tolist()
+    assert all(k in key_q for k in key_p[::97])
+
+
+def test_resolvent_c2(c2):
+    _check_resolvent(c2)
+
+
+def test_apply_c2(ctx, c2):
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(4, c2.ch.dim)) + 1j * rng.normal(size=(4, c2.ch.dim))
+    g = [0, 1, 0, -1]
+    Gsel = np.stack([c2.G[k] if k >= 0 else np.ones(c2.ch.dim) for k in g])
+    Yo = c2.K.row_apply(X * Gsel)
+    Y = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g)
+    assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
+
+
+def test_solves_c2_against_oracle_gmres(ctx, c2):
+    ch, N = c2.ch, c2.prob.rp.Np_WP
+    Uo = np.zeros((ch.nd, ch.nd, ch.nE), complex)
+    for dr in range(ch.nd):
+        for j in range(ch.nE):
+            e = np.zeros(ch.dim)
+            e[ch.deut[dr] * N * N + ch.q_idx[j] * N] = 1.0
+            b = c2.K.row_apply(e[None, :])[0].astype(complex)
+            g = c2.G[j]
+            op = sla.LinearOperator((ch.dim, ch.dim), dtype=complex,
+                                    matvec=lambda v, g=g: v - c2.K.row_apply((v * g)[None, :])[0])
+            y, info = sla.gmres(op, b, rtol=1e-13, restart=60, maxiter=50)
+            assert info == 0
+            for dc in range(ch.nd):
+                Uo[dr, dc, j] = y[ch.deut[dc] * N * N + ch.q_idx[j] * N]
+    U64, st64 = ch.solve(ctx, c2.P, c2.G, method=tt.SOLVE_GMRES)
+    assert np.abs(U64 - Uo).max() <= 1e-10 * np.abs(Uo).max()
+    Umx, stmx = ch.solve(ctx, c2.P, c2.G, method=tt.SOLVE_MIXED)
+    assert stmx.n_apply_fp32 > 0
+    assert np.abs(Umx - U64).max() <= 1e-8 * np.abs(U64).max()
+    Upa, _ = ch.solve(ctx, c2.P, c2.G, method=tt.SOLVE_PADE)
+    assert np.abs(Upa - U64).max() <= 1e-5 * np.abs(U64).max()
+
+
+def test_error_paths_on_device(ctx, c1):
+    with pytest.raises(tt.TictacError) as e:
+        tt.P123.from_host(ctx, 4, np.ones(2), np.array([0, 9], np.int32), np.array([0, 1, 2, 2, 2], np.uint64))
+    assert e.value.code == tt._lib.TT_ERR_ARG
+    # P123 of another dimension
+    bad = tt.P123.from_host(ctx, 8, np.ones(1), np.array([0], np.int32), np.array([0] + [1] * 8, np.uint64))
+    with pytest.raises(tt.TictacError) as e:
+        c1.ch.solve(ctx, bad, c1.G)
+    assert e.value.code == tt._lib.TT_ERR_ARG

@@ -101,99 +101,146 @@ __global__ void k_resolvent(double2 *__restrict__ G, const double *__restrict__ 
 }
 
 // ------------------------------------------------------------------ block ops
-// Y[(a,q,p)][k] = sum_{e in part(a)} sum_p' B_e[p][p'] * Xg[(alpha_e,q,p')][k]
-// with Xg = X .* G_{gidx[k]} when G != nullptr.  One thread per output element.
+// Y[(a,q,p)][k] = sum_{e in part(a)} sum_p' B_e[p][p'] * Xg[(alpha_e,q,p')][k]   (or Z - that sum)
+// with Xg = X .* G_{gidx[k]} when G != nullptr.  One workgroup per output tile (a, q): the
+// partner rows Xg[(alpha_e, q, :)][:] are staged in LDS, then each thread forms outputs (p, k).
 template <typename TI, typename TO>
-__global__ void k_block(const TI *__restrict__ X, TO *__restrict__ Y, const int *__restrict__ part_ptr,
-                        const int *__restrict__ part_alpha, const double *__restrict__ blocks,
-                        const double2 *__restrict__ G, const int *__restrict__ gidx, int Na, int Nq, int Np,
-                        int r) {
+__global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
+                                               const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
+                                               const double *__restrict__ blocks, const double2 *__restrict__ G,
+                                               const int *__restrict__ gidx, int Na, int Nq, int Np, int r,
+                                               const double2 *__restrict__ Zsub) {
+    extern __shared__ double2 xs[];  // [npart][Np][r]
     const int64_t dim = (int64_t)Na * Nq * Np;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= dim * r) return;
-    const int k = (int)(t % r);
-    const int64_t idx = t / r;
-    const int a = (int)(idx / (Nq * Np));
-    const int q = (int)((idx / Np) % Nq);
-    const int p = (int)(idx % Np);
-    const int g = G ? gidx[k] : -1;
-    double sr = 0.0, si = 0.0;
-    for (int e = part_ptr[a]; e < part_ptr[a + 1]; ++e) {
-        const int ap = part_alpha[e];
-        const double *B = blocks + ((size_t)e * Np + p) * Np;
-        const int64_t base = ((int64_t)ap * Nq + q) * Np;
-        for (int pp = 0; pp < Np; ++pp) {
-            const int64_t src = base + pp;
-            double xr = (double)X[src * r + k].x, xi = (double)X[src * r + k].y;
+    const int a = blockIdx.x / Nq, q = blockIdx.x % Nq;
+    const int e0 = part_ptr[a], ne = part_ptr[a + 1] - e0;
+    const int tile = Np * r;
+    for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
+        const int e = idx / tile, rem = idx % tile, pp = rem / r, k = rem % r;
+        const int64_t src = ((int64_t)part_alpha[e0 + e] * Nq + q) * Np + pp;
+        double xr = (double)X[src * r + k].x, xi = (double)X[src * r + k].y;
+        if (G) {
+            const int g = gidx[k];
             if (g >= 0) {
                 const double2 gv = G[(int64_t)g * dim + src];
                 const double tr = xr * gv.x - xi * gv.y;
                 xi = xr * gv.y + xi * gv.x;
                 xr = tr;
             }
-            sr += B[pp] * xr;
-            si += B[pp] * xi;
         }
+        xs[idx] = make_double2(xr, xi);
     }
-    Y[t].x = sr;
-    Y[t].y = si;
+    __syncthreads();
+    for (int o = threadIdx.x; o < tile; o += blockDim.x) {
+        const int p = o / r, k = o % r;
+        double sr = 0.0, si = 0.0;
+        for (int e = 0; e < ne; ++e) {
+            const double *B = blocks + ((size_t)(e0 + e) * Np + p) * Np;
+            const double2 *xe = xs + e * tile + k;
+            for (int pp = 0; pp < Np; ++pp) {
+                const double b = B[pp];
+                const double2 x = xe[pp * r];
+                sr += b * x.x;
+                si += b * x.y;
+            }
+        }
+        const int64_t dst = (((int64_t)a * Nq + q) * Np + p) * r + k;
+        if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
+            sr = Zsub[dst].x - sr;
+            si = Zsub[dst].y - si;
+        }
+        Y[dst].x = sr;
+        Y[dst].y = si;
+    }
 }
 
 // ------------------------------------------------------------------ gather SpMM
-// U[c][v] = scale * sum_{nz in column c} val[nz] * S[row[nz]][v], v < R2 (real lanes).
-// One wavefront per column.  The wave loads 64 consecutive (val, row) pairs with one
-// coalesced access, then the lanes (slot, v) walk them through cross-lane shuffles.
-template <typename T>
+// U[c][v] = scale * sum_{nz in column c} val[nz] * S[row[nz]][v], v < R2 (real components of
+// the r complex right-hand sides).  One wavefront per column of the CSC (= row of P^T).
+// Lane = slot * NG + g: the 64/NG slots walk the column's nonzeros with one coalesced
+// (val, row) load each, and every lane gathers VPL consecutive values of its row of S;
+// UNROLL independent nonzeros per lane are in flight at once.  A cross-lane xor reduction
+// over the slots ends the column.
+template <typename T, int VPL, int UNROLL>
 __global__ __launch_bounds__(256) void k_spmm_gather(const uint64_t *__restrict__ colptr,
                                                      const int32_t *__restrict__ row, const T *__restrict__ val,
                                                      const T *__restrict__ S, T *__restrict__ U, int64_t dim,
-                                                     int R2, int R2p, T scale) {
+                                                     int R2, int NG, T scale) {
     const int lane = threadIdx.x & 63;
     const int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (col >= dim) return;  // uniform per wavefront
     const uint64_t beg = colptr[col], end = colptr[col + 1];
-    if (R2p <= 64) {
-        const int slots = 64 / R2p;
-        const int slot = lane / R2p, v = lane % R2p;
-        const bool active = v < R2;
-        T acc = 0;
-        for (uint64_t base = beg; base < end; base += 64) {
-            const int cnt = (int)min<uint64_t>(64, end - base);
-            T myval = 0;
-            int myrow = 0;
-            if (lane < cnt) {
-                myval = val[base + lane];
-                myrow = row[base + lane];
-            }
-            for (int j0 = 0; j0 < cnt; j0 += slots) {
-                const int src = j0 + slot;
-                const T pv = __shfl(myval, src < 64 ? src : 0);
-                const int pr = __shfl(myrow, src < 64 ? src : 0);
-                if (active && src < cnt) acc += pv * S[(int64_t)pr * R2 + v];
-            }
+    const int slots = 64 / NG;
+    const int slot = lane / NG, g = lane % NG;
+    const int v0 = g * VPL;
+    T acc[VPL];
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) acc[i] = 0;
+    for (uint64_t base = beg + slot; base < end; base += (uint64_t)slots * UNROLL) {
+        T pv[UNROLL];
+        int pr[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const uint64_t nz = base + (uint64_t)u * slots;
+            const bool ok = nz < end;
+            pv[u] = ok ? val[nz] : (T)0;
+            pr[u] = ok ? row[nz] : 0;
         }
-        for (int off = R2p; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
-        if (slot == 0 && active) U[col * R2 + v] = scale * acc;
-    } else {
-        for (int v0 = 0; v0 < R2; v0 += 64) {
-            const int v = v0 + lane;
-            T acc = 0;
-            for (uint64_t base = beg; base < end; base += 64) {
-                const int cnt = (int)min<uint64_t>(64, end - base);
-                T myval = 0;
-                int myrow = 0;
-                if (lane < cnt) {
-                    myval = val[base + lane];
-                    myrow = row[base + lane];
-                }
-                for (int j = 0; j < cnt; ++j) {
-                    const T pv = __shfl(myval, j);
-                    const int pr = __shfl(myrow, j);
-                    if (v < R2) acc += pv * S[(int64_t)pr * R2 + v];
-                }
-            }
-            if (v < R2) U[col * R2 + v] = scale * acc;
+        T sv[UNROLL][VPL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const T *src = S + (int64_t)pr[u] * R2 + v0;
+#pragma unroll
+            for (int i = 0; i < VPL; ++i) sv[u][i] = (v0 + i < R2) ? src[i] : (T)0;
         }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+            for (int i = 0; i < VPL; ++i) acc[i] += pv[u] * sv[u][i];
+    }
+    for (int off = NG; off < 64; off <<= 1)
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) acc[i] += __shfl_xor(acc[i], off);
+    if (slot == 0)
+#pragma unroll
+        for (int i = 0; i < VPL; ++i)
+            if (v0 + i < R2) U[col * R2 + v0 + i] = scale * acc[i];
+}
+
+// Launch shape of the gather SpMM for R2 real components: NG value groups (a power of two)
+// of VPL <= 16 values each.
+struct SpmmShape {
+    int NG, VPL;
+};
+static SpmmShape spmm_shape(int R2) {
+    int NG = 1;
+    while ((R2 + NG - 1) / NG > 16) NG <<= 1;
+    int need = (R2 + NG - 1) / NG;
+    int VPL = need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : need <= 12 ? 12 : 16;
+    return {NG, VPL};
+}
+
+template <typename T>
+static void launch_spmm(hipStream_t s, const DeviceCSC *P, const T *val, const T *S, T *U, int64_t dim, int R2,
+                        T scale) {
+    const SpmmShape sh = spmm_shape(R2);
+    const dim3 grid((unsigned)((dim + 3) / 4)), block(256);
+    switch (sh.VPL) {
+        case 2:
+            hipLaunchKernelGGL((k_spmm_gather<T, 2, 4>), grid, block, 0, s, P->colptr, P->row, val, S, U, dim, R2, sh.NG, scale);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_spmm_gather<T, 4, 4>), grid, block, 0, s, P->colptr, P->row, val, S, U, dim, R2, sh.NG, scale);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_spmm_gather<T, 8, 4>), grid, block, 0, s, P->colptr, P->row, val, S, U, dim, R2, sh.NG, scale);
+            break;
+        case 12:
+            hipLaunchKernelGGL((k_spmm_gather<T, 12, 2>), grid, block, 0, s, P->colptr, P->row, val, S, U, dim, R2, sh.NG, scale);
+            break;
+        default:
+            hipLaunchKernelGGL((k_spmm_gather<T, 16, 2>), grid, block, 0, s, P->colptr, P->row, val, S, U, dim, R2, sh.NG, scale);
+            break;
     }
 }
 
@@ -265,15 +312,6 @@ __global__ void k_add_combo(double2 *__restrict__ X, const double2 *__restrict__
     X[t] = x;
 }
 
-// Y[e] = alpha_k * X[e] (complex per column), or Y = X - Z when Z != nullptr and alpha == nullptr
-__global__ void k_scale(double2 *__restrict__ Y, const double2 *__restrict__ X, const double2 *__restrict__ alpha,
-                        int64_t n_elem, int r) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_elem) return;
-    const double2 a = alpha[t % r], x = X[t];
-    Y[t] = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
-}
-
 __global__ void k_axpby(double2 *__restrict__ Y, const double2 *__restrict__ A, const double2 *__restrict__ B,
                         double alpha, double beta, int64_t n_elem) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,6 +333,32 @@ __global__ void k_unit_rows(double2 *__restrict__ X, const int64_t *__restrict__
     if (t < r) X[dst_idx[t] * r + t] = make_double2(1.0, 0.0);
 }
 
+// out[i][k] = sum_b partial[i][b][k], fixed summation order
+__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nvec, int nb, int r,
+                                  double2 *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvec * r) return;
+    const int i = t / r, k = t % r;
+    double ar = 0.0, ai = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const double2 v = partial[((int64_t)i * nb + b) * r + k];
+        ar += v.x;
+        ai += v.y;
+    }
+    out[t] = make_double2(ar, ai);
+}
+
+// Y[e] = X[e] / sqrt(n2[k].x), zero where the squared norm is zero
+__global__ void k_scale_by_norm(double2 *__restrict__ Y, const double2 *__restrict__ X, const double2 *__restrict__ n2,
+                                int64_t n_elem, int r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
+    const double nn = n2[t % r].x;
+    const double f = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+    const double2 x = X[t];
+    Y[t] = make_double2(f * x.x, f * x.y);
+}
+
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------------ the operator
@@ -310,14 +374,44 @@ struct RowOperator {
     DevBuf<double2> S, U;      // FP64 scratch (dim * r)
     DevBuf<float2> S32, U32;   // FP32 scratch
     DevBuf<int> gidx;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double ms_apply = 0.0;
-    int n_apply = 0, n_apply32 = 0;
+    // Event pool: four events per timed application (before the first block op, around the
+    // SpMM, after the last block op); read back once in collect(), no per-apply synchronisation.
+    std::vector<hipEvent_t> evs;
+    size_t ev_used = 0;
+    std::vector<double> spmm_bytes_of;   // algorithmic bytes of each timed SpMM launch
+    double ms_apply = 0.0, ms_spmm = 0.0, spmm_bytes = 0.0;
+    int n_apply = 0, n_apply32 = 0, n_spmm = 0;
     int r_cap = 0;
+    int c_npart_max = 1, vc_npart_max = 1;
 
     ~RowOperator() {
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    }
+
+    hipEvent_t next_event() {
+        if (ev_used == evs.size()) {
+            hipEvent_t e;
+            TT_HIP(hipEventCreate(&e));
+            evs.push_back(e);
+        }
+        return evs[ev_used++];
+    }
+
+    // Sum the device times of the applications recorded since the last collect().
+    void collect() {
+        if (ev_used == 0) return;
+        TT_HIP(hipEventSynchronize(evs[ev_used - 1]));
+        for (size_t i = 0; i + 3 < ev_used; i += 4) {
+            float a = 0.f, b = 0.f;
+            TT_HIP(hipEventElapsedTime(&a, evs[i], evs[i + 3]));
+            TT_HIP(hipEventElapsedTime(&b, evs[i + 1], evs[i + 2]));
+            ms_apply += a;
+            ms_spmm += b;
+            spmm_bytes += spmm_bytes_of[i / 4];
+            ++n_spmm;
+        }
+        ev_used = 0;
+        spmm_bytes_of.clear();
     }
 
     static void block_list(const std::vector<double> &B, int Na, int Np, bool transpose, std::vector<int> &ptr,
@@ -379,6 +473,11 @@ struct RowOperator {
         std::vector<double> b1, b2;
         block_list(Cop, Na, Np, false, p1, a1, b1);   // S = C_op (X G)
         block_list(VCop, Na, Np, true, p2, a2, b2);   // Y = VC_op^T U
+        c_npart_max = vc_npart_max = 1;
+        for (int a = 0; a < Na; ++a) {
+            c_npart_max = std::max(c_npart_max, p1[a + 1] - p1[a]);
+            vc_npart_max = std::max(vc_npart_max, p2[a + 1] - p2[a]);
+        }
         c_ptr.upload(p1.data(), p1.size(), s);
         c_alpha.upload(a1.data(), std::max<size_t>(a1.size(), 1), s);
         c_blocks.upload(b1.data(), std::max<size_t>(b1.size(), 1), s);
@@ -386,10 +485,6 @@ struct RowOperator {
         vc_alpha.upload(a2.data(), std::max<size_t>(a2.size(), 1), s);
         vc_blocks.upload(b2.data(), std::max<size_t>(b2.size(), 1), s);
         if (nE > 0 && Gh) G.upload(reinterpret_cast<const double2 *>(Gh), (size_t)nE * dim, s);
-        if (!ev0) {
-            TT_HIP(hipEventCreate(&ev0));
-            TT_HIP(hipEventCreate(&ev1));
-        }
     }
 
     void reserve(int r, bool fp32) {
@@ -421,40 +516,51 @@ struct RowOperator {
     }
 
     // Y = (X .* G_{gidx}) A ; X, Y device, dim*r complex; use_G selects whether G multiplies X.
-    void apply(const double2 *X, double2 *Y, int r, bool use_G, bool fp32, bool timed = true) {
+    // Y = (X .* G_{gidx}) A, or Y = Zsub - (X .* G_{gidx}) A when Zsub is given.
+    void apply(const double2 *X, double2 *Y, int r, bool use_G, bool fp32, bool timed = true,
+               const double2 *Zsub = nullptr) {
         reserve(r, fp32);
-        if (timed) TT_HIP(hipEventRecord(ev0, s));
-        const int64_t ne = dim * r;
-        int R2 = 2 * r, R2p = 1;
-        while (R2p < R2) R2p <<= 1;
-        const unsigned spmm_grid = nblk(dim, 4);
+        if (timed && ev_used + 4 > 4096) collect();
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        if (timed) {
+            e0 = next_event();
+            e1 = next_event();
+            e2 = next_event();
+            e3 = next_event();
+            TT_HIP(hipEventRecord(e0, s));
+        }
+        const int R2 = 2 * r;
+        const double sv = fp32 ? 4.0 : 8.0, sc = fp32 ? 8.0 : 16.0;
+        const dim3 bgrid((unsigned)(Na * Nq)), bblk(256);
+        const size_t sh_c = (size_t)c_npart_max * Np * r * sizeof(double2);
+        const size_t sh_vc = (size_t)vc_npart_max * Np * r * sizeof(double2);
+        TT_REQUIRE(sh_c <= 160 * 1024 && sh_vc <= 160 * 1024, "block-op tile exceeds LDS (too many right-hand sides)");
         if (!fp32) {
-            hipLaunchKernelGGL((k_block<double2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, X, S.p, c_ptr.p,
-                               c_alpha.p, c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r);
-            hipLaunchKernelGGL((k_spmm_gather<double>), dim3(spmm_grid), dim3(256), 0, s, P->colptr, P->row, P->val,
-                               reinterpret_cast<const double *>(S.p), reinterpret_cast<double *>(U.p), dim, R2, R2p,
-                               2.0);
-            hipLaunchKernelGGL((k_block<double2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, U.p, Y, vc_ptr.p,
-                               vc_alpha.p, vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r);
+            hipLaunchKernelGGL((k_block<double2, double2>), bgrid, bblk, sh_c, s, X, S.p, c_ptr.p, c_alpha.p,
+                               c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr);
+            if (timed) TT_HIP(hipEventRecord(e1, s));
+            launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p), reinterpret_cast<double *>(U.p),
+                                dim, R2, 2.0);
+            if (timed) TT_HIP(hipEventRecord(e2, s));
+            hipLaunchKernelGGL((k_block<double2, double2>), bgrid, bblk, sh_vc, s, U.p, Y, vc_ptr.p, vc_alpha.p,
+                               vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r, Zsub);
             ++n_apply;
         } else {
             ensure_f32();
-            hipLaunchKernelGGL((k_block<double2, float2>), dim3(nblk(ne, 256)), dim3(256), 0, s, X, S32.p, c_ptr.p,
-                               c_alpha.p, c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r);
-            hipLaunchKernelGGL((k_spmm_gather<float>), dim3(spmm_grid), dim3(256), 0, s, P->colptr, P->row, P->val32,
-                               reinterpret_cast<const float *>(S32.p), reinterpret_cast<float *>(U32.p), dim, R2, R2p,
-                               2.0f);
-            hipLaunchKernelGGL((k_block<float2, double2>), dim3(nblk(ne, 256)), dim3(256), 0, s, U32.p, Y, vc_ptr.p,
-                               vc_alpha.p, vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r);
+            hipLaunchKernelGGL((k_block<double2, float2>), bgrid, bblk, sh_c, s, X, S32.p, c_ptr.p, c_alpha.p,
+                               c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr);
+            if (timed) TT_HIP(hipEventRecord(e1, s));
+            launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p), reinterpret_cast<float *>(U32.p),
+                               dim, R2, 2.0f);
+            if (timed) TT_HIP(hipEventRecord(e2, s));
+            hipLaunchKernelGGL((k_block<float2, double2>), bgrid, bblk, sh_vc, s, U32.p, Y, vc_ptr.p, vc_alpha.p,
+                               vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r, Zsub);
             ++n_apply;
             ++n_apply32;
         }
         if (timed) {
-            TT_HIP(hipEventRecord(ev1, s));
-            TT_HIP(hipEventSynchronize(ev1));
-            float ms = 0.f;
-            TT_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-            ms_apply += ms;
+            TT_HIP(hipEventRecord(e3, s));
+            spmm_bytes_of.push_back((double)P->nnz * (sv + 4.0) + 8.0 * (double)(dim + 1) + 2.0 * (double)dim * r * sc);
         }
     }
 };
@@ -501,73 +607,86 @@ static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
 }
 
 // ------------------------------------------------------------------ GMRES (batched)
+// r independent systems y (1 - G_k A) = b_k share every operator application.  The Arnoldi
+// step (operator, two classical Gram-Schmidt passes, normalisation) runs entirely on the
+// device; its Hessenberg column is copied to pinned host memory and consumed by the host's
+// Givens rotations one step later, so the host never stalls the stream inside a cycle.
 struct Gmres {
     RowOperator &op;
     int r, m;
     int64_t ne;
-    DevBuf<double2> V, W, Xs, R, tmp, hdev;
-    DevBuf<double2> partial;
     int nb = 64;
-    std::vector<double2> hpart;
+    int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
+    DevBuf<double2> V, W, R, zdev, partial, Hd, n2d;
+    double2 *Hh = nullptr;              // pinned mirror of Hd
+    std::vector<hipEvent_t> step_ev;
 
     Gmres(RowOperator &o, int r_, int m_) : op(o), r(r_), m(m_) {
+        TT_REQUIRE(r <= 256, "at most 256 simultaneous systems");
         ne = o.dim * r;
+        slot_len = (2 * (m + 1) + 1) * r;
         V.alloc((size_t)(m + 1) * ne);
         W.alloc(ne);
-        Xs.alloc(ne);
         R.alloc(ne);
-        tmp.alloc(ne);
-        hdev.alloc((size_t)(m + 1) * r);
+        zdev.alloc((size_t)(m + 1) * r);
         partial.alloc((size_t)(m + 1) * nb * r);
+        Hd.alloc((size_t)m * slot_len);
+        n2d.alloc(r);
+        TT_HIP(hipHostMalloc(&Hh, sizeof(double2) * (size_t)m * slot_len, hipHostMallocDefault));
+        step_ev.resize(m);
+        for (auto &e : step_ev) TT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ~Gmres() {
+        for (auto e : step_ev) (void)hipEventDestroy(e);
+        if (Hh) (void)hipHostFree(Hh);
     }
 
-    // H[i][k] = <V_i, Wv> for i < nvec (host result)
-    void dots(const double2 *Vb, int nvec, const double2 *Wv, std::vector<cdouble> &H) {
+    // out[i][k] = <Vb_i, Wv>_k for i < nvec, on the device
+    void dots_dev(const double2 *Vb, int nvec, const double2 *Wv, double2 *out) {
         int bs = 256 - 256 % r;
-        if (bs < r) bs = r;
         hipLaunchKernelGGL(k_dots, dim3(nb, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, nvec, Wv, partial.p,
                            ne, r);
-        hpart.resize((size_t)nvec * nb * r);
-        TT_HIP(hipMemcpyAsync(hpart.data(), partial.p, sizeof(double2) * hpart.size(), hipMemcpyDeviceToHost, op.s));
-        TT_HIP(hipStreamSynchronize(op.s));
-        H.assign((size_t)nvec * r, 0.0);
-        for (int i = 0; i < nvec; ++i)
-            for (int b = 0; b < nb; ++b)
-                for (int k = 0; k < r; ++k) {
-                    const double2 v = hpart[((size_t)i * nb + b) * r + k];
-                    H[(size_t)i * r + k] += cdouble(v.x, v.y);
-                }
+        hipLaunchKernelGGL(k_reduce_partials, dim3(nblk((int64_t)nvec * r, 64)), dim3(64), 0, op.s, partial.p, nvec,
+                           nb, r, out);
     }
 
     std::vector<double> norms(const double2 *Wv) {
-        std::vector<cdouble> H;
-        dots(Wv, 1, Wv, H);
+        dots_dev(Wv, 1, Wv, n2d.p);
+        std::vector<double2> h(r);
+        TT_HIP(hipMemcpyAsync(h.data(), n2d.p, sizeof(double2) * r, hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipStreamSynchronize(op.s));
         std::vector<double> n(r);
-        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, H[k].real()));
+        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, h[k].x));
         return n;
     }
 
-    void upload_coeffs(const std::vector<cdouble> &c) {
-        hdev.upload(reinterpret_cast<const double2 *>(c.data()), c.size(), op.s);
-    }
+    // out = M(y) = y - (y .* G) A
+    void apply_M(const double2 *y, double2 *out, bool fp32) { op.apply(y, out, r, true, fp32, true, y); }
 
-    // M(y) = y - (y .* G) A  into out
-    void apply_M(const double2 *y, double2 *out, bool fp32) {
-        op.apply(y, tmp.p, r, true, fp32);
-        hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, out, y, tmp.p, 1.0, -1.0, ne);
+    // Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
+    void step(int j, bool fp32) {
+        double2 *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
+        double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
+        apply_M(vj, W.p, fp32);
+        dots_dev(V.p, j + 1, W.p, h1);
+        hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r);
+        dots_dev(V.p, j + 1, W.p, h2);
+        hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h2, ne, r);
+        dots_dev(W.p, 1, W.p, n2);
+        hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, n2, ne, r);
+        TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipEventRecord(step_ev[j], op.s));
     }
 
     // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).
-    // Returns iterations; res = final true relative residual per system (max).
+    // Returns iterations; res_out = final true relative residual (max over systems).
     int solve(const double2 *b, double2 *x, double tol, int max_iter, bool fp32, double &res_out,
               const std::vector<double> &bnorm) {
         int it_total = 0;
         std::vector<cdouble> H((size_t)(m + 1) * m * r), cs((size_t)m * r), sn((size_t)m * r), g((size_t)(m + 1) * r);
-        std::vector<char> done(r, 0);
         double res_max = 0.0;
         while (true) {
-            // true residual R = b - M x
-            apply_M(x, R.p, fp32);
+            apply_M(x, R.p, fp32);  // R = M x, then R = b - M x
             hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, R.p, b, R.p, 1.0, -1.0, ne);
             std::vector<double> beta = norms(R.p);
             res_max = 0.0;
@@ -578,38 +697,24 @@ struct Gmres {
                 if (rel > tol) all = false;
             }
             if (all || it_total >= max_iter) break;
-            // v0 = R / beta
-            std::vector<cdouble> inv(r);
-            for (int k = 0; k < r; ++k) inv[k] = beta[k] > 0 ? 1.0 / beta[k] : 0.0;
-            upload_coeffs(inv);
-            hipLaunchKernelGGL(k_scale, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R.p, hdev.p, ne, r);
+            hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R.p, n2d.p, ne, r);
             std::fill(g.begin(), g.end(), 0.0);
             for (int k = 0; k < r; ++k) g[k] = beta[k];
-            int j = 0;
-            for (; j < m && it_total < max_iter; ++j, ++it_total) {
-                double2 *vj = V.p + (size_t)j * ne;
-                double2 *vj1 = V.p + (size_t)(j + 1) * ne;
-                apply_M(vj, W.p, fp32);
-                // CGS2
-                std::vector<cdouble> h1, h2;
-                dots(V.p, j + 1, W.p, h1);
-                upload_coeffs(h1);
-                hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, hdev.p,
-                                   ne, r);
-                dots(V.p, j + 1, W.p, h2);
-                upload_coeffs(h2);
-                hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, hdev.p,
-                                   ne, r);
-                std::vector<double> hn = norms(W.p);
-                std::vector<cdouble> inv2(r);
-                for (int k = 0; k < r; ++k) inv2[k] = hn[k] > 0 ? 1.0 / hn[k] : 0.0;
-                upload_coeffs(inv2);
-                hipLaunchKernelGGL(k_scale, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, hdev.p, ne, r);
+            int used = 0;
+            const int jmax = std::min(m, max_iter - it_total);
+            step(0, fp32);
+            for (int j = 0; j < jmax; ++j) {
+                if (j + 1 < jmax) step(j + 1, fp32);   // keep the device busy while the host rotates column j
+                TT_HIP(hipEventSynchronize(step_ev[j]));
+                const double2 *hj = Hh + (size_t)j * slot_len;
                 bool conv_all = true;
                 for (int k = 0; k < r; ++k) {
                     auto Hk = [&](int i, int jj) -> cdouble & { return H[((size_t)i * m + jj) * r + k]; };
-                    for (int i = 0; i <= j; ++i) Hk(i, j) = h1[(size_t)i * r + k] + h2[(size_t)i * r + k];
-                    Hk(j + 1, j) = hn[k];
+                    for (int i = 0; i <= j; ++i) {
+                        const double2 a1 = hj[(size_t)i * r + k], a2 = hj[(size_t)(m + 1 + i) * r + k];
+                        Hk(i, j) = cdouble(a1.x + a2.x, a1.y + a2.y);
+                    }
+                    Hk(j + 1, j) = std::sqrt(std::max(0.0, hj[(size_t)2 * (m + 1) * r + k].x));
                     for (int i = 0; i < j; ++i) {
                         cdouble c = cs[(size_t)i * r + k], sv = sn[(size_t)i * r + k];
                         cdouble t0 = std::conj(c) * Hk(i, j) + std::conj(sv) * Hk(i + 1, j);
@@ -630,25 +735,23 @@ struct Gmres {
                     double rel = std::abs(g[(size_t)(j + 1) * r + k]) / std::max(bnorm[k], 1e-300);
                     if (rel > 0.5 * tol) conv_all = false;
                 }
-                if (conv_all) {
-                    ++j;
-                    ++it_total;
-                    break;
-                }
+                used = j + 1;
+                ++it_total;
+                if (conv_all) break;
             }
-            // back substitution and update x += V z
+            // x += V z from the triangular least-squares solution
             std::vector<cdouble> z((size_t)(m + 1) * r, 0.0);
             for (int k = 0; k < r; ++k) {
                 auto Hk = [&](int i, int jj) { return H[((size_t)i * m + jj) * r + k]; };
-                for (int i = j - 1; i >= 0; --i) {
+                for (int i = used - 1; i >= 0; --i) {
                     cdouble sacc = g[(size_t)i * r + k];
-                    for (int l = i + 1; l < j; ++l) sacc -= Hk(i, l) * z[(size_t)l * r + k];
+                    for (int l = i + 1; l < used; ++l) sacc -= Hk(i, l) * z[(size_t)l * r + k];
                     z[(size_t)i * r + k] = Hk(i, i) != cdouble(0.0) ? sacc / Hk(i, i) : 0.0;
                 }
             }
-            upload_coeffs(z);
-            hipLaunchKernelGGL(k_add_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, j, hdev.p, ne, r);
-            (void)done;
+            TT_HIP(hipStreamSynchronize(op.s));  // a speculative step may still use zdev's neighbours
+            zdev.upload(reinterpret_cast<const double2 *>(z.data()), z.size(), op.s);
+            hipLaunchKernelGGL(k_add_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r);
         }
         res_out = res_max;
         return it_total;
@@ -765,9 +868,8 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
         }
     }
     for (int e = 0; e < nel; ++e) U[e] = PA[(size_t)e * (NM_max + 1) + best_idx[e]];
+    if (o.neumann_terms) std::memcpy(o.neumann_terms, a.data(), sizeof(cdouble) * a.size());
     st.iterations = n_done + 1;
-    std::swap(rows.p, rows2.p);  // both buffers freed by their owners
-    std::swap(rows.p, rows2.p);
 }
 
 static void solve_krylov(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
@@ -810,7 +912,6 @@ static void solve_krylov(RowOperator &op, const tt_channel_os &os, const tt_solv
     for (size_t e = 0; e < out.size(); ++e) U[e] = out[e];
     st.iterations = iters;
     st.residual = res;
-    if (res > o.tol) fail(TT_ERR_NOT_CONVERGED, "Krylov solve did not reach the requested tolerance");
 }
 
 // Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
@@ -941,6 +1042,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->max_iter = 2000;
     o->restart = 40;
     o->NM_max = 14;
+    o->neumann_terms = nullptr;
 }
 
 tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
@@ -1067,12 +1169,19 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
             fail(TT_ERR_ARG, "unknown solve method");
     }
     TT_HIP(hipStreamSynchronize(ctx->stream));
+    op.collect();
     std::memcpy(U, Uc.data(), sizeof(cdouble) * Uc.size());
     st.n_apply = op.n_apply;
     st.n_apply_fp32 = op.n_apply32;
     st.ms_apply = op.ms_apply;
+    st.n_spmm = op.n_spmm;
+    st.ms_spmm = op.ms_spmm;
+    st.spmm_bytes = op.spmm_bytes;
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
+    if ((o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED) && !(st.residual <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "Krylov solve stopped at relative residual " + std::to_string(st.residual) +
+                                       " above the requested tolerance (U holds the last iterate)");
     TT_GUARD_END
 }
 

@@ -60,12 +60,13 @@ class P123Stats(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int),
-                ("NM_max", C.c_int)]
+                ("NM_max", C.c_int), ("neumann_terms", _dp)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("n_apply", C.c_int), ("n_apply_fp32", C.c_int), ("iterations", C.c_int),
-                ("residual", C.c_double), ("ms_total", C.c_double), ("ms_apply", C.c_double)]
+                ("residual", C.c_double), ("ms_total", C.c_double), ("ms_apply", C.c_double),
+                ("n_spmm", C.c_int), ("ms_spmm", C.c_double), ("spmm_bytes", C.c_double)]
 
 
 # exported symbols and their prototypes (restype, argtypes)

@@ -230,8 +230,11 @@ class Channel:
                                              C.byref(self.view), dptr(E), len(E), dptr(G)))
         return G.view(np.complex128).reshape(len(E), self.dim)
 
-    def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14):
-        """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics."""
+    def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
+              neumann=False):
+        """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
+
+        With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n]."""
         if G is None:
             G = self.resolvent(ctx)
         G = np.ascontiguousarray(G, np.complex128)
@@ -239,6 +242,10 @@ class Channel:
         o = L.SolveOpts()
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
+        nterms = 2 * NM_max + 1
+        a = np.zeros(self.nd * self.nd * self.nE * nterms * 2)
+        if neumann:
+            o.neumann_terms = dptr(a)
         st = L.SolveStats()
         U = np.zeros(self.nd * self.nd * self.nE * 2)
         Gf = G.view(np.float64).ravel()
@@ -246,7 +253,10 @@ class Channel:
                                                  C.byref(P.csc), dptr(self.prob.V_unco), dptr(self.prob.V_coup),
                                                  C.byref(self.prob.swp), C.byref(self.os), C.byref(self.view),
                                                  C.byref(o), C.byref(st)))
-        return U.view(np.complex128).reshape(self.nd, self.nd, self.nE), st
+        U = U.view(np.complex128).reshape(self.nd, self.nd, self.nE)
+        if neumann:
+            return U, st, a.view(np.complex128).reshape(self.nd, self.nd, self.nE, nterms)
+        return U, st
 
     def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False):
         """Y = (X .* G_sel) A for rows X (r, dim) complex."""

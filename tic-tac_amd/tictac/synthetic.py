@@ -10,7 +10,7 @@ import numpy as np
 
 DEFAULT_SEED = 7
 # C_1S0, C_3S1 [fm^2]; C_3P0, C_1P1, C_3P1, C_SD [fm^4]
-DEFAULT_CONTACTS = (-0.10, -0.165, 0.20, 0.15, 0.20, 0.02)
+DEFAULT_CONTACTS = (-0.10, -0.12653, 0.20, 0.15, 0.20, 0.02)
 THETA_NAMES = ("Y_0.7", "Y_2.0", "Y_4.0", "C_1S0", "C_3S1", "C_3P0", "C_1P1", "C_3P1", "C_SD",
                "lam_1S0", "lam_3S1", "lam_1P1")
 
