@@ -201,6 +201,12 @@ typedef enum {
     TT_SOLVE_MIXED = 3          /* FP32 products + FP64 iterative refinement                 */
 } tt_solve_method;
 
+/* How the product over P123 is formed (results agree to rounding). */
+typedef enum {
+    TT_P123_CELLS = 0,          /* dense Nalpha x Nalpha blocks per overlapping bin cell (default) */
+    TT_P123_CSC = 1             /* gather over the reference's CSC arrays                          */
+} tt_p123_format;
+
 typedef struct {
     int method;                 /* tt_solve_method                         */
     double tol;                 /* relative residual target (GMRES/MIXED)   */
@@ -211,6 +217,7 @@ typedef struct {
      * [(d_row*nd + d_col)*nE + j][n] for n < 2*NM_max+1; terms not computed stay 0.
      * The reference appends them to neumann_terms*.txt (CPP/solve_faddeev.cpp:907-917). */
     double *neumann_terms;
+    int p123_format;            /* tt_p123_format                           */
 } tt_solve_opts;
 void tt_solve_opts_default(tt_solve_opts *o);
 
@@ -244,11 +251,12 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
 /* Raw kernel-matrix application for tests and profiling: Y = (X .* G_sel) A in
  * the reference's row form (CPP/solve_faddeev.cpp:956-1088), r rows.
  * X, Y: host, r*dim complex (row-major).  g_index[k] selects the energy of row k
- * (-1: no G factor).  G: host nE*dim complex. */
+ * (-1: no G factor).  G: host nE*dim complex.  flags: bit 0 = FP32 products,
+ * bit 1 = gather over the CSC instead of the cell blocks. */
 tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123,
                         const double *V_unco, const double *V_coup, const tt_swp_states *swp,
                         const tt_pw_states *chn, const double *G, int nE, const int *g_index,
-                        int r, const double *X, double *Y, int fp32);
+                        int r, const double *X, double *Y, int flags);
 
 #ifdef __cplusplus
 }

@@ -91,6 +91,19 @@ def test_apply_c1(ctx, c1):
     assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
     Y32 = c1.ch.apply_rows(ctx, c1.P, X, G=c1.G, g_index=[0, 1, -1], fp32=True)
     assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
+    Ycsc = c1.ch.apply_rows(ctx, c1.P, X, G=c1.G, g_index=[0, 1, -1], csc=True)
+    assert np.abs(Ycsc - Yo).max() <= 1e-13 * np.abs(Yo).max()
+
+
+def test_uploaded_reference_csc_gives_same_solution(ctx, c1):
+    """A CSC supplied by the caller (the reference's format, here the oracle's P123) is
+    converted to cell blocks on first use and solves to the same U."""
+    rp = c1.prob.rp
+    ent, _ = pm.p123_rows(c1.och, c1.prob.p_edges, c1.prob.q_edges, rp.Nphi, rp.Nx)
+    vo, ro, po = pm.to_csc(ent, c1.ch.dim)
+    Pu = tt.P123.from_host(ctx, c1.ch.dim, vo, ro, po)
+    U, st = c1.ch.solve(ctx, Pu, c1.G, method=tt.SOLVE_GMRES)
+    assert np.abs(U - c1.Ud).max() <= 1e-10 * np.abs(c1.Ud).max()
 
 
 @pytest.mark.parametrize("method", [tt.SOLVE_DENSE, tt.SOLVE_GMRES, tt.SOLVE_MIXED])
@@ -160,6 +173,10 @@ def test_apply_c2(ctx, c2):
     Yo = c2.K.row_apply(X * Gsel)
     Y = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g)
     assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
+    Ycsc = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g, csc=True)
+    assert np.abs(Ycsc - Yo).max() <= 1e-13 * np.abs(Yo).max()
+    Y32 = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g, fp32=True)
+    assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
 
 
 def test_solves_c2_against_oracle_gmres(ctx, c2):

@@ -373,7 +373,16 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
     h2d(d_ptup.p, ang.pair_tuple, s);
     h2d(d_pat.p, ang.pair_atilde, s);
     const int64_t nval = ncell * Na * Na;
-    DBuf<double> d_V(std::max<int64_t>(nval, 1));
+    // the block array becomes the cell-block form of the result (owned by the DeviceCSC)
+    double *d_Vraw = nullptr;
+    TT_HIP(hipMalloc(&d_Vraw, sizeof(double) * (nval + BLK_SLACK)));
+    struct Own {
+        double *p;
+        ~Own() {
+            if (p) (void)hipFree(p);
+        }
+    } d_V{d_Vraw};
+    TT_HIP(hipMemsetAsync(d_V.p + nval, 0, sizeof(double) * BLK_SLACK, s));
     if (nval)
         hipLaunchKernelGGL(k_expand, dim3(nb(nval, 256)), dim3(256), 0, s, d_V.p, d_H.p, ncell, Na, T, d_pptr.p,
                            d_ptup.p, d_pat.p, d_cbra.p, d_cket.p, NqNp, production_run);
@@ -398,6 +407,17 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
         hipLaunchKernelGGL(k_fill, dim3(nb(dim, 256)), dim3(256), 0, s, csc->val, csc->row, csc->colptr, d_V.p,
                            d_ketptr.p, d_cbra.p, Na, NqNp);
         TT_HIP(hipGetLastError());
+        // cell-block form for the A-application
+        csc->Na = Na;
+        csc->NqNp = NqNp;
+        csc->ncell = ncell;
+        for (int k = 0; k < NqNp; ++k) csc->max_ket_cells = std::max(csc->max_ket_cells, ket_ptr[k + 1] - ket_ptr[k]);
+        TT_HIP(hipMalloc(&csc->ket_ptr, sizeof(int) * (NqNp + 1)));
+        TT_HIP(hipMalloc(&csc->cell_bra, sizeof(int) * std::max<int64_t>(ncell, 1)));
+        TT_HIP(hipMemcpyAsync(csc->ket_ptr, d_ketptr.p, sizeof(int) * (NqNp + 1), hipMemcpyDeviceToDevice, s));
+        if (ncell) TT_HIP(hipMemcpyAsync(csc->cell_bra, d_cbra.p, sizeof(int) * ncell, hipMemcpyDeviceToDevice, s));
+        csc->blk = d_V.p;
+        d_V.p = nullptr;
         TT_HIP(hipStreamSynchronize(s));
     } catch (...) {
         delete csc;

@@ -53,6 +53,70 @@ DeviceCSC::~DeviceCSC() {
     if (val32) (void)hipFree(val32);
     if (row) (void)hipFree(row);
     if (colptr) (void)hipFree(colptr);
+    if (ket_ptr) (void)hipFree(ket_ptr);
+    if (cell_bra) (void)hipFree(cell_bra);
+    if (blk) (void)hipFree(blk);
+    if (blk32) (void)hipFree(blk32);
+}
+
+// CSC -> cell blocks: column c = (alpha, ket), row = (alpha', bra); every (bra, ket) pair
+// that holds a nonzero becomes a cell.  One host pass over a device->host copy.
+void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s) {
+    const int NqNp = Nq * Np;
+    TT_REQUIRE((int64_t)Na * NqNp == P.dim, "P123 dimension does not match Nalpha*Nq*Np");
+    std::vector<double> val(P.nnz);
+    std::vector<int32_t> row(P.nnz);
+    std::vector<uint64_t> ptr(P.dim + 1);
+    if (P.nnz) {
+        TT_HIP(hipMemcpyAsync(val.data(), P.val, sizeof(double) * P.nnz, hipMemcpyDeviceToHost, s));
+        TT_HIP(hipMemcpyAsync(row.data(), P.row, sizeof(int32_t) * P.nnz, hipMemcpyDeviceToHost, s));
+    }
+    TT_HIP(hipMemcpyAsync(ptr.data(), P.colptr, sizeof(uint64_t) * (P.dim + 1), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    std::vector<int> ket_ptr(NqNp + 1, 0), cell_bra;
+    std::vector<int> mark(NqNp, -1);
+    for (int ket = 0; ket < NqNp; ++ket) {
+        std::vector<int> bras;
+        for (int a = 0; a < Na; ++a) {
+            const int64_t c = (int64_t)a * NqNp + ket;
+            for (uint64_t nz = ptr[c]; nz < ptr[c + 1]; ++nz) {
+                const int bra = row[nz] % NqNp;
+                if (mark[bra] != ket) {
+                    mark[bra] = ket;
+                    bras.push_back(bra);
+                }
+            }
+        }
+        std::sort(bras.begin(), bras.end());
+        cell_bra.insert(cell_bra.end(), bras.begin(), bras.end());
+        ket_ptr[ket + 1] = (int)cell_bra.size();
+    }
+    const int64_t ncell = (int64_t)cell_bra.size();
+    std::vector<double> blk((size_t)ncell * Na * Na, 0.0);
+    std::vector<int> pos(NqNp, -1);
+    for (int ket = 0; ket < NqNp; ++ket) {
+        for (int c = ket_ptr[ket]; c < ket_ptr[ket + 1]; ++c) pos[cell_bra[c]] = c;
+        for (int a = 0; a < Na; ++a) {
+            const int64_t col = (int64_t)a * NqNp + ket;
+            for (uint64_t nz = ptr[col]; nz < ptr[col + 1]; ++nz) {
+                const int ap = row[nz] / NqNp, bra = row[nz] % NqNp;
+                blk[((size_t)pos[bra] * Na + ap) * Na + a] = val[nz];
+            }
+        }
+    }
+    TT_HIP(hipMalloc(&P.ket_ptr, sizeof(int) * (NqNp + 1)));
+    TT_HIP(hipMalloc(&P.cell_bra, sizeof(int) * std::max<int64_t>(ncell, 1)));
+    blk.resize(blk.size() + BLK_SLACK, 0.0);
+    TT_HIP(hipMalloc(&P.blk, sizeof(double) * blk.size()));
+    TT_HIP(hipMemcpyAsync(P.ket_ptr, ket_ptr.data(), sizeof(int) * (NqNp + 1), hipMemcpyHostToDevice, s));
+    if (ncell) TT_HIP(hipMemcpyAsync(P.cell_bra, cell_bra.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, s));
+    TT_HIP(hipMemcpyAsync(P.blk, blk.data(), sizeof(double) * blk.size(), hipMemcpyHostToDevice, s));
+    TT_HIP(hipStreamSynchronize(s));
+    P.Na = Na;
+    P.NqNp = NqNp;
+    P.ncell = ncell;
+    P.max_ket_cells = 0;
+    for (int k = 0; k < NqNp; ++k) P.max_ket_cells = std::max(P.max_ket_cells, ket_ptr[k + 1] - ket_ptr[k]);
 }
 
 // ------------------------------------------------------------------ resolvent
@@ -102,23 +166,54 @@ __global__ void k_resolvent(double2 *__restrict__ G, const double *__restrict__ 
 
 // ------------------------------------------------------------------ block ops
 // Y[(a,q,p)][k] = sum_{e in part(a)} sum_p' B_e[p][p'] * Xg[(alpha_e,q,p')][k]   (or Z - that sum)
-// with Xg = X .* G_{gidx[k]} when G != nullptr.  One workgroup per output tile (a, q): the
-// partner rows Xg[(alpha_e, q, :)][:] are staged in LDS, then each thread forms outputs (p, k).
+// with Xg = X .* G_{gidx[k]} when G != nullptr.  One workgroup per output tile (a, q, k-chunk
+// of kc columns): the partner blocks B_e (coalesced) and the partner rows Xg[(alpha_e, q, :)]
+// are staged in LDS, then each thread forms outputs (p, k) from LDS only.
+// in_cell / out_cell select the cell-major memory order [(q, p)][alpha][k] of the cell-block
+// product for the input / output instead of the reference order [(alpha, q, p)][k].  With
+// ket_item_ptr the input is the array of work-item partial panels of the cell-block product
+// (cell-major per item) and Xg = in_scale * (sum of the ket's item panels, in item order).
 template <typename TI, typename TO>
 __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
                                                const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
                                                const double *__restrict__ blocks, const double2 *__restrict__ G,
-                                               const int *__restrict__ gidx, int Na, int Nq, int Np, int r,
-                                               const double2 *__restrict__ Zsub) {
-    extern __shared__ double2 xs[];  // [npart][Np][r]
+                                               const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
+                                               int ne_max, const double2 *__restrict__ Zsub, int in_cell,
+                                               int out_cell, const int *__restrict__ ket_item_ptr,
+                                               double in_scale) {
+    extern __shared__ double smem[];
     const int64_t dim = (int64_t)Na * Nq * Np;
     const int a = blockIdx.x / Nq, q = blockIdx.x % Nq;
+    const int k0 = blockIdx.y * kc, nk = min(kc, r - k0);
     const int e0 = part_ptr[a], ne = part_ptr[a + 1] - e0;
-    const int tile = Np * r;
+    const int NpNp = Np * Np;
+    double *Bs = smem;                                                        // ne * Np * Np
+    double2 *xs = reinterpret_cast<double2 *>(smem + ((ne_max * NpNp + 1) & ~1));  // ne * Np * nk
+    const double *Bg = blocks + (size_t)e0 * NpNp;
+    for (int i = threadIdx.x; i < ne * NpNp; i += blockDim.x) Bs[i] = Bg[i];
+    const int tile = Np * nk;
     for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
-        const int e = idx / tile, rem = idx % tile, pp = rem / r, k = rem % r;
-        const int64_t src = ((int64_t)part_alpha[e0 + e] * Nq + q) * Np + pp;
-        double xr = (double)X[src * r + k].x, xi = (double)X[src * r + k].y;
+        const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, k = k0 + rem - (rem / nk) * nk;
+        const int ae = part_alpha[e0 + e];
+        const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
+        double xr, xi;
+        if (ket_item_ptr) {
+            const int ket = q * Np + pp;
+            xr = 0.0;
+            xi = 0.0;
+            for (int it = ket_item_ptr[ket]; it < ket_item_ptr[ket + 1]; ++it) {
+                const TI v = X[((int64_t)it * Na + ae) * r + k];
+                xr += (double)v.x;
+                xi += (double)v.y;
+            }
+            xr *= in_scale;
+            xi *= in_scale;
+        } else {
+            const int64_t srcm = in_cell ? ((int64_t)q * Np + pp) * Na + ae : src;  // memory index of X
+            const TI v = X[srcm * r + k];
+            xr = (double)v.x;
+            xi = (double)v.y;
+        }
         if (G) {
             const int g = gidx[k];
             if (g >= 0) {
@@ -132,19 +227,20 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     }
     __syncthreads();
     for (int o = threadIdx.x; o < tile; o += blockDim.x) {
-        const int p = o / r, k = o % r;
+        const int p = o / nk, kk = o - p * nk, k = k0 + kk;
         double sr = 0.0, si = 0.0;
         for (int e = 0; e < ne; ++e) {
-            const double *B = blocks + ((size_t)(e0 + e) * Np + p) * Np;
-            const double2 *xe = xs + e * tile + k;
+            const double *B = Bs + (e * Np + p) * Np;
+            const double2 *xe = xs + e * tile + kk;
+#pragma unroll 6
             for (int pp = 0; pp < Np; ++pp) {
                 const double b = B[pp];
-                const double2 x = xe[pp * r];
+                const double2 x = xe[pp * nk];
                 sr += b * x.x;
                 si += b * x.y;
             }
         }
-        const int64_t dst = (((int64_t)a * Nq + q) * Np + p) * r + k;
+        const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
         if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
             sr = Zsub[dst].x - sr;
             si = Zsub[dst].y - si;
@@ -152,6 +248,20 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         Y[dst].x = sr;
         Y[dst].y = si;
     }
+}
+
+// Launch of k_block over all (a, q) and k-chunks of at most 16 columns.
+template <typename TI, typename TO>
+static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
+                         const double *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
+                         int r, const double2 *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
+                         double in_scale) {
+    const int kc = std::min(r, 16);
+    const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(double) + (size_t)ne_max * Np * kc * sizeof(double2);
+    TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
+    const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
+    hipLaunchKernelGGL((k_block<TI, TO>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na, Nq,
+                       Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale);
 }
 
 // ------------------------------------------------------------------ gather SpMM
@@ -207,6 +317,235 @@ __global__ __launch_bounds__(256) void k_spmm_gather(const uint64_t *__restrict_
             if (v0 + i < R2) U[col * R2 + v0 + i] = scale * acc[i];
 }
 
+// ------------------------------------------------------------------ cell-block product
+// U[ket][a][v] = scale * sum_{cells c of ket} sum_{a'} blk[c][a'][a] * S[bra(c)][a'][v]
+// (the same product as the gather SpMM: U = 2 P123^T S, in cell-major order).
+//
+// The blocks of the cells of one ket are contiguous, so for a run of n cells they form one
+// (n*Na) x Na row-major matrix Bk with rows kg = c*Na + a'.  The ket's panel is then a single
+// long-K product  U_ket (Na x R2) = Bk^T (Na x nNa) . Sk (nNa x R2)  with Sk[kg][v] =
+// S[bra(c)][a'][v] gathered row by row: K is concatenated over the cells, so no cell is padded.
+// It runs on the 16x16x4 MFMA (FP64 or FP32): M = a in MT tiles of 16, N = v in NT tiles of
+// 16, K = kg in steps of 4.  The M tiles interleave the columns: row i of tile m is
+// a = MT*i + m, so lane l = 16*kl + il needs the MT consecutive values Bk[kg][MT*il ...] for
+// its A operands and loads them with one wide load; the four lane groups read four
+// consecutive rows, i.e. one contiguous stretch of memory per k-step.  B[kl][il] =
+// Sk[kg][16n + il].  Everything goes straight from global memory into the operand registers
+// (no LDS staging: every block value is used exactly once).
+//
+// Workgroup = one work item, a run of at most CELL_ITEM cells of one ket (long kets are split
+// so that no workgroup dominates the launch).  Its K range is cut into batches of 4*UN rows;
+// wave w takes batches w, w+4, ..., loading the next batch while the MFMAs of the current one
+// run.  The four waves' accumulators are added in LDS in a fixed order, the items of a ket by
+// k_cell_reduce, so the result does not depend on timing.  blockIdx.y selects the v-tile.
+// Measured at Nalpha = 27, R2 = 8 on 22k cells: 16-cell items, batches of 2 k-steps (FP64)
+// or 4 (FP32) per wave.
+constexpr int CELL_ITEM = 16;
+template <typename T>
+constexpr int cell_un() {
+    return sizeof(T) == 8 ? 2 : 4;
+}
+
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of the 16x16 result held in accumulator register r of lane l (column = l % 16)
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T, int N>
+struct Run {
+    T v[N];
+};
+
+template <typename T, int MT, int NT, int UN>
+__global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
+                                                  const T *__restrict__ B, const T *__restrict__ S,
+                                                  T *__restrict__ Upart, int Na, int R2) {
+    typedef typename Mfma16<T>::acc_t acc_t;
+    __shared__ acc_t red[4][MT * NT][64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int kl = lane >> 4, il = lane & 15;
+    const int item = blockIdx.x;
+    const int v0 = blockIdx.y * 16 * NT;
+    const int c_beg = items[item].x, ncell = items[item].y - c_beg;
+    const int K = ncell * Na;
+    const T *Bk = B + (int64_t)c_beg * Na * Na;
+    // lane j < ncell (<= CELL_ITEM <= 64) holds the bra of cell j of the run
+    const int my_bra = lane < ncell ? cell_bra[c_beg + lane] : 0;
+    acc_t acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = acc_t{0, 0, 0, 0};
+
+    T a0[UN][MT], b0[UN][NT], a1[UN][MT], b1[UN][NT];
+    // Rows kg = base + 4u + kl of one batch.  Branch-free: every lane loads from a clamped,
+    // in-range address (finite data; the block array carries BLK_SLACK zeros past its end for
+    // the columns beyond Na of the last row).  Columns a >= Na of the block or v >= R2 of the
+    // panel only reach result rows / columns that are never stored, so they need no mask; rows
+    // kg >= K are zeroed in the panel operand when the batch is consumed, not when it is
+    // loaded, so that the loads of the next batch stay in flight during the MFMAs.  The cell
+    // of a row is (kg + 1/2) / Na in single precision, exact for kg < 4096 and Na <= 64 (the
+    // fractional part stays 1/(2 Na) away from an integer, far above the rounding).
+    const float rNa = 1.0f / (float)Na;
+    auto load_batch = [&](int base, T (&a)[UN][MT], T (&b)[UN][NT]) {
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const int kc = min(base + 4 * u + kl, K - 1);
+            const int c = (int)(((float)kc + 0.5f) * rNa);
+            const int ap = kc - c * Na;
+            const int bra = __shfl(my_bra, c);
+            const Run<T, MT> ra = *reinterpret_cast<const Run<T, MT> *>(Bk + kc * Na + MT * il);
+            const T *srow = S + ((int64_t)bra * Na + ap) * R2;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) a[u][m] = ra.v[m];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) b[u][n] = srow[min(v0 + 16 * n + il, R2 - 1)];
+        }
+    };
+    auto mma_batch = [&](int base, const T (&a)[UN][MT], const T (&b)[UN][NT]) {
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const T mk = (base + 4 * u + kl < K) ? (T)1 : (T)0;
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                const T bv = b[u][n] * mk;
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m][n] = Mfma16<T>::fma(a[u][m], bv, acc[m][n]);
+            }
+        }
+    };
+    // Wave w consumes batches w, w+4, ... (nbw of them, rounded up to even); two register
+    // buffers alternate (a copy would wait for the loads in flight).  Nothing in the loop is
+    // conditional: past the end the loads re-read the clamped last row and the row mask makes
+    // the MFMAs of a missing odd batch add zeros.  A load behind a branch would be sunk into
+    // it and the compiler's wait counts would assume the worst case at the join.  A scheduling
+    // barrier after each prefetch keeps its loads above the MFMAs of the batch in hand.
+    const int step = 4 * UN, stride = 4 * step;
+    int base = wave * step;
+    const int nbw = base < K ? (K - base + stride - 1) / stride : 0;
+    load_batch(base, a0, b0);
+    for (int i = 0; i < nbw; i += 2) {
+        load_batch(base + stride, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_batch(base, a0, b0);
+        load_batch(base + 2 * stride, a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_batch(base + stride, a1, b1);
+        base += 2 * stride;
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) red[wave][m * NT + n][lane] = acc[m][n];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                const acc_t t = ((red[0][m * NT + n][lane] + red[1][m * NT + n][lane]) + red[2][m * NT + n][lane]) +
+                                red[3][m * NT + n][lane];
+                const int v = v0 + 16 * n + il;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int a = MT * Mfma16<T>::row(lane, r) + m;
+                    if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = t[r];
+                }
+            }
+    }
+}
+
+// U[ket][a][v] = scale * sum of the partial panels of the ket's work items, in item order
+template <typename T>
+__global__ void k_cell_reduce(const int *__restrict__ ket_item_ptr, const T *__restrict__ Upart, T *__restrict__ U,
+                              int NqNp, int Na, int R2, T scale) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)Na * R2;
+    if (t >= (int64_t)NqNp * per) return;
+    const int ket = (int)(t / per);
+    const int64_t o = t - (int64_t)ket * per;
+    T acc = 0;
+    for (int i = ket_item_ptr[ket]; i < ket_item_ptr[ket + 1]; ++i) acc += Upart[(int64_t)i * per + o];
+    U[t] = scale * acc;
+}
+
+// Work items of the cell-block product: runs of at most item_len (<= 64) cells of one ket.
+struct CellItems {
+    int n = 0;
+    DevBuf<int2> items;
+    DevBuf<int> ket_item_ptr;
+    void build(const DeviceCSC &P, hipStream_t s, int item_len = CELL_ITEM) {
+        TT_REQUIRE(item_len >= 1 && item_len <= 64, "work item length must be in [1, 64]");
+        std::vector<int> kp(P.NqNp + 1);
+        TT_HIP(hipMemcpyAsync(kp.data(), P.ket_ptr, sizeof(int) * (P.NqNp + 1), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+        std::vector<int2> it;
+        std::vector<int> kip(P.NqNp + 1, 0);
+        for (int k = 0; k < P.NqNp; ++k) {
+            for (int c = kp[k]; c < kp[k + 1]; c += item_len) it.push_back(make_int2(c, std::min(c + item_len, kp[k + 1])));
+            kip[k + 1] = (int)it.size();
+        }
+        n = (int)it.size();
+        items.upload(it.data(), std::max<size_t>(it.size(), 1), s);
+        ket_item_ptr.upload(kip.data(), kip.size(), s);
+    }
+};
+
+template <typename T, int MT, int NT>
+static void launch_cellmfma(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S,
+                            T *Upart, int R2) {
+    const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
+    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, cell_un<T>()>), grid, block, 0, s, ci.items.p, P->cell_bra, blk, S, Upart,
+                       P->Na, R2);
+}
+
+template <typename T, int MT>
+static void launch_cellmfma_nt(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S,
+                               T *Upart, int R2) {
+    if (R2 <= 16)
+        launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
+    else
+        launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2);
+}
+
+// Upart = the work items' partial panels (items * Nalpha * R2); when U is given also
+// U = scale * (sum of each ket's item panels), the full product in cell-major order.
+template <typename T>
+static void launch_cellmm(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S, T *U,
+                          T *Upart, int R2, T scale) {
+    TT_REQUIRE(P->Na >= 1 && P->Na <= 64, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
+    if (ci.n > 0) {
+        if (P->Na <= 16)
+            launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2);
+        else if (P->Na <= 32)
+            launch_cellmfma_nt<T, 2>(s, P, ci, blk, S, Upart, R2);
+        else if (P->Na <= 48)
+            launch_cellmfma_nt<T, 3>(s, P, ci, blk, S, Upart, R2);
+        else
+            launch_cellmfma_nt<T, 4>(s, P, ci, blk, S, Upart, R2);
+    }
+    if (U) {
+        const int64_t nout = (int64_t)P->NqNp * P->Na * R2;
+        hipLaunchKernelGGL((k_cell_reduce<T>), dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s,
+                           ci.ket_item_ptr.p, Upart, U, P->NqNp, P->Na, R2, scale);
+    }
+}
+
 // Launch shape of the gather SpMM for R2 real components: NG value groups (a power of two)
 // of VPL <= 16 values each.
 struct SpmmShape {
@@ -250,8 +589,13 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, in
 }
 
 // ------------------------------------------------------------------ Krylov vector ops
-// partial[(i*nb + b)*r + k] = sum over block b's slice of conj(V_i[e]) * W[e] for column k.
-__global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, int nvec, const double2 *__restrict__ W,
+// partial[i][b][k] = sum over block b's slice of conj(V_i[e]) W[e] in column k.  The sum over
+// the blocks is a separate launch (k_dots_reduce): a device-scope fence per block, which a
+// single-launch "last block finishes" scheme needs, writes back the whole L2 of the block's
+// XCD and costs several times the dot products themselves.
+constexpr int DOT_UN = 8;
+
+__global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const double2 *__restrict__ W,
                        double2 *__restrict__ partial, int64_t n_elem, int r) {
     extern __shared__ double2 sh[];
     const int i = blockIdx.y;
@@ -262,54 +606,121 @@ __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, int nvec,
     const int sub = threadIdx.x / r;
     double sr = 0.0, si = 0.0;
     if (sub < tpr) {
-        const int64_t rows = n_elem / r;
-        for (int64_t e = (int64_t)b * tpr + sub; e < rows; e += (int64_t)nb * tpr) {
-            const double2 v = Vi[e * r + k], w = W[e * r + k];
-            sr += v.x * w.x + v.y * w.y;
-            si += v.x * w.y - v.y * w.x;
+        // DOT_UN independent loads per batch, all issued before use: clamped to the last row
+        // and masked, so no load sits behind a branch
+        const int64_t rows = n_elem / r, step = (int64_t)nb * tpr;
+        for (int64_t e = (int64_t)b * tpr + sub; e < rows; e += DOT_UN * step) {
+            double2 v[DOT_UN], w[DOT_UN];
+#pragma unroll
+            for (int u = 0; u < DOT_UN; ++u) {
+                const int64_t ee = min(e + u * step, rows - 1);
+                v[u] = Vi[ee * r + k];
+                w[u] = W[ee * r + k];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their use
+#pragma unroll
+            for (int u = 0; u < DOT_UN; ++u) {
+                const double mk = e + u * step < rows ? 1.0 : 0.0;
+                sr += mk * (v[u].x * w[u].x + v[u].y * w[u].y);
+                si += mk * (v[u].x * w[u].y - v[u].y * w[u].x);
+            }
         }
     }
     sh[threadIdx.x] = make_double2(sr, si);
     __syncthreads();
     if (threadIdx.x < r) {
         double ar = 0.0, ai = 0.0;
-        for (int s = 0; s < tpr; ++s) {
-            ar += sh[s * r + threadIdx.x].x;
-            ai += sh[s * r + threadIdx.x].y;
+        for (int s2 = 0; s2 < tpr; ++s2) {
+            ar += sh[s2 * r + threadIdx.x].x;
+            ai += sh[s2 * r + threadIdx.x].y;
         }
         partial[((int64_t)i * nb + b) * r + threadIdx.x] = make_double2(ar, ai);
     }
-    (void)nvec;
 }
 
-// W[e] -= sum_i h[i][k] V_i[e]
-__global__ void k_sub_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
-                            const double2 *__restrict__ h, int64_t n_elem, int r) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_elem) return;
-    const int k = (int)(t % r);
-    double2 w = W[t];
-    for (int i = 0; i < nvec; ++i) {
-        const double2 c = h[i * r + k], v = V[(int64_t)i * vstride + t];
-        w.x -= c.x * v.x - c.y * v.y;
-        w.y -= c.x * v.y + c.y * v.x;
+// out[i][k] = sum_b partial[i][b][k]; one workgroup per vector i.  G = 256 / r threads (a power
+// of two, at most 64, so a group is inside one wavefront) share one output: thread sub of a
+// group adds partials sub, sub + G, ... in order and the group combines them by an xor
+// butterfly, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_dots_reduce(const double2 *__restrict__ partial, int nb, int r,
+                                                     double2 *__restrict__ out) {
+    const int i = blockIdx.x;
+    int G = 1;
+    while (G < 64 && 2 * G * r <= (int)blockDim.x) G *= 2;
+    const int gsub = threadIdx.x % G;
+    for (int t0 = 0; t0 < r; t0 += (int)blockDim.x / G) {
+        const int kk = t0 + (int)threadIdx.x / G;
+        const bool act = kk < r;
+        const double2 *pp = partial + (int64_t)i * nb * r + (act ? kk : 0);
+        double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+        if (act) {
+            int bb = gsub;
+            for (; bb + 3 * G < nb; bb += 4 * G) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double2 v = pp[(int64_t)(bb + u * G) * r];
+                    ar[u] += v.x;
+                    ai[u] += v.y;
+                }
+            }
+            for (; bb < nb; bb += G) {
+                const double2 v = pp[(int64_t)bb * r];
+                ar[0] += v.x;
+                ai[0] += v.y;
+            }
+        }
+        double xr = (ar[0] + ar[1]) + (ar[2] + ar[3]), xi = (ai[0] + ai[1]) + (ai[2] + ai[3]);
+        for (int off = 1; off < G; off <<= 1) {
+            xr += __shfl_xor(xr, off);
+            xi += __shfl_xor(xi, off);
+        }
+        if (act && gsub == 0) out[i * r + kk] = make_double2(xr, xi);
     }
-    W[t] = w;
 }
 
-// X[e] += sum_i z[i][k] V_i[e]
-__global__ void k_add_combo(double2 *__restrict__ X, const double2 *__restrict__ V, int64_t vstride, int nvec,
-                            const double2 *__restrict__ z, int64_t n_elem, int r) {
+// W[e] -= sum_i h[i][k] V_i[e]   (sign = -1) or X[e] += sum_i z[i][k] V_i[e]   (sign = +1).
+// The vectors are taken in batches of COMBO_UN with all loads issued before use (clamped and
+// masked past nvec), so each thread keeps COMBO_UN loads in flight.  With norm_partial the
+// block also writes the squared norms of its slice of the result per column k (blockDim a
+// multiple of r), summed over the blocks by k_dots_reduce.
+constexpr int COMBO_UN = 8;
+__global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
+                        const double2 *__restrict__ h, int64_t n_elem, int r, double sign,
+                        double2 *__restrict__ norm_partial) {
+    extern __shared__ double nsh[];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_elem) return;
+    const bool act = t < n_elem;
     const int k = (int)(t % r);
-    double2 x = X[t];
-    for (int i = 0; i < nvec; ++i) {
-        const double2 c = z[i * r + k], v = V[(int64_t)i * vstride + t];
-        x.x += c.x * v.x - c.y * v.y;
-        x.y += c.x * v.y + c.y * v.x;
+    double2 w = make_double2(0.0, 0.0);
+    if (act) {
+        w = W[t];
+        for (int i0 = 0; i0 < nvec; i0 += COMBO_UN) {
+            double2 c[COMBO_UN], v[COMBO_UN];
+#pragma unroll
+            for (int u = 0; u < COMBO_UN; ++u) {
+                const int i = min(i0 + u, nvec - 1);
+                c[u] = h[i * r + k];
+                v[u] = V[(int64_t)i * vstride + t];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their use
+#pragma unroll
+            for (int u = 0; u < COMBO_UN; ++u) {
+                const double f = i0 + u < nvec ? sign : 0.0;
+                w.x += f * (c[u].x * v[u].x - c[u].y * v[u].y);
+                w.y += f * (c[u].x * v[u].y + c[u].y * v[u].x);
+            }
+        }
+        W[t] = w;
     }
-    X[t] = x;
+    if (norm_partial) {
+        nsh[threadIdx.x] = w.x * w.x + w.y * w.y;
+        __syncthreads();
+        if ((int)threadIdx.x < r) {
+            double acc = 0.0;
+            for (int q = (int)threadIdx.x; q < (int)blockDim.x; q += r) acc += nsh[q];
+            norm_partial[(int64_t)blockIdx.x * r + threadIdx.x] = make_double2(acc, 0.0);
+        }
+    }
 }
 
 __global__ void k_axpby(double2 *__restrict__ Y, const double2 *__restrict__ A, const double2 *__restrict__ B,
@@ -331,21 +742,6 @@ __global__ void k_gather(const double2 *__restrict__ X, const int64_t *__restric
 __global__ void k_unit_rows(double2 *__restrict__ X, const int64_t *__restrict__ dst_idx, int r) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < r) X[dst_idx[t] * r + t] = make_double2(1.0, 0.0);
-}
-
-// out[i][k] = sum_b partial[i][b][k], fixed summation order
-__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nvec, int nb, int r,
-                                  double2 *__restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nvec * r) return;
-    const int i = t / r, k = t % r;
-    double ar = 0.0, ai = 0.0;
-    for (int b = 0; b < nb; ++b) {
-        const double2 v = partial[((int64_t)i * nb + b) * r + k];
-        ar += v.x;
-        ai += v.y;
-    }
-    out[t] = make_double2(ar, ai);
 }
 
 // Y[e] = X[e] / sqrt(n2[k].x), zero where the squared norm is zero
@@ -383,6 +779,9 @@ struct RowOperator {
     int n_apply = 0, n_apply32 = 0, n_spmm = 0;
     int r_cap = 0;
     int c_npart_max = 1, vc_npart_max = 1;
+    bool cells = true;   // cell-block product (default) or gather over the CSC
+    CellItems items;
+    DevBuf<double> upart;  // partial panels of the work items (FP64 and FP32 share it)
 
     ~RowOperator() {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
@@ -441,8 +840,10 @@ struct RowOperator {
     }
 
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
-               const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, const double *Gh, int nE_) {
+               const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, const double *Gh, int nE_,
+               int p123_format = TT_P123_CELLS) {
         s = stream;
+        cells = p123_format == TT_P123_CELLS;
         Na = chn.Nalpha;
         Nq = swp.Nq_WP;
         Np = swp.Np_WP;
@@ -450,6 +851,9 @@ struct RowOperator {
         dim = (int64_t)Na * Nq * Np;
         P = csc;
         TT_REQUIRE(csc && csc->dim == dim, "P123 dimension does not match Nalpha*Nq*Np");
+        if (cells && csc->Na == 0) build_cell_blocks(*const_cast<DeviceCSC *>(csc), Na, Nq, Np, s);
+        TT_REQUIRE(!cells || (csc->Na == Na && csc->NqNp == Nq * Np), "P123 cell blocks do not match the channel");
+        if (cells) items.build(*csc, s);
         int nu, nc;
         num_2N_states(rp, nu, nc);
         // V.C per 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array, lines 1828-1855)
@@ -509,10 +913,25 @@ struct RowOperator {
 
     void ensure_f32() {
         DeviceCSC *m = const_cast<DeviceCSC *>(P);
-        if (!m->val32) {
+        if (cells) {
+            if (!m->blk32) {
+                const int64_t n = m->ncell * m->Na * m->Na;
+                TT_HIP(hipMalloc(&m->blk32, sizeof(float) * (n + BLK_SLACK)));
+                hipLaunchKernelGGL(k_to_f32, dim3(nblk(n + BLK_SLACK, 256)), dim3(256), 0, s, m->blk, m->blk32,
+                                   n + BLK_SLACK);
+            }
+        } else if (!m->val32) {
             TT_HIP(hipMalloc(&m->val32, sizeof(float) * std::max<int64_t>(m->nnz, 1)));
             if (m->nnz) hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz, 256)), dim3(256), 0, s, m->val, m->val32, m->nnz);
         }
+    }
+
+    // Bytes the product over P123 must move per launch (one pass over P, S read and U written once).
+    double product_bytes(int r, bool fp32) const {
+        const double sv = fp32 ? 4.0 : 8.0, sc = fp32 ? 8.0 : 16.0;
+        const double vec = 2.0 * (double)dim * r * sc;
+        if (cells) return (double)P->ncell * P->Na * P->Na * sv + 4.0 * (double)(P->ncell + P->NqNp + 1) + vec;
+        return (double)P->nnz * (sv + 4.0) + 8.0 * (double)(dim + 1) + vec;
     }
 
     // Y = (X .* G_{gidx}) A ; X, Y device, dim*r complex; use_G selects whether G multiplies X.
@@ -530,37 +949,48 @@ struct RowOperator {
             TT_HIP(hipEventRecord(e0, s));
         }
         const int R2 = 2 * r;
-        const double sv = fp32 ? 4.0 : 8.0, sc = fp32 ? 8.0 : 16.0;
-        const dim3 bgrid((unsigned)(Na * Nq)), bblk(256);
-        const size_t sh_c = (size_t)c_npart_max * Np * r * sizeof(double2);
-        const size_t sh_vc = (size_t)vc_npart_max * Np * r * sizeof(double2);
-        TT_REQUIRE(sh_c <= 160 * 1024 && sh_vc <= 160 * 1024, "block-op tile exceeds LDS (too many right-hand sides)");
+        const int cm = cells ? 1 : 0;
+        if (cells && upart.n < (size_t)items.n * Na * R2) upart.alloc((size_t)std::max(items.n, 1) * Na * R2);
+        // cell path: S -> work-item partial panels; the item sum and the factor 2 are folded
+        // into the input staging of the V.C block op
+        const int *kip = cells ? items.ket_item_ptr.p : nullptr;
+        const double in_scale = cells ? 2.0 : 1.0;
         if (!fp32) {
-            hipLaunchKernelGGL((k_block<double2, double2>), bgrid, bblk, sh_c, s, X, S.p, c_ptr.p, c_alpha.p,
-                               c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr);
+            launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, use_G ? G.p : nullptr,
+                                           gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
             if (timed) TT_HIP(hipEventRecord(e1, s));
-            launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p), reinterpret_cast<double *>(U.p),
-                                dim, R2, 2.0);
+            if (cells)
+                launch_cellmm<double>(s, P, items, P->blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
+                                      2.0);
+            else
+                launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
+                                    reinterpret_cast<double *>(U.p), dim, R2, 2.0);
             if (timed) TT_HIP(hipEventRecord(e2, s));
-            hipLaunchKernelGGL((k_block<double2, double2>), bgrid, bblk, sh_vc, s, U.p, Y, vc_ptr.p, vc_alpha.p,
-                               vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r, Zsub);
+            const double2 *Uin = cells ? reinterpret_cast<const double2 *>(upart.p) : U.p;
+            launch_block<double2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
+                                           Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
             ++n_apply;
         } else {
             ensure_f32();
-            hipLaunchKernelGGL((k_block<double2, float2>), bgrid, bblk, sh_c, s, X, S32.p, c_ptr.p, c_alpha.p,
-                               c_blocks.p, use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr);
+            launch_block<double2, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
+                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
             if (timed) TT_HIP(hipEventRecord(e1, s));
-            launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p), reinterpret_cast<float *>(U32.p),
-                               dim, R2, 2.0f);
+            if (cells)
+                launch_cellmm<float>(s, P, items, P->blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                                     reinterpret_cast<float *>(upart.p), R2, 2.0f);
+            else
+                launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
+                                   reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
             if (timed) TT_HIP(hipEventRecord(e2, s));
-            hipLaunchKernelGGL((k_block<float2, double2>), bgrid, bblk, sh_vc, s, U32.p, Y, vc_ptr.p, vc_alpha.p,
-                               vc_blocks.p, nullptr, gidx.p, Na, Nq, Np, r, Zsub);
+            const float2 *Uin = cells ? reinterpret_cast<const float2 *>(upart.p) : U32.p;
+            launch_block<float2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
+                                          Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
             ++n_apply;
             ++n_apply32;
         }
         if (timed) {
             TT_HIP(hipEventRecord(e3, s));
-            spmm_bytes_of.push_back((double)P->nnz * (sv + 4.0) + 8.0 * (double)(dim + 1) + 2.0 * (double)dim * r * sc);
+            spmm_bytes_of.push_back(product_bytes(r, fp32));
         }
     }
 };
@@ -615,7 +1045,8 @@ struct Gmres {
     RowOperator &op;
     int r, m;
     int64_t ne;
-    int nb = 64;
+    // blocks per vector of k_dots: each thread takes about one batch of DOT_UN rows
+    int nb_dots = 1;
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
     DevBuf<double2> V, W, R, zdev, partial, Hd, n2d;
     double2 *Hh = nullptr;              // pinned mirror of Hd
@@ -629,7 +1060,9 @@ struct Gmres {
         W.alloc(ne);
         R.alloc(ne);
         zdev.alloc((size_t)(m + 1) * r);
-        partial.alloc((size_t)(m + 1) * nb * r);
+        const int tpr = (256 - 256 % r) / r;
+        nb_dots = (int)std::min<int64_t>(512, std::max<int64_t>(1, (o.dim + (int64_t)tpr * DOT_UN - 1) / ((int64_t)tpr * DOT_UN)));
+        partial.alloc(std::max<size_t>((size_t)nb_dots * (m + 1) * r, (size_t)nblk(ne, 256 - 256 % r) * r));
         Hd.alloc((size_t)m * slot_len);
         n2d.alloc(r);
         TT_HIP(hipHostMalloc(&Hh, sizeof(double2) * (size_t)m * slot_len, hipHostMallocDefault));
@@ -644,10 +1077,9 @@ struct Gmres {
     // out[i][k] = <Vb_i, Wv>_k for i < nvec, on the device
     void dots_dev(const double2 *Vb, int nvec, const double2 *Wv, double2 *out) {
         int bs = 256 - 256 % r;
-        hipLaunchKernelGGL(k_dots, dim3(nb, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, nvec, Wv, partial.p,
-                           ne, r);
-        hipLaunchKernelGGL(k_reduce_partials, dim3(nblk((int64_t)nvec * r, 64)), dim3(64), 0, op.s, partial.p, nvec,
-                           nb, r, out);
+        hipLaunchKernelGGL(k_dots, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, partial.p, ne,
+                           r);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(nvec), dim3(256), 0, op.s, partial.p, nb_dots, r, out);
     }
 
     std::vector<double> norms(const double2 *Wv) {
@@ -669,10 +1101,14 @@ struct Gmres {
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
         apply_M(vj, W.p, fp32);
         dots_dev(V.p, j + 1, W.p, h1);
-        hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r);
+        hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r, -1.0,
+                           nullptr);
         dots_dev(V.p, j + 1, W.p, h2);
-        hipLaunchKernelGGL(k_sub_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h2, ne, r);
-        dots_dev(W.p, 1, W.p, n2);
+        const int bs = 256 - 256 % r;  // the norm partials need whole columns per block
+        const unsigned nbc = nblk(ne, bs);
+        hipLaunchKernelGGL(k_combo, dim3(nbc), dim3(bs), bs * sizeof(double), op.s, W.p, V.p, ne, j + 1, h2, ne, r,
+                           -1.0, partial.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(256), 0, op.s, partial.p, (int)nbc, r, n2);
         hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, n2, ne, r);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
         TT_HIP(hipEventRecord(step_ev[j], op.s));
@@ -751,7 +1187,8 @@ struct Gmres {
             }
             TT_HIP(hipStreamSynchronize(op.s));  // a speculative step may still use zdev's neighbours
             zdev.upload(reinterpret_cast<const double2 *>(z.data()), z.size(), op.s);
-            hipLaunchKernelGGL(k_add_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r);
+            hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r, 1.0,
+                               nullptr);
         }
         res_out = res_max;
         return it_total;
@@ -1043,6 +1480,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->restart = 40;
     o->NM_max = 14;
     o->neumann_terms = nullptr;
+    o->p123_format = TT_P123_CELLS;
 }
 
 tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
@@ -1150,7 +1588,7 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
     tt_solve_stats st;
     std::memset(&st, 0, sizeof(st));
     RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, os->num_T_lab);
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, os->num_T_lab, o.p123_format);
     std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
     switch (o.method) {
         case TT_SOLVE_PADE:
@@ -1187,12 +1625,13 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
 
 tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123, const double *V_unco,
                         const double *V_coup, const tt_swp_states *swp, const tt_pw_states *chn, const double *G,
-                        int nE, const int *g_index, int r, const double *X, double *Y, int fp32) {
+                        int nE, const int *g_index, int r, const double *X, double *Y, int flags) {
     TT_GUARD_BEGIN
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
     TT_HIP(hipSetDevice(ctx->device));
     RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, nE);
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, nE,
+             (flags & 2) ? TT_P123_CSC : TT_P123_CELLS);
     std::vector<int> g(r, -1);
     bool useG = false;
     if (g_index)
@@ -1210,7 +1649,7 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
         for (int64_t i = 0; i < dim; ++i) xt[(size_t)i * r + k] = make_double2(X[2 * ((size_t)k * dim + i)], X[2 * ((size_t)k * dim + i) + 1]);
     DevBuf<double2> dx, dy((size_t)dim * r);
     dx.upload(xt.data(), xt.size(), ctx->stream);
-    op.apply(dx.p, dy.p, r, useG, fp32 != 0);
+    op.apply(dx.p, dy.p, r, useG, (flags & 1) != 0);
     TT_HIP(hipMemcpyAsync(yt.data(), dy.p, sizeof(double2) * yt.size(), hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < r; ++k)

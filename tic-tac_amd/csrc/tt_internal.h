@@ -93,14 +93,33 @@ struct P123Angular {
 P123Angular build_p123_angular(const tt_pw_states &chn);
 
 // ------------------------------------------------------------ library ctx
+// P123 on the device: the reference's CSC (the boundary format) and the cell-block form
+// the A-application uses.  Every nonzero of P123 lies on an overlapping bin cell
+// (bra (q',p'), ket (q,p)); on each cell the values form a dense Nalpha x Nalpha block
+// blk[cell][alpha'][alpha] (structural zeros stored).  Cells are ordered ket-major,
+// bras ascending: cells of ket k are ket_ptr[k] .. ket_ptr[k+1]-1.
+// Zeroed elements allocated past the last cell of a block array: the MFMA product loads
+// 16 * ceil(Na / 16) values from the start of a block row and so runs up to 15 past the end.
+constexpr int64_t BLK_SLACK = 64;
+
 struct DeviceCSC {
     int64_t dim = 0, nnz = 0;
     double *val = nullptr;
     float *val32 = nullptr;          // lazily made FP32 copy for the mixed path
     int32_t *row = nullptr;
     uint64_t *colptr = nullptr;
+    // cell-block form (Na == 0 until built)
+    int Na = 0, NqNp = 0;
+    int64_t ncell = 0;
+    int max_ket_cells = 0;           // most cells of any one ket
+    int *ket_ptr = nullptr;          // NqNp + 1
+    int *cell_bra = nullptr;         // ncell
+    double *blk = nullptr;           // ncell * Na * Na + BLK_SLACK (zeroed slack)
+    float *blk32 = nullptr;          // lazily made FP32 copy, same length
     ~DeviceCSC();
 };
+// Builds the cell-block form of a CSC P123 (host pass over a device->host copy).
+void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s);
 
 }  // namespace tt
 

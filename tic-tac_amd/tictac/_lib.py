@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libtictac.so")
 
 TT_OK, TT_ERR_ARG, TT_ERR_HIP, TT_ERR_ALLOC, TT_ERR_PHYSICS, TT_ERR_NOT_CONVERGED, TT_ERR_UNSUPPORTED = range(7)
 SOLVE_PADE, SOLVE_DENSE, SOLVE_GMRES, SOLVE_MIXED = range(4)
+P123_CELLS, P123_CSC = range(2)
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -60,7 +61,7 @@ class P123Stats(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int),
-                ("NM_max", C.c_int), ("neumann_terms", _dp)]
+                ("NM_max", C.c_int), ("neumann_terms", _dp), ("p123_format", C.c_int)]
 
 
 class SolveStats(C.Structure):

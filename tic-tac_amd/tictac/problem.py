@@ -231,7 +231,7 @@ class Channel:
         return G.view(np.complex128).reshape(len(E), self.dim)
 
     def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
-              neumann=False):
+              neumann=False, p123_format=L.P123_CELLS):
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
 
         With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n]."""
@@ -242,6 +242,7 @@ class Channel:
         o = L.SolveOpts()
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
+        o.p123_format = p123_format
         nterms = 2 * NM_max + 1
         a = np.zeros(self.nd * self.nd * self.nE * nterms * 2)
         if neumann:
@@ -258,7 +259,7 @@ class Channel:
             return U, st, a.view(np.complex128).reshape(self.nd, self.nd, self.nE, nterms)
         return U, st
 
-    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False):
+    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False, csc=False):
         """Y = (X .* G_sel) A for rows X (r, dim) complex."""
         X = np.ascontiguousarray(X, np.complex128)
         r = X.shape[0]
@@ -271,7 +272,7 @@ class Channel:
         check(L.lib().tt_apply_rows(ctx.handle, C.byref(self.prob.rp), C.byref(P.csc), dptr(self.prob.V_unco),
                                     dptr(self.prob.V_coup), C.byref(self.prob.swp), C.byref(self.view),
                                     dptr(Gf), nE, iptr(gi) if gi is not None else None, r,
-                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)), int(fp32)))
+                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)), int(fp32) | (2 if csc else 0)))
         return Y
 
     def __del__(self):
