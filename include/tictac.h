@@ -218,6 +218,11 @@ typedef struct {
      * The reference appends them to neumann_terms*.txt (CPP/solve_faddeev.cpp:907-917). */
     double *neumann_terms;
     int p123_format;            /* tt_p123_format                           */
+    int use_graphs;             /* 1: each Arnoldi step is captured once as a HIP graph and
+                                 * replayed (default); 0: plain stream launches.  The context
+                                 * keeps the operator and Krylov workspace of recent channels
+                                 * (keyed by the P123 object and a hash of V, C and the channel),
+                                 * so repeated solves of a channel only upload G. */
 } tt_solve_opts;
 void tt_solve_opts_default(tt_solve_opts *o);
 
@@ -228,10 +233,12 @@ typedef struct {
     double residual;            /* final relative residual (max over systems) */
     double ms_total;
     double ms_apply;            /* device time inside A applications        */
-    /* The gather SpMM over P123 (dominant kernel): device time summed over its
-     * launches and the algorithmic bytes those launches must move:
-     * nnz*(s_v+4) + 8*(dim+1) + 2*dim*r*s_c per launch (s_v, s_c: value and
-     * complex-vector element sizes of the precision used). */
+    /* The product over P123 (dominant kernel: the cell-block MFMA product, or the
+     * gather over the CSC): device time summed over its launches (HIP events on the
+     * solve stream around each launch) and the algorithmic bytes those launches must
+     * move: per launch ncell*Na^2*s_v + 4*(ncell + NqNp + 1) + 2*dim*r*s_c for the
+     * cell blocks, nnz*(s_v+4) + 8*(dim+1) + 2*dim*r*s_c for the CSC (s_v, s_c:
+     * value and complex-vector element sizes of the precision used). */
     int n_spmm;
     double ms_spmm;
     double spmm_bytes;

@@ -203,6 +203,44 @@ def test_solves_c2_against_oracle_gmres(ctx, c2):
     assert np.abs(Upa - U64).max() <= 1e-5 * np.abs(U64).max()
 
 
+def test_graph_replays_match_eager_launches(ctx, c2):
+    """Each Arnoldi step is captured once as a HIP graph and replayed by later solves of the
+    channel (the context keeps the channel's plan).  Every replay, with eager work and
+    synchronisation in between, gives the result of plain stream launches."""
+    ch = c2.ch
+    for method in (tt.SOLVE_GMRES, tt.SOLVE_MIXED):
+        U_eager, st_e = ch.solve(ctx, c2.P, c2.G, method=method, graphs=False)
+        for rep in range(3):
+            G = ch.resolvent(ctx)   # eager work between the replays
+            U_g, st_g = ch.solve(ctx, c2.P, G, method=method, graphs=True)
+            assert st_g.iterations == st_e.iterations
+            assert np.abs(U_g - U_eager).max() <= 1e-14 * np.abs(U_eager).max(), (method, rep)
+            assert st_g.n_spmm <= st_g.n_apply
+
+
+def test_plan_follows_potential_and_p123_changes(ctx):
+    """A cached plan is reused only for the same P123 object and the same V, C and channel:
+    changing the potential strengths on the same P123, or rebuilding P123, gives the
+    oracle's result for the new inputs."""
+    rp, pot, T = synthetic_config(N=12, J_2N_max=1)
+    th = tt.synthetic_params()
+    for scale, rebuild in ((1.0, False), (1.05, False), (1.05, True), (1.0, False)):
+        p = th.copy()
+        p[:12] *= scale
+        prob = tt.Problem(rp, tt.Potential("synthetic", p))
+        ch = prob.channel(0, T)
+        if rebuild or 'P' not in locals():
+            P = ch.build_p123(ctx)
+        G = ch.resolvent(ctx)
+        U, st = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES)
+        val, row, ptr = P.to_host()
+        tf, ib = flags(prob)
+        Vu, Vc = oracle_V(prob)
+        K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
+        Uo = so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, rp.Nq_WP, rp.Np_WP)
+        assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max(), (scale, rebuild)
+
+
 def test_error_paths_on_device(ctx, c1):
     with pytest.raises(tt.TictacError) as e:
         tt.P123.from_host(ctx, 4, np.ones(2), np.array([0, 9], np.int32), np.array([0, 1, 2, 2, 2], np.uint64))

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -47,6 +48,11 @@ struct DevBuf {
         if (count) TT_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
     }
 };
+
+uint64_t DeviceCSC::next_gen() {
+    static std::atomic<uint64_t> counter{1};
+    return counter.fetch_add(1);
+}
 
 DeviceCSC::~DeviceCSC() {
     if (val) (void)hipFree(val);
@@ -782,6 +788,9 @@ struct RowOperator {
     bool cells = true;   // cell-block product (default) or gather over the CSC
     CellItems items;
     DevBuf<double> upart;  // partial panels of the work items (FP64 and FP32 share it)
+    // bumped whenever a buffer the kernels address is (re)allocated: captured graphs hold
+    // these addresses and are rebuilt when it changes
+    uint64_t layout_gen = 0;
 
     ~RowOperator() {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
@@ -840,14 +849,14 @@ struct RowOperator {
     }
 
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
-               const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, const double *Gh, int nE_,
+               const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn,
                int p123_format = TT_P123_CELLS) {
         s = stream;
         cells = p123_format == TT_P123_CELLS;
         Na = chn.Nalpha;
         Nq = swp.Nq_WP;
         Np = swp.Np_WP;
-        nE = nE_;
+        nE = 0;
         dim = (int64_t)Na * Nq * Np;
         P = csc;
         TT_REQUIRE(csc && csc->dim == dim, "P123 dimension does not match Nalpha*Nq*Np");
@@ -888,7 +897,15 @@ struct RowOperator {
         vc_ptr.upload(p2.data(), p2.size(), s);
         vc_alpha.upload(a2.data(), std::max<size_t>(a2.size(), 1), s);
         vc_blocks.upload(b2.data(), std::max<size_t>(b2.size(), 1), s);
-        if (nE > 0 && Gh) G.upload(reinterpret_cast<const double2 *>(Gh), (size_t)nE * dim, s);
+    }
+
+    // Resolvent of nE energies (host, [j][idx] complex), copied into the device buffer.
+    void set_G(const double *Gh, int nE_) {
+        nE = nE_;
+        if (nE <= 0 || !Gh) return;
+        const size_t n = (size_t)nE * dim;
+        if (n > G.n) ++layout_gen;
+        G.upload(reinterpret_cast<const double2 *>(Gh), n, s);
     }
 
     void reserve(int r, bool fp32) {
@@ -899,10 +916,24 @@ struct RowOperator {
             r_cap = r;
             S32.release();
             U32.release();
+            ++layout_gen;
         }
         if (fp32 && !S32.p) {
             S32.alloc((size_t)dim * r_cap);
             U32.alloc((size_t)dim * r_cap);
+            ++layout_gen;
+        }
+    }
+
+    // Every allocation an application with r rows needs, done ahead (graph capture may not
+    // allocate).
+    void prepare(int r, bool fp32) {
+        reserve(r, fp32);
+        if (fp32) ensure_f32();
+        const size_t need = (size_t)std::max(items.n, 1) * Na * 2 * r;
+        if (cells && upart.n < need) {
+            upart.alloc(need);
+            ++layout_gen;
         }
     }
 
@@ -936,21 +967,36 @@ struct RowOperator {
 
     // Y = (X .* G_{gidx}) A ; X, Y device, dim*r complex; use_G selects whether G multiplies X.
     // Y = (X .* G_{gidx}) A, or Y = Zsub - (X .* G_{gidx}) A when Zsub is given.
+    // With ext (four events), the timing events are the caller's (a captured graph records
+    // them on every replay and its owner reads them with add_timing); otherwise timed
+    // applications take four events of the pool, read back in collect().
     void apply(const double2 *X, double2 *Y, int r, bool use_G, bool fp32, bool timed = true,
-               const double2 *Zsub = nullptr) {
-        reserve(r, fp32);
-        if (timed && ev_used + 4 > 4096) collect();
+               const double2 *Zsub = nullptr, const hipEvent_t *ext = nullptr) {
+        prepare(r, fp32);
+        if (!ext && timed && ev_used + 4 > 4096) collect();
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-        if (timed) {
+        if (ext) {
+            e0 = ext[0];
+            e1 = ext[1];
+            e2 = ext[2];
+            e3 = ext[3];
+            timed = true;
+        } else if (timed) {
             e0 = next_event();
             e1 = next_event();
             e2 = next_event();
             e3 = next_event();
-            TT_HIP(hipEventRecord(e0, s));
         }
+        // caller events are recorded as external event nodes when the stream is being captured
+        auto record = [&](hipEvent_t e) {
+            if (ext)
+                TT_HIP(hipEventRecordWithFlags(e, s, hipEventRecordExternal));
+            else
+                TT_HIP(hipEventRecord(e, s));
+        };
+        if (timed) record(e0);
         const int R2 = 2 * r;
         const int cm = cells ? 1 : 0;
-        if (cells && upart.n < (size_t)items.n * Na * R2) upart.alloc((size_t)std::max(items.n, 1) * Na * R2);
         // cell path: S -> work-item partial panels; the item sum and the factor 2 are folded
         // into the input staging of the V.C block op
         const int *kip = cells ? items.ket_item_ptr.p : nullptr;
@@ -958,14 +1004,14 @@ struct RowOperator {
         if (!fp32) {
             launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, use_G ? G.p : nullptr,
                                            gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
-            if (timed) TT_HIP(hipEventRecord(e1, s));
+            if (timed) record(e1);
             if (cells)
                 launch_cellmm<double>(s, P, items, P->blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
                                       2.0);
             else
                 launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
                                     reinterpret_cast<double *>(U.p), dim, R2, 2.0);
-            if (timed) TT_HIP(hipEventRecord(e2, s));
+            if (timed) record(e2);
             const double2 *Uin = cells ? reinterpret_cast<const double2 *>(upart.p) : U.p;
             launch_block<double2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
                                            Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
@@ -974,14 +1020,14 @@ struct RowOperator {
             ensure_f32();
             launch_block<double2, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
                                           use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
-            if (timed) TT_HIP(hipEventRecord(e1, s));
+            if (timed) record(e1);
             if (cells)
                 launch_cellmm<float>(s, P, items, P->blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                      reinterpret_cast<float *>(upart.p), R2, 2.0f);
             else
                 launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
                                    reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
-            if (timed) TT_HIP(hipEventRecord(e2, s));
+            if (timed) record(e2);
             const float2 *Uin = cells ? reinterpret_cast<const float2 *>(upart.p) : U32.p;
             launch_block<float2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
                                           Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
@@ -989,9 +1035,24 @@ struct RowOperator {
             ++n_apply32;
         }
         if (timed) {
-            TT_HIP(hipEventRecord(e3, s));
-            spmm_bytes_of.push_back(product_bytes(r, fp32));
+            record(e3);
+            if (!ext) spmm_bytes_of.push_back(product_bytes(r, fp32));
         }
+    }
+
+    // Adds the timing of one application recorded into caller events (after they completed).
+    // Returns false, adding nothing, where the runtime keeps no timestamps for them.
+    bool add_timing(const hipEvent_t *ev, double bytes) {
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, ev[0], ev[3]) != hipSuccess || hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        ms_apply += a;
+        ms_spmm += b;
+        spmm_bytes += bytes;
+        ++n_spmm;
+        return true;
     }
 };
 
@@ -1041,18 +1102,27 @@ static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
 // step (operator, two classical Gram-Schmidt passes, normalisation) runs entirely on the
 // device; its Hessenberg column is copied to pinned host memory and consumed by the host's
 // Givens rotations one step later, so the host never stalls the stream inside a cycle.
+// With use_graphs each step (j, precision) is captured once into a HIP graph and replayed
+// afterwards: one launch per step instead of about a dozen.
 struct Gmres {
     RowOperator &op;
     int r, m;
     int64_t ne;
+    bool use_graphs;
     // blocks per vector of k_dots: each thread takes about one batch of DOT_UN rows
     int nb_dots = 1;
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
     DevBuf<double2> V, W, R, zdev, partial, Hd, n2d;
     double2 *Hh = nullptr;              // pinned mirror of Hd
     std::vector<hipEvent_t> step_ev;
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint64_t gen = 0;
+    };
+    std::vector<StepGraph> graphs[2];   // [fp32][j]
 
-    Gmres(RowOperator &o, int r_, int m_) : op(o), r(r_), m(m_) {
+    Gmres(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
         TT_REQUIRE(r <= 256, "at most 256 simultaneous systems");
         ne = o.dim * r;
         slot_len = (2 * (m + 1) + 1) * r;
@@ -1068,8 +1138,16 @@ struct Gmres {
         TT_HIP(hipHostMalloc(&Hh, sizeof(double2) * (size_t)m * slot_len, hipHostMallocDefault));
         step_ev.resize(m);
         for (auto &e : step_ev) TT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        graphs[0].resize(m);
+        graphs[1].resize(m);
     }
     ~Gmres() {
+        for (auto &gv : graphs)
+            for (auto &g : gv) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                for (auto e : g.ev)
+                    if (e) (void)hipEventDestroy(e);
+            }
         for (auto e : step_ev) (void)hipEventDestroy(e);
         if (Hh) (void)hipHostFree(Hh);
     }
@@ -1093,13 +1171,15 @@ struct Gmres {
     }
 
     // out = M(y) = y - (y .* G) A
-    void apply_M(const double2 *y, double2 *out, bool fp32) { op.apply(y, out, r, true, fp32, true, y); }
+    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr) {
+        op.apply(y, out, r, true, fp32, true, y, ev);
+    }
 
-    // Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
-    void step(int j, bool fp32) {
+    // The work of Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
+    void enqueue_step(int j, bool fp32, const hipEvent_t *ev) {
         double2 *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
-        apply_M(vj, W.p, fp32);
+        apply_M(vj, W.p, fp32, ev);
         dots_dev(V.p, j + 1, W.p, h1);
         hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r, -1.0,
                            nullptr);
@@ -1111,7 +1191,55 @@ struct Gmres {
         hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(256), 0, op.s, partial.p, (int)nbc, r, n2);
         hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, n2, ne, r);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
+    }
+
+    // Launch Arnoldi step j (a replay of its graph, captured on first use) and record step_ev[j].
+    void step(int j, bool fp32) {
+        if (!use_graphs) {
+            enqueue_step(j, fp32, nullptr);
+            TT_HIP(hipEventRecord(step_ev[j], op.s));
+            return;
+        }
+        StepGraph &g = graphs[fp32 ? 1 : 0][j];
+        op.prepare(r, fp32);
+        if (!g.exec || g.gen != op.layout_gen) {
+            if (g.exec) TT_HIP(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+            for (auto &e : g.ev)
+                if (!e) TT_HIP(hipEventCreate(&e));
+            const int na = op.n_apply, na32 = op.n_apply32;  // capture runs nothing: count launches below
+            TT_HIP(hipStreamBeginCapture(op.s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                enqueue_step(j, fp32, g.ev);
+            } catch (...) {
+                (void)hipStreamEndCapture(op.s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            op.n_apply = na;
+            op.n_apply32 = na32;
+            TT_HIP(hipStreamEndCapture(op.s, &graph));
+            hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                g.exec = nullptr;
+                fail(TT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            g.gen = op.layout_gen;
+        }
+        TT_HIP(hipGraphLaunch(g.exec, op.s));
         TT_HIP(hipEventRecord(step_ev[j], op.s));
+        ++op.n_apply;
+        if (fp32) ++op.n_apply32;
+    }
+
+    // Timing of a completed step (graph mode records the application into the graph's events
+    // as external event nodes); stops asking once the runtime reports no timestamps for them.
+    bool graph_timing = true;
+    void read_step_timing(int j, bool fp32) {
+        if (use_graphs && graph_timing)
+            graph_timing = op.add_timing(graphs[fp32 ? 1 : 0][j].ev, op.product_bytes(r, fp32));
     }
 
     // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).
@@ -1139,9 +1267,14 @@ struct Gmres {
             int used = 0;
             const int jmax = std::min(m, max_iter - it_total);
             step(0, fp32);
+            int launched = 1;
             for (int j = 0; j < jmax; ++j) {
-                if (j + 1 < jmax) step(j + 1, fp32);   // keep the device busy while the host rotates column j
+                if (j + 1 < jmax) {
+                    step(j + 1, fp32);   // keep the device busy while the host rotates column j
+                    launched = j + 2;
+                }
                 TT_HIP(hipEventSynchronize(step_ev[j]));
+                read_step_timing(j, fp32);
                 const double2 *hj = Hh + (size_t)j * slot_len;
                 bool conv_all = true;
                 for (int k = 0; k < r; ++k) {
@@ -1186,6 +1319,7 @@ struct Gmres {
                 }
             }
             TT_HIP(hipStreamSynchronize(op.s));  // a speculative step may still use zdev's neighbours
+            if (launched > used) read_step_timing(used, fp32);
             zdev.upload(reinterpret_cast<const double2 *>(z.data()), z.size(), op.s);
             hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r, 1.0,
                                nullptr);
@@ -1309,43 +1443,141 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
     st.iterations = n_done + 1;
 }
 
-static void solve_krylov(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
+// ------------------------------------------------------------------ solve plans
+// Everything a channel's solves reuse: the operator (V.C and C^T blocks, work items, P123
+// in cell form), the Krylov workspace with its captured step graphs, the right-hand-side and
+// solution vectors.  Cached per context, keyed by the P123 object and a hash of the inputs
+// the operator is built from; only G (the energies) changes between reuses.
+struct SolvePlan {
+    uint64_t p_gen = 0, key = 0;
+    int format = TT_P123_CELLS;
+    uint64_t last_use = 0;
+    RowOperator op;
+    std::unique_ptr<Gmres> gm;
+    DevBuf<double2> b, x, rres, d;
+
+    Gmres &gmres(int r, int m, bool graphs) {
+        if (!gm || gm->r != r || gm->m != m || gm->use_graphs != graphs) {
+            gm.reset();
+            gm.reset(new Gmres(op, r, m, graphs));
+        }
+        return *gm;
+    }
+};
+
+struct PlanCache {
+    std::vector<std::unique_ptr<SolvePlan>> plans;
+    uint64_t tick = 0;
+    static constexpr size_t max_plans = 32;
+};
+
+void free_plan_cache(PlanCache *c) { delete c; }
+
+static uint64_t hash_bytes(uint64_t h, const void *data, size_t n) {
+    // FNV-1a over 64-bit words (tail bytes folded in one at a time)
+    const unsigned char *b = static_cast<const unsigned char *>(data);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = (h ^ w) * 0x100000001b3ull;
+    }
+    for (; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+static uint64_t plan_key(const tt_run_params &rp, const double *V_unco, const double *V_coup,
+                         const tt_swp_states &swp, const tt_pw_states &chn, int format) {
+    int nu, nc;
+    num_2N_states(rp, nu, nc);
+    const size_t Np = (size_t)swp.Np_WP;
+    uint64_t h = 0xcbf29ce484222325ull;
+    const int flags[6] = {rp.tensor_force, rp.isospin_breaking_1S0, swp.Np_WP, swp.Nq_WP, chn.Nalpha, format};
+    h = hash_bytes(h, flags, sizeof(flags));
+    h = hash_bytes(h, V_unco, sizeof(double) * nu * Np * Np);
+    if (nc) h = hash_bytes(h, V_coup, sizeof(double) * nc * 4 * Np * Np);
+    h = hash_bytes(h, swp.C_SWP_unco, sizeof(double) * nu * Np * Np);
+    if (nc) h = hash_bytes(h, swp.C_SWP_coup, sizeof(double) * nc * 4 * Np * Np);
+    const int *arrs[] = {chn.L_2N, chn.S_2N, chn.J_2N, chn.T_2N, chn.L_1N, chn.two_J_1N, chn.two_J_3N,
+                         chn.two_T_3N, chn.P_3N};
+    for (const int *a : arrs) h = hash_bytes(h, a, sizeof(int) * chn.Nalpha);
+    return h;
+}
+
+// The context's plan for these inputs: reused when the P123 object and every input hash
+// match, else built (evicting the least recently used plan beyond max_plans).
+static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC *P, const double *V_unco,
+                           const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format) {
+    if (!ctx->plans) ctx->plans = new PlanCache;
+    PlanCache &c = *ctx->plans;
+    const uint64_t key = plan_key(rp, V_unco, V_coup, swp, chn, format);
+    for (auto &pl : c.plans)
+        if (pl->p_gen == P->gen && pl->key == key && pl->format == format) {
+            pl->last_use = ++c.tick;
+            return *pl;
+        }
+    if (c.plans.size() >= PlanCache::max_plans) {
+        auto lru = std::min_element(c.plans.begin(), c.plans.end(),
+                                    [](const std::unique_ptr<SolvePlan> &a, const std::unique_ptr<SolvePlan> &b) {
+                                        return a->last_use < b->last_use;
+                                    });
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+        c.plans.erase(lru);
+    }
+    std::unique_ptr<SolvePlan> pl(new SolvePlan);
+    pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format);
+    pl->p_gen = P->gen;
+    pl->key = key;
+    pl->format = format;
+    pl->last_use = ++c.tick;
+    c.plans.push_back(std::move(pl));
+    return *c.plans.back();
+}
+
+static void solve_krylov(SolvePlan &pl, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
                          tt_solve_stats &st, bool mixed) {
+    RowOperator &op = pl.op;
     OnshellLayout L = onshell_layout(os, op.Nq, op.Np);
     op.set_gidx(L.row_g);
     const int64_t ne = op.dim * L.r;
-    DevBuf<double2> b(ne), x(ne);
-    on_shell_rows_of_A(op, L, b.p);
+    auto fit = [&](DevBuf<double2> &v) {
+        if (v.n < (size_t)ne) v.alloc(ne);
+    };
+    fit(pl.b);
+    fit(pl.x);
+    on_shell_rows_of_A(op, L, pl.b.p);
     const int m = o.restart > 0 ? o.restart : 40;
-    Gmres gm(op, L.r, m);
-    std::vector<double> bnorm = gm.norms(b.p);
-    TT_HIP(hipMemsetAsync(x.p, 0, sizeof(double2) * ne, op.s));
+    Gmres &gm = pl.gmres(L.r, m, o.use_graphs != 0);
+    std::vector<double> bnorm = gm.norms(pl.b.p);
+    TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * ne, op.s));
     double res = 0.0;
     int iters = 0;
     if (!mixed) {
-        iters = gm.solve(b.p, x.p, o.tol, o.max_iter, false, res, bnorm);
+        iters = gm.solve(pl.b.p, pl.x.p, o.tol, o.max_iter, false, res, bnorm);
     } else {
         // iterative refinement: FP64 residual, FP32-product GMRES corrections
-        DevBuf<double2> rres(ne), d(ne);
+        fit(pl.rres);
+        fit(pl.d);
         for (int outer = 0; outer < 20; ++outer) {
-            gm.apply_M(x.p, rres.p, false);
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, rres.p, b.p, rres.p, 1.0, -1.0, ne);
-            std::vector<double> rn = gm.norms(rres.p);
+            gm.apply_M(pl.x.p, pl.rres.p, false);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.rres.p, pl.b.p, pl.rres.p, 1.0,
+                               -1.0, ne);
+            std::vector<double> rn = gm.norms(pl.rres.p);
             res = 0.0;
             for (int k = 0; k < L.r; ++k) res = std::max(res, rn[k] / std::max(bnorm[k], 1e-300));
             if (res <= o.tol || iters >= o.max_iter) break;
-            TT_HIP(hipMemsetAsync(d.p, 0, sizeof(double2) * ne, op.s));
+            TT_HIP(hipMemsetAsync(pl.d.p, 0, sizeof(double2) * ne, op.s));
             double inner_res = 0.0;
             // inner tolerance relative to this residual
             std::vector<double> rnorm = rn;
             double inner_tol = std::max(1e-6, o.tol / std::max(res, 1e-300));
             inner_tol = std::min(inner_tol, 1e-4);
-            iters += gm.solve(rres.p, d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm);
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x.p, x.p, d.p, 1.0, 1.0, ne);
+            iters += gm.solve(pl.rres.p, pl.d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm);
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.x.p, pl.x.p, pl.d.p, 1.0, 1.0, ne);
         }
     }
     std::vector<cdouble> out;
-    gather_onshell(op, x.p, L, os, out);
+    gather_onshell(op, pl.x.p, L, os, out);
     for (size_t e = 0; e < out.size(); ++e) U[e] = out[e];
     st.iterations = iters;
     st.residual = res;
@@ -1460,6 +1692,9 @@ tt_status tt_ctx_create(int device, tt_ctx **out) {
 void tt_ctx_destroy(tt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_plan_cache(c->plans);
+    c->plans = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1481,6 +1716,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->NM_max = 14;
     o->neumann_terms = nullptr;
     o->p123_format = TT_P123_CELLS;
+    o->use_graphs = 1;
 }
 
 tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
@@ -1587,8 +1823,14 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
     if (opts) o = *opts;
     tt_solve_stats st;
     std::memset(&st, 0, sizeof(st));
-    RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, os->num_T_lab, o.p123_format);
+    const DeviceCSC *P = dev_of(P123);
+    TT_REQUIRE(P->dim == (int64_t)chn->Nalpha * swp->Nq_WP * swp->Np_WP, "P123 dimension does not match Nalpha*Nq*Np");
+    SolvePlan &pl = get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format);
+    RowOperator &op = pl.op;
+    op.collect();
+    op.ms_apply = op.ms_spmm = op.spmm_bytes = 0.0;
+    op.n_apply = op.n_apply32 = op.n_spmm = 0;
+    op.set_G(G, os->num_T_lab);
     std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
     switch (o.method) {
         case TT_SOLVE_PADE:
@@ -1598,10 +1840,10 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
             solve_dense(op, *os, Uc.data(), st);
             break;
         case TT_SOLVE_GMRES:
-            solve_krylov(op, *os, o, Uc.data(), st, false);
+            solve_krylov(pl, *os, o, Uc.data(), st, false);
             break;
         case TT_SOLVE_MIXED:
-            solve_krylov(op, *os, o, Uc.data(), st, true);
+            solve_krylov(pl, *os, o, Uc.data(), st, true);
             break;
         default:
             fail(TT_ERR_ARG, "unknown solve method");
@@ -1630,8 +1872,8 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
     TT_HIP(hipSetDevice(ctx->device));
     RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, G, nE,
-             (flags & 2) ? TT_P123_CSC : TT_P123_CELLS);
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, (flags & 2) ? TT_P123_CSC : TT_P123_CELLS);
+    op.set_G(G, nE);
     std::vector<int> g(r, -1);
     bool useG = false;
     if (g_index)

@@ -103,6 +103,9 @@ P123Angular build_p123_angular(const tt_pw_states &chn);
 constexpr int64_t BLK_SLACK = 64;
 
 struct DeviceCSC {
+    // distinct for every DeviceCSC the process creates: cached solve plans key on it, so a
+    // new matrix at a freed one's address never matches a stale plan
+    uint64_t gen = next_gen();
     int64_t dim = 0, nnz = 0;
     double *val = nullptr;
     float *val32 = nullptr;          // lazily made FP32 copy for the mixed path
@@ -117,9 +120,14 @@ struct DeviceCSC {
     double *blk = nullptr;           // ncell * Na * Na + BLK_SLACK (zeroed slack)
     float *blk32 = nullptr;          // lazily made FP32 copy, same length
     ~DeviceCSC();
+    static uint64_t next_gen();
 };
 // Builds the cell-block form of a CSC P123 (host pass over a device->host copy).
 void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s);
+
+// Solve plans cached per context (operator set-up, Krylov workspaces, captured graphs).
+struct PlanCache;
+void free_plan_cache(PlanCache *c);
 
 }  // namespace tt
 
@@ -139,4 +147,5 @@ void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s);
 struct tt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    tt::PlanCache *plans = nullptr;
 };

@@ -231,7 +231,7 @@ class Channel:
         return G.view(np.complex128).reshape(len(E), self.dim)
 
     def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
-              neumann=False, p123_format=L.P123_CELLS):
+              neumann=False, p123_format=L.P123_CELLS, graphs=True):
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
 
         With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n]."""
@@ -243,6 +243,7 @@ class Channel:
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
         o.p123_format = p123_format
+        o.use_graphs = int(bool(graphs))
         nterms = 2 * NM_max + 1
         a = np.zeros(self.nd * self.nd * self.nE * nterms * 2)
         if neumann:
