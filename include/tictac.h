@@ -223,6 +223,9 @@ typedef struct {
                                  * keeps the operator and Krylov workspace of recent channels
                                  * (keyed by the P123 object and a hash of V, C and the channel),
                                  * so repeated solves of a channel only upload G. */
+    int resolvent_on_device;    /* 1: ignore G (may be NULL) and compute the resolvent of the
+                                 * on-shell energies E_com + E_bound on the device, straight
+                                 * into the solver's buffer (no host round trip); 0: use G. */
 } tt_solve_opts;
 void tt_solve_opts_default(tt_solve_opts *o);
 

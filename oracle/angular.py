@@ -9,7 +9,7 @@ Restates the GSL calls the reference makes:
 Wigner symbols are exact: Racah's single-sum formulas in integer arithmetic,
 the square root taken last.  Arguments are doubled (two_j), as in GSL.
 
-Pins (tests/test_oracle_angular.py): 3j/6j/9j against sympy.physics.wigner;
+Pins (tests/test_oracle_pins.py): 3j/6j/9j against sympy.physics.wigner;
 Plm against scipy.special.sph_harm at phi = 0 (Condon-Shortley phase included).
 """
 import math

@@ -14,7 +14,7 @@ The element loop is vectorised over the (phi, x) quadrature grid; the sum over
 grid points is a numpy reduction rather than the reference's sequential loop,
 so values agree to rounding, not bit for bit.
 
-Pins (tests/test_oracle_p123.py): the S-wave special case G-tilde = A0/2 (the
+Pins (tests/test_oracle_pins.py): the S-wave special case G-tilde = A0/2 (the
 closed form of CPP/auxiliary.cpp:991-1063 at L=l=L'=l'=0); for constant
 G-tilde the element equals a brute-force Monte-Carlo/nested quadrature of the
 bin-overlap volume; the mask is compared with a dense point test.

@@ -15,7 +15,7 @@ and the two models the workloads use:
 Momenta are in MeV, V in MeV^-2, with the measure p^2 dp of the reference
 (MT: V_0/(pi p p') Q_0(z) == V_0/(2 pi p p') ln(...)).
 
-Pins (tests/test_oracle_potential.py): Q_L against mpmath.legenq; the MT
+Pins (tests/test_oracle_pins.py): Q_L against mpmath.legenq; the MT
 S-wave formula against the Yukawa partial-wave projection at L = 0.
 """
 import math

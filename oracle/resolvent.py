@@ -6,7 +6,7 @@ Restates CPP/make_resolvent.cpp:4-211 (Kukulin 2007 eqs. 32, 34-35):
 with mu1 = Mn (Mn+Mp+Eb)/(2Mn+Mp+Eb), Eb = the p-bin's lower edge when negative, else 0.
 Index order (alpha, q, p), as in the permutation matrix (line 201).
 
-Pins (tests/test_oracle_resolvent.py): the closed forms are checked against a
+Pins (tests/test_oracle_pins.py): the closed forms are checked against a
 brute-force bin average of 1/(E - e_p - e_q + i0) done by quadrature of the
 principal value plus -i pi delta.
 """

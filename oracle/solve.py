@@ -21,7 +21,7 @@ reference (lines 1264, 1273: out of bounds).  The array is value-initialised
 to zero, so the oracle takes PA_{-1} = 0; this choice is "parity unpinned"
 (no reference test or run can pin undefined behaviour).
 
-Pins (tests/test_oracle_solve.py): matrix-free row apply against the explicit
+Pins (tests/test_oracle_pins.py): matrix-free row apply against the explicit
 dense A; dense solve residual; Pade of a geometric series a_n = r^n equals
 1/(1-r) exactly for N >= 1 and agrees with scipy.interpolate.pade.
 """

@@ -10,7 +10,7 @@ The reference diagonalises with LAPACKE_dspevd; here numpy.linalg.eigh.  The
 eigenvector sign is arbitrary in both, and every on-shell U element is
 invariant under it (C enters as C ... C^T with the diagonal G in between).
 
-Pins (tests/test_oracle_swp.py): C^T C = 1 and C^T H C diagonal; MT-III deuteron
+Pins (tests/test_oracle_pins.py): C^T C = 1 and C^T H C diagonal; MT-III deuteron
 near -2.23 MeV at large N (CPP/Interactions/malfliet_tjon.cpp:11); V_WP of the
 MT potential against scipy.integrate.dblquad over one bin pair.
 """

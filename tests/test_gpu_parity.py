@@ -218,6 +218,14 @@ def test_graph_replays_match_eager_launches(ctx, c2):
             assert st_g.n_spmm <= st_g.n_apply
 
 
+def test_device_resolvent_matches_host_G(ctx, c2):
+    """resolvent_on_device computes G of the on-shell energies inside the solve; the result
+    equals the solve fed the host G of tt_calculate_resolvent (same kernel, same values)."""
+    U_h, _ = c2.ch.solve(ctx, c2.P, c2.G, method=tt.SOLVE_GMRES)
+    U_d, _ = c2.ch.solve(ctx, c2.P, None, method=tt.SOLVE_GMRES, device_resolvent=True)
+    assert np.abs(U_d - U_h).max() <= 1e-14 * np.abs(U_h).max()
+
+
 def test_plan_follows_potential_and_p123_changes(ctx):
     """A cached plan is reused only for the same P123 object and the same V, C and channel:
     changing the potential strengths on the same P123, or rebuilding P123, gives the

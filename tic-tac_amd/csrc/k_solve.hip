@@ -899,6 +899,17 @@ struct RowOperator {
         vc_blocks.upload(b2.data(), std::max<size_t>(b2.size(), 1), s);
     }
 
+    // Device buffer for the resolvent of nE energies (contents written by the caller).
+    double2 *G_buffer(int nE_) {
+        nE = nE_;
+        const size_t n = (size_t)std::max(nE, 1) * dim;
+        if (n > G.n) {
+            G.alloc(n);
+            ++layout_gen;
+        }
+        return G.p;
+    }
+
     // Resolvent of nE energies (host, [j][idx] complex), copied into the device buffer.
     void set_G(const double *Gh, int nE_) {
         nE = nE_;
@@ -1443,6 +1454,29 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
     st.iterations = n_done + 1;
 }
 
+// G[j][idx] at the total energies E[0..nE) (host), written to device memory dG (nE*dim).
+static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt_swp_states &swp,
+                                const tt_pw_states &chn, const double *E, int nE, double2 *dG) {
+    std::vector<int> which, offset;
+    swp_edge_offsets(chn, swp, rp, which, offset);
+    const int Na = chn.Nalpha, Nq = swp.Nq_WP, Np = swp.Np_WP;
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const size_t nu = (size_t)swp.num_2N_unco_states * (Np + 1), nc = (size_t)swp.num_2N_coup_states * 2 * (Np + 1);
+    DevBuf<double> eu, ec, qe, dE;
+    DevBuf<int> dw, doff;
+    eu.upload(swp.e_SWP_unco, std::max<size_t>(nu, 1), s);
+    if (nc) ec.upload(swp.e_SWP_coup, nc, s);
+    qe.upload(swp.q_WP, Nq + 1, s);
+    dE.upload(E, nE, s);
+    dw.upload(which.data(), Na, s);
+    doff.upload(offset.data(), Na, s);
+    hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p, doff.p, qe.p,
+                       dE.p, Na, Nq, Np, nE);
+    TT_HIP(hipGetLastError());
+    // the staging buffers are freed on return: let the kernel finish with them first
+    TT_HIP(hipStreamSynchronize(s));
+}
+
 // ------------------------------------------------------------------ solve plans
 // Everything a channel's solves reuse: the operator (V.C and C^T blocks, work items, P123
 // in cell form), the Krylov workspace with its captured step graphs, the right-hand-side and
@@ -1717,6 +1751,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->neumann_terms = nullptr;
     o->p123_format = TT_P123_CELLS;
     o->use_graphs = 1;
+    o->resolvent_on_device = 0;
 }
 
 tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
@@ -1787,23 +1822,9 @@ tt_status tt_calculate_resolvent(tt_ctx *ctx, const tt_run_params *rp, const tt_
     TT_GUARD_BEGIN
     TT_REQUIRE(ctx && rp && swp && chn && E && G && nE > 0, "null argument");
     TT_HIP(hipSetDevice(ctx->device));
-    std::vector<int> which, offset;
-    swp_edge_offsets(*chn, *swp, *rp, which, offset);
-    const int Na = chn->Nalpha, Nq = swp->Nq_WP, Np = swp->Np_WP;
-    const int64_t dim = (int64_t)Na * Nq * Np;
-    const size_t nu = (size_t)swp->num_2N_unco_states * (Np + 1), nc = (size_t)swp->num_2N_coup_states * 2 * (Np + 1);
-    DevBuf<double> eu, ec, qe, dE;
-    DevBuf<int> dw, doff;
+    const int64_t dim = (int64_t)chn->Nalpha * swp->Nq_WP * swp->Np_WP;
     DevBuf<double2> dG((size_t)dim * nE);
-    eu.upload(swp->e_SWP_unco, std::max<size_t>(nu, 1), ctx->stream);
-    if (nc) ec.upload(swp->e_SWP_coup, nc, ctx->stream);
-    qe.upload(swp->q_WP, Nq + 1, ctx->stream);
-    dE.upload(E, nE, ctx->stream);
-    dw.upload(which.data(), Na, ctx->stream);
-    doff.upload(offset.data(), Na, ctx->stream);
-    hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, ctx->stream, dG.p, eu.p, ec.p, dw.p,
-                       doff.p, qe.p, dE.p, Na, Nq, Np, nE);
-    TT_HIP(hipGetLastError());
+    resolvent_on_device(ctx->stream, *rp, *swp, *chn, E, nE, dG.p);
     TT_HIP(hipMemcpyAsync(G, dG.p, sizeof(double2) * dim * nE, hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
     TT_GUARD_END
@@ -1814,7 +1835,8 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
                                      const tt_swp_states *swp, const tt_channel_os *os, const tt_pw_states *chn,
                                      const tt_solve_opts *opts, tt_solve_stats *stats) {
     TT_GUARD_BEGIN
-    TT_REQUIRE(ctx && rp && U && G && P123 && V_unco && swp && os && chn, "null argument");
+    TT_REQUIRE(ctx && rp && U && P123 && V_unco && swp && os && chn, "null argument");
+    TT_REQUIRE(G || (opts && opts->resolvent_on_device), "G required unless opts->resolvent_on_device is set");
     TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
     TT_HIP(hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
@@ -1830,7 +1852,14 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
     op.collect();
     op.ms_apply = op.ms_spmm = op.spmm_bytes = 0.0;
     op.n_apply = op.n_apply32 = op.n_spmm = 0;
-    op.set_G(G, os->num_T_lab);
+    if (o.resolvent_on_device) {
+        // E_com + E_bound of the on-shell bins (CPP/main.cpp:391), straight into the plan's G
+        std::vector<double> E(os->num_T_lab);
+        for (int j = 0; j < os->num_T_lab; ++j) E[j] = os->E_com[j] + swp->E_bound;
+        resolvent_on_device(ctx->stream, *rp, *swp, *chn, E.data(), os->num_T_lab, op.G_buffer(os->num_T_lab));
+    } else {
+        op.set_G(G, os->num_T_lab);
+    }
     std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
     switch (o.method) {
         case TT_SOLVE_PADE:

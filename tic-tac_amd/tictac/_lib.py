@@ -61,7 +61,8 @@ class P123Stats(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int),
-                ("NM_max", C.c_int), ("neumann_terms", _dp), ("p123_format", C.c_int), ("use_graphs", C.c_int)]
+                ("NM_max", C.c_int), ("neumann_terms", _dp), ("p123_format", C.c_int), ("use_graphs", C.c_int),
+                ("resolvent_on_device", C.c_int)]
 
 
 class SolveStats(C.Structure):

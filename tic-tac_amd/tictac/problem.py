@@ -231,27 +231,33 @@ class Channel:
         return G.view(np.complex128).reshape(len(E), self.dim)
 
     def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
-              neumann=False, p123_format=L.P123_CELLS, graphs=True):
+              neumann=False, p123_format=L.P123_CELLS, graphs=True, device_resolvent=False):
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
 
-        With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n]."""
-        if G is None:
-            G = self.resolvent(ctx)
-        G = np.ascontiguousarray(G, np.complex128)
-        assert G.shape == (self.nE, self.dim)
+        With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n].
+        With ``device_resolvent=True`` G is computed on the device inside the call (G ignored)."""
+        if device_resolvent:
+            G = None
+        else:
+            if G is None:
+                G = self.resolvent(ctx)
+            G = np.ascontiguousarray(G, np.complex128)
+            assert G.shape == (self.nE, self.dim)
         o = L.SolveOpts()
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
         o.p123_format = p123_format
         o.use_graphs = int(bool(graphs))
+        o.resolvent_on_device = int(bool(device_resolvent))
         nterms = 2 * NM_max + 1
         a = np.zeros(self.nd * self.nd * self.nE * nterms * 2)
         if neumann:
             o.neumann_terms = dptr(a)
         st = L.SolveStats()
         U = np.zeros(self.nd * self.nd * self.nE * 2)
-        Gf = G.view(np.float64).ravel()
-        check(L.lib().tt_solve_faddeev_equations(ctx.handle, C.byref(self.prob.rp), dptr(U), dptr(Gf),
+        Gf = None if G is None else G.view(np.float64).ravel()
+        check(L.lib().tt_solve_faddeev_equations(ctx.handle, C.byref(self.prob.rp), dptr(U),
+                                                 dptr(Gf) if Gf is not None else None,
                                                  C.byref(P.csc), dptr(self.prob.V_unco), dptr(self.prob.V_coup),
                                                  C.byref(self.prob.swp), C.byref(self.os), C.byref(self.view),
                                                  C.byref(o), C.byref(st)))
