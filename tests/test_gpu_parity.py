@@ -95,6 +95,25 @@ def test_apply_c1(ctx, c1):
     assert np.abs(Ycsc - Yo).max() <= 1e-13 * np.abs(Yo).max()
 
 
+def test_c1_against_golden_fixture(ctx, c1):
+    """The library's SWP edges, P123, G and converged U at the Malfliet-Tjon N = 16 case equal
+    the committed fixture (tests/golden/mt_np16_elastic.npz)."""
+    from test_golden import load
+    g = load()
+    assert abs(c1.prob.E_bound - float(g["E_bound"])) <= 1e-12 * abs(float(g["E_bound"]))
+    assert np.array_equal(np.asarray(c1.ch.q_idx), g["q_idx"])
+    assert np.array_equal(c1.ptr, g["p123_colptr"]) and np.array_equal(c1.row, g["p123_row"])
+    vs = c1.val[g["p123_val_sample_idx"]]
+    assert np.abs(vs - g["p123_val_sample"]).max() <= 1e-13 * np.abs(g["p123_val_sample"]).max()
+    assert abs(np.abs(c1.val).sum() - float(g["p123_abs_sum"])) <= 1e-12 * float(g["p123_abs_sum"])
+    # resolvent: the continuum closed form loses digits far off shell; bound by its term sizes
+    for j, E in enumerate(c1.ch.energies()):
+        _, scale = rs.resolvent_array(E, c1.och, c1.oswp, c1.tf, c1.ib, with_scale=True)
+        assert (np.abs(c1.G[j] - g["G"][j]) <= 64 * np.finfo(float).eps * scale + 1e-15 * np.abs(g["G"][j])).all()
+    U, _ = c1.ch.solve(ctx, c1.P, c1.G, method=tt.SOLVE_GMRES)
+    assert np.abs(U - g["U"]).max() <= 1e-10 * np.abs(g["U"]).max()
+
+
 def test_uploaded_reference_csc_gives_same_solution(ctx, c1):
     """A CSC supplied by the caller (the reference's format, here the oracle's P123) is
     converted to cell blocks on first use and solves to the same U."""
