@@ -258,6 +258,26 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
                                      const tt_pw_states *chn, const tt_solve_opts *opts,
                                      tt_solve_stats *stats);
 
+/* Batched solve: every item is one 3N channel with all its on-shell bins (the energies
+ * requested for it), solved as one block Krylov system (r = n_d * n_bins right-hand sides
+ * share each A-application).  G is computed on the device for each item (E_com + E_bound,
+ * os->E_com required).  Items are solved back to back on the context's stream, each through
+ * its channel's cached plan (operator, workspace, captured step graphs), so a repeated batch
+ * over the same channels only recomputes G and solves.  No reference counterpart: it replaces
+ * the per-channel, per-energy loop of CPP/main.cpp:166-492 around solve_faddeev_equations.
+ * Every item is attempted; each gets its own status; the return value is the first failure. */
+typedef struct {
+    const tt_pw_states *chn;    /* channel view (tt_pw_channel)                    */
+    const tt_channel_os *os;    /* its on-shell bins (tt_find_on_shell)            */
+    const tt_csc *P123;         /* its P123 (tt_fill_P123_arrays / tt_csc_upload)  */
+    double *U;                  /* out: nd*nd*nE complex, layout as above          */
+    tt_solve_stats *stats;      /* optional per-item statistics                    */
+    tt_status status;           /* out                                             */
+} tt_batch_item;
+tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_unco,
+                         const double *V_coup, const tt_swp_states *swp, tt_batch_item *items,
+                         int n_items, const tt_solve_opts *opts);
+
 /* Raw kernel-matrix application for tests and profiling: Y = (X .* G_sel) A in
  * the reference's row form (CPP/solve_faddeev.cpp:956-1088), r rows.
  * X, Y: host, r*dim complex (row-major).  g_index[k] selects the energy of row k

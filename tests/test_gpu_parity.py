@@ -106,10 +106,11 @@ def test_c1_against_golden_fixture(ctx, c1):
     vs = c1.val[g["p123_val_sample_idx"]]
     assert np.abs(vs - g["p123_val_sample"]).max() <= 1e-13 * np.abs(g["p123_val_sample"]).max()
     assert abs(np.abs(c1.val).sum() - float(g["p123_abs_sum"])) <= 1e-12 * float(g["p123_abs_sum"])
-    # resolvent: the continuum closed form loses digits far off shell; bound by its term sizes
-    for j, E in enumerate(c1.ch.energies()):
-        _, scale = rs.resolvent_array(E, c1.och, c1.oswp, c1.tf, c1.ib, with_scale=True)
-        assert (np.abs(c1.G[j] - g["G"][j]) <= 64 * np.finfo(float).eps * scale + 1e-15 * np.abs(g["G"][j])).all()
+    # resolvent end to end: the fixture's G comes from the oracle's own SWP edges (numpy eigh),
+    # which differ from the library's (Jacobi) at ~1e-13; the continuum closed form amplifies
+    # that by its conditioning, so the bound is 1e-9 of max|G| here (parity on identical
+    # edges, to 64 ulp of the term sizes, is test_resolvent_c1)
+    assert np.abs(c1.G - g["G"]).max() <= 1e-9 * np.abs(g["G"]).max()
     U, _ = c1.ch.solve(ctx, c1.P, c1.G, method=tt.SOLVE_GMRES)
     assert np.abs(U - g["U"]).max() <= 1e-10 * np.abs(g["U"]).max()
 
@@ -266,6 +267,34 @@ def test_plan_follows_potential_and_p123_changes(ctx):
         K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
         Uo = so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, rp.Nq_WP, rp.Np_WP)
         assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max(), (scale, rebuild)
+
+
+def test_batch_over_channels_and_energies_matches_single_solves(ctx):
+    """tt_solve_batch: four 3N channels (1/2+-, 3/2+-), two lab energies each (all their
+    on-shell bins in one block system, G on the device) give the single-channel results, and
+    the FP64 items agree with the oracle's dense solve."""
+    rp, pot, _ = synthetic_config(N=10, J_2N_max=1, two_J_3N_max=3)
+    T = [5.0, 50.0]
+    prob = tt.Problem(rp, pot)
+    chans = [prob.channel(c, T) for c in range(prob.n_channels)]
+    chans = [ch for ch in chans if ch.nd > 0]
+    assert len(chans) >= 3
+    Ps = [ch.build_p123(ctx) for ch in chans]
+    for method, tol in ((tt.SOLVE_GMRES, 1e-10), (tt.SOLVE_MIXED, 1e-8)):
+        out = prob.solve_batch(ctx, chans, Ps, method=method)
+        for (U, st, status), ch, P in zip(out, chans, Ps):
+            assert status == 0 and st.residual <= 1e-12
+            U1, _ = ch.solve(ctx, P, None, method=tt.SOLVE_GMRES, device_resolvent=True)
+            assert np.abs(U - U1).max() <= tol * np.abs(U1).max()
+    ch, P = chans[0], Ps[0]
+    val, row, ptr = P.to_host()
+    tf, ib = flags(prob)
+    Vu, Vc = oracle_V(prob)
+    K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
+    G = ch.resolvent(ctx)
+    Uo = so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, rp.Nq_WP, rp.Np_WP)
+    U = prob.solve_batch(ctx, [ch], [P], method=tt.SOLVE_GMRES)[0][0]
+    assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max()
 
 
 def test_error_paths_on_device(ctx, c1):

@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
                                                   const T *__restrict__ B, const T *__restrict__ S,
                                                   T *__restrict__ Upart, int Na, int R2) {
     typedef typename Mfma16<T>::acc_t acc_t;
-    __shared__ acc_t red[4][MT * NT][64];
+    __shared__ acc_t red[3][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
     const int item = blockIdx.x;
@@ -454,18 +454,16 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
         mma_batch(base + stride, a1, b1);
         base += 2 * stride;
     }
+    // waves 1-3 hand their accumulators to wave 0 one tile at a time (fixed order
+    // ((w0 + w1) + w2) + w3), so the buffer does not grow with MT*NT
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int n = 0; n < NT; ++n) red[wave][m * NT + n][lane] = acc[m][n];
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                const acc_t t = ((red[0][m * NT + n][lane] + red[1][m * NT + n][lane]) + red[2][m * NT + n][lane]) +
-                                red[3][m * NT + n][lane];
+        for (int n = 0; n < NT; ++n) {
+            if (wave > 0) red[wave - 1][lane] = acc[m][n];
+            __syncthreads();
+            if (wave == 0) {
+                const acc_t t = ((acc[m][n] + red[0][lane]) + red[1][lane]) + red[2][lane];
                 const int v = v0 + 16 * n + il;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -473,7 +471,8 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
                     if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = t[r];
                 }
             }
-    }
+            __syncthreads();
+        }
 }
 
 // U[ket][a][v] = scale * sum of the partial panels of the ket's work items, in item order
@@ -520,13 +519,17 @@ static void launch_cellmfma(hipStream_t s, const DeviceCSC *P, const CellItems &
                        P->Na, R2);
 }
 
+// N tiles per pass: up to 64 real components (32 complex right-hand sides) stream the blocks
+// once; wider batches take ceil(R2 / 64) passes.
 template <typename T, int MT>
 static void launch_cellmfma_nt(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S,
                                T *Upart, int R2) {
     if (R2 <= 16)
         launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
-    else
+    else if (R2 <= 32)
         launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2);
+    else
+        launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2);
 }
 
 // Upart = the work items' partial panels (items * Nalpha * R2); when U is given also
@@ -1830,19 +1833,19 @@ tt_status tt_calculate_resolvent(tt_ctx *ctx, const tt_run_params *rp, const tt_
     TT_GUARD_END
 }
 
-tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G,
-                                     const tt_csc *P123, const double *V_unco, const double *V_coup,
-                                     const tt_swp_states *swp, const tt_channel_os *os, const tt_pw_states *chn,
-                                     const tt_solve_opts *opts, tt_solve_stats *stats) {
-    TT_GUARD_BEGIN
+}  // extern "C"
+
+namespace tt {
+// One channel's on-shell solve (the body of tt_solve_faddeev_equations; throws on error).
+static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G, const tt_csc *P123,
+                          const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                          const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o,
+                          tt_solve_stats *stats) {
     TT_REQUIRE(ctx && rp && U && P123 && V_unco && swp && os && chn, "null argument");
-    TT_REQUIRE(G || (opts && opts->resolvent_on_device), "G required unless opts->resolvent_on_device is set");
+    TT_REQUIRE(G || o.resolvent_on_device, "G required unless opts->resolvent_on_device is set");
     TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
     TT_HIP(hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
-    tt_solve_opts o;
-    tt_solve_opts_default(&o);
-    if (opts) o = *opts;
     tt_solve_stats st;
     std::memset(&st, 0, sizeof(st));
     const DeviceCSC *P = dev_of(P123);
@@ -1854,6 +1857,7 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
     op.n_apply = op.n_apply32 = op.n_spmm = 0;
     if (o.resolvent_on_device) {
         // E_com + E_bound of the on-shell bins (CPP/main.cpp:391), straight into the plan's G
+        TT_REQUIRE(os->E_com, "resolvent_on_device needs the on-shell energies (os->E_com)");
         std::vector<double> E(os->num_T_lab);
         for (int j = 0; j < os->num_T_lab; ++j) E[j] = os->E_com[j] + swp->E_bound;
         resolvent_on_device(ctx->stream, *rp, *swp, *chn, E.data(), os->num_T_lab, op.G_buffer(os->num_T_lab));
@@ -1891,6 +1895,47 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
     if ((o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED) && !(st.residual <= o.tol))
         fail(TT_ERR_NOT_CONVERGED, "Krylov solve stopped at relative residual " + std::to_string(st.residual) +
                                        " above the requested tolerance (U holds the last iterate)");
+}
+}  // namespace tt
+
+extern "C" {
+
+tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G,
+                                     const tt_csc *P123, const double *V_unco, const double *V_coup,
+                                     const tt_swp_states *swp, const tt_channel_os *os, const tt_pw_states *chn,
+                                     const tt_solve_opts *opts, tt_solve_stats *stats) {
+    TT_GUARD_BEGIN
+    tt_solve_opts o;
+    tt_solve_opts_default(&o);
+    if (opts) o = *opts;
+    solve_channel(ctx, rp, U, G, P123, V_unco, V_coup, swp, os, chn, o, stats);
+    TT_GUARD_END
+}
+
+tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_unco, const double *V_coup,
+                         const tt_swp_states *swp, tt_batch_item *items, int n_items, const tt_solve_opts *opts) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && V_unco && swp && (items || n_items == 0) && n_items >= 0, "null argument");
+    tt_solve_opts o;
+    tt_solve_opts_default(&o);
+    if (opts) o = *opts;
+    o.resolvent_on_device = 1;
+    tt_status first = TT_OK;
+    std::string first_msg;
+    for (int i = 0; i < n_items; ++i) {
+        tt_batch_item &it = items[i];
+        it.status = TT_OK;
+        try {
+            solve_channel(ctx, rp, it.U, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o, it.stats);
+        } catch (const Error &e) {
+            it.status = e.code;
+            if (first == TT_OK) {
+                first = e.code;
+                first_msg = "batch item " + std::to_string(i) + ": " + e.msg;
+            }
+        }
+    }
+    if (first != TT_OK) fail(first, first_msg);
     TT_GUARD_END
 }
 

@@ -71,6 +71,11 @@ class SolveStats(C.Structure):
                 ("n_spmm", C.c_int), ("ms_spmm", C.c_double), ("spmm_bytes", C.c_double)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("chn", C.POINTER(PwStates)), ("os", C.POINTER(ChannelOS)), ("P123", C.POINTER(CSC)),
+                ("U", _dp), ("stats", C.POINTER(SolveStats)), ("status", C.c_int)]
+
+
 # exported symbols and their prototypes (restype, argtypes)
 PROTOTYPES = {
     "tt_last_error": (C.c_char_p, []),
@@ -110,6 +115,8 @@ PROTOTYPES = {
     "tt_solve_faddeev_equations": (C.c_int, [C.c_void_p, C.POINTER(RunParams), _dp, _dp, C.POINTER(CSC), _dp, _dp,
                                              C.POINTER(SwpStates), C.POINTER(ChannelOS), C.POINTER(PwStates),
                                              C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "tt_solve_batch": (C.c_int, [C.c_void_p, C.POINTER(RunParams), _dp, _dp, C.POINTER(SwpStates),
+                                 C.POINTER(BatchItem), C.c_int, C.POINTER(SolveOpts)]),
     "tt_apply_rows": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(CSC), _dp, _dp, C.POINTER(SwpStates),
                                 C.POINTER(PwStates), _dp, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int]),
 }

@@ -129,6 +129,33 @@ class Problem:
     def channel(self, c, T_lab):
         return Channel(self, c, T_lab)
 
+    def solve_batch(self, ctx, channels, P123s, method=L.SOLVE_MIXED, tol=1e-12, max_iter=2000, restart=40,
+                    graphs=True):
+        """tt_solve_batch over channels (each with all its on-shell bins), G on the device.
+        Returns [(U[d_row, d_col, j], stats, status)] per channel; raises on the first failure
+        after every channel was attempted."""
+        o = L.SolveOpts()
+        L.lib().tt_solve_opts_default(C.byref(o))
+        o.method, o.tol, o.max_iter, o.restart = method, tol, max_iter, restart
+        o.use_graphs = int(bool(graphs))
+        n = len(channels)
+        items = (L.BatchItem * max(n, 1))()
+        Us, stats = [], []
+        for i, (ch, P) in enumerate(zip(channels, P123s)):
+            U = np.zeros(ch.nd * ch.nd * ch.nE * 2)
+            st = L.SolveStats()
+            Us.append(U)
+            stats.append(st)
+            items[i].chn = C.pointer(ch.view)
+            items[i].os = C.pointer(ch.os)
+            items[i].P123 = C.pointer(P.csc)
+            items[i].U = dptr(U)
+            items[i].stats = C.pointer(st)
+        check(L.lib().tt_solve_batch(ctx.handle, C.byref(self.rp), dptr(self.V_unco), dptr(self.V_coup),
+                                     C.byref(self.swp), items, n, C.byref(o)))
+        return [(U.view(np.complex128).reshape(ch.nd, ch.nd, ch.nE), st, items[i].status)
+                for i, (U, st, ch) in enumerate(zip(Us, stats, channels))]
+
     def pw_arrays(self):
         n = self.pw.Nalpha
         names = ["L_2N", "S_2N", "J_2N", "T_2N", "L_1N", "two_J_1N", "two_J_3N", "two_T_3N", "P_3N"]
