@@ -58,6 +58,42 @@ def timed_steps(ctx, ch, P, method, steps, warmup, host_G=False):
     return time.perf_counter() - t0, U, stats
 
 
+BATCH_WORKLOAD = ("same synthetic NN, all 3N channels 2J=1..17 both parities (18 channels, N_alpha 27..63, "
+                  "dim 24300..56700), Np=Nq=30, T_lab = linspace(1, 200, 8) MeV -> 6 on-shell bins per channel, "
+                  "each channel one block Krylov system (tt_solve_batch)")
+
+
+def batched(ctx, reps=2):
+    """All channels x 8 energies through tt_solve_batch (G on the device, P123 resident):
+    channel-solves/s = 18 * 8 / wall time of one batch, mixed and FP64, best of reps."""
+    T = list(np.linspace(1.0, 200.0, 8))
+    rp = tt.run_params(two_J_3N_max=17, J_2N_max=3, Np_WP=30, Nq_WP=30, chebyshev_s=tt.chebyshev_scale(30))
+    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
+    t0 = time.perf_counter()
+    chans = [prob.channel(c, T) for c in range(prob.n_channels)]
+    Ps = [ch.build_p123(ctx) for ch in chans]
+    ctx.synchronize()
+    t_build = time.perf_counter() - t0
+    res = {}
+    U = {}
+    for method, name in ((tt.SOLVE_MIXED, "mixed"), (tt.SOLVE_GMRES, "fp64")):
+        prob.solve_batch(ctx, chans, Ps, method=method)   # warm-up: plans and step graphs
+        best = None
+        for _ in range(reps):
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            out = prob.solve_batch(ctx, chans, Ps, method=method)
+            ctx.synchronize()
+            dt = time.perf_counter() - t1
+            best = dt if best is None else min(best, dt)
+        res[name] = round(len(chans) * len(T) / best, 2)
+        U[name] = [u for (u, _, _) in out]
+    rel = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(U["mixed"], U["fp64"]))
+    return {"workload": BATCH_WORKLOAD, "unit": "channel-solves/s", "channel_solves": len(chans) * len(T),
+            "mixed": res["mixed"], "fp64": res["fp64"], "mixed_vs_fp64_max_rel": rel,
+            "p123_build_s_all_channels": round(t_build, 2)}
+
+
 def product_roofline(stats, kernel):
     """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
     algorithmic bytes per launch (every block value once, the S panel read and the U panel
@@ -116,6 +152,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the all-channel batched line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
@@ -156,6 +193,8 @@ def main():
         "roofline": product_roofline(st_mix, "k_cellmfma<float, 2, 1, 4>"),
         "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, 2, 1, 2>"),
     }
+    if not args.no_batched:
+        out["batched"] = batched(ctx)
     if not args.no_cpu:
         G = ch.resolvent(ctx)
         cb, U_cpu = cpu_baseline(prob, ch, P, G)

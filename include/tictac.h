@@ -278,11 +278,47 @@ tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_u
                          const double *V_coup, const tt_swp_states *swp, tt_batch_item *items,
                          int n_items, const tt_solve_opts *opts);
 
+/* ------------------------------------------------------------ derivatives
+ * No reference counterpart (the reference has no derivatives).  For a potential that is
+ * linear in its first n_theta parameters (the "synthetic" model: its 12 strengths, with the
+ * ranges beta fixed), V_WP(theta) = sum_i theta_i V_i with the basis matrices
+ * V_i = V_WP(theta = e_i) filled here: Vb_unco n_theta*num_unco*Np*Np, Vb_coup
+ * n_theta*num_coup*4*Np*Np.  params: the full parameter vector (its first n_theta entries
+ * are ignored); the potential keeps params on return. */
+tt_status tt_potential_basis_matrices(const tt_run_params *rp, const tt_fwp_states *fwp,
+                                      const tt_pw_states *pw_full, tt_potential *pot, const double *params,
+                                      int n_params, int n_theta, double *Vb_unco, double *Vb_coup);
+
+typedef struct {
+    double ms_forward;          /* V, SWP states, G and the FP64 row solve            */
+    double ms_total;            /* forward + adjoint column solve + assembly          */
+    int n_apply_forward;        /* row-operator applications                          */
+    int n_apply_adjoint;        /* column-operator applications                       */
+    double residual_forward, residual_adjoint;
+} tt_grad_stats;
+
+/* U(theta) of one 3N channel (index into pw_full) at the on-shell bins of T_lab, and, when
+ * dU_dtheta is not NULL, dU/dtheta_i for i < n_theta by one adjoint (column) solve:
+ * V = sum_i theta_i V_i, SWP states, on-shell bins and G are rebuilt for theta, the on-shell
+ * U solved by FP64 GMRES (opts->tol).  The derivative includes the change of the SWP basis
+ * (eigenvectors, bin edges, deuteron energy) and of G; the choice of on-shell bins is held
+ * fixed.  U: nd*nd*nE complex ([d_row][d_col][j], as tt_solve_faddeev_equations);
+ * dU_dtheta: n_theta consecutive blocks of that layout (block i at offset i*nd*nd*nE);
+ * capacity: complex elements U can hold (dU_dtheta must hold n_theta*capacity); n_out:
+ * nd*nd*nE, the block size written. */
+tt_status tt_solve_gradient(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_states *fwp,
+                            const tt_pw_states *pw_full, int channel, const tt_csc *P123,
+                            const double *Vb_unco, const double *Vb_coup, int n_theta,
+                            const double *theta, const double *T_lab, int n_T_lab, double *U,
+                            double *dU_dtheta, int capacity, int *n_out, const tt_solve_opts *opts,
+                            tt_grad_stats *stats);
+
 /* Raw kernel-matrix application for tests and profiling: Y = (X .* G_sel) A in
  * the reference's row form (CPP/solve_faddeev.cpp:956-1088), r rows.
  * X, Y: host, r*dim complex (row-major).  g_index[k] selects the energy of row k
  * (-1: no G factor).  G: host nE*dim complex.  flags: bit 0 = FP32 products,
- * bit 1 = gather over the CSC instead of the cell blocks. */
+ * bit 1 = gather over the CSC instead of the cell blocks, bit 2 = the column operator
+ * Y_k = A (G_k .* X_k) (products with P123 itself) instead of the row form. */
 tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123,
                         const double *V_unco, const double *V_coup, const tt_swp_states *swp,
                         const tt_pw_states *chn, const double *G, int nE, const int *g_index,

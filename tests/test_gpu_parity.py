@@ -115,6 +115,20 @@ def test_c1_against_golden_fixture(ctx, c1):
     assert np.abs(U - g["U"]).max() <= 1e-10 * np.abs(g["U"]).max()
 
 
+def test_column_operator_c2(ctx, c2):
+    """The column form Y_k = A (G_k .* X_k) (products with P123 itself through the transposed
+    cell blocks, used by the adjoint) against the oracle's A X."""
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(3, c2.ch.dim)) + 1j * rng.normal(size=(3, c2.ch.dim))
+    g = [0, -1, 1]
+    Gsel = np.stack([c2.G[k] if k >= 0 else np.ones(c2.ch.dim) for k in g])
+    Yo = c2.K.col_apply((X * Gsel).T).T
+    Y = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g, column=True)
+    assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
+    Y32 = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g, column=True, fp32=True)
+    assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
+
+
 def test_uploaded_reference_csc_gives_same_solution(ctx, c1):
     """A CSC supplied by the caller (the reference's format, here the oracle's P123) is
     converted to cell blocks on first use and solves to the same U."""
@@ -295,6 +309,99 @@ def test_batch_over_channels_and_energies_matches_single_solves(ctx):
     Uo = so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, rp.Nq_WP, rp.Np_WP)
     U = prob.solve_batch(ctx, [ch], [P], method=tt.SOLVE_GMRES)[0][0]
     assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max()
+
+
+@pytest.fixture(scope="module")
+def grad_case(ctx):
+    rp, pot, _ = synthetic_config(N=10, J_2N_max=1, two_J_3N_max=1)
+    c = Case()
+    c.rp, c.T = rp, [5.0, 50.0]
+    c.prob = tt.Problem(rp, pot)
+    c.ch = c.prob.channel(0, c.T)
+    c.P = c.ch.build_p123(ctx)
+    c.basis = c.prob.basis_matrices(12)
+    c.theta = tt.synthetic_params()[:12]
+    c.U, c.dU, c.st = c.prob.gradient(ctx, 0, c.P, c.theta, c.T, basis=c.basis)
+    return c
+
+
+def test_gradient_forward_is_the_solve(ctx, grad_case):
+    c = grad_case
+    U1, _ = c.ch.solve(ctx, c.P, None, method=tt.SOLVE_GMRES, device_resolvent=True)
+    assert np.abs(c.U - U1).max() <= 1e-10 * np.abs(U1).max()
+    assert c.st.residual_forward <= 1e-12 and c.st.residual_adjoint <= 1e-12
+
+
+def test_gradient_matches_central_differences(ctx, grad_case):
+    """Adjoint dU/dtheta_i against central differences of the library's own U(theta),
+    h = 1e-6 |theta_i|; the FD error (truncation ~h^2, GMRES tolerance / h) is ~1e-6."""
+    c = grad_case
+    for i in range(12):
+        h = 1e-6 * abs(c.theta[i])
+        tp, tm = c.theta.copy(), c.theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        Up = c.prob.gradient(ctx, 0, c.P, tp, c.T, basis=c.basis, grad=False)[0]
+        Um = c.prob.gradient(ctx, 0, c.P, tm, c.T, basis=c.basis, grad=False)[0]
+        fd = (Up - Um) / (2 * h)
+        scale = max(np.abs(fd).max(), np.abs(c.dU[i]).max(), 1e-300)
+        assert np.abs(c.dU[i] - fd).max() <= 1e-5 * scale, (i, np.abs(c.dU[i] - fd).max() / scale)
+
+
+def test_gradient_against_dense_reference(ctx, grad_case):
+    """Dense FP64 reference: central differences of the oracle's own chain (numpy SWP
+    eigenvectors and edges, closed-form resolvent, dense A = C^T 2P VC, dense LU solve) on the
+    same basis potential; the oracle has independent eigen-solvers and no Krylov tolerance."""
+    from oracle import states as ost
+    from oracle import swp as osw
+    c = grad_case
+    rp = c.rp
+    Np = rp.Np_WP
+    nu, nc = c.prob.nu, c.prob.nc
+    fwp = ost.make_fwp(Np, rp.Nq_WP, rp.chebyshev_s)
+    pw_full = helpers_pw_full(c.prob)
+    pw = oracle_pw(c.ch)
+    val, row, ptr = c.P.to_host()
+    Pm = so.p_matrix(val, row, ptr, c.ch.dim)
+    tf, ib = flags(c.prob)
+
+    def U_oracle(theta):
+        Vu = np.tensordot(theta, c.basis[0], 1).reshape(nu, Np, Np)
+        Vc = np.tensordot(theta, c.basis[1], 1).reshape(nc, 2 * Np, 2 * Np)
+        swp = osw.make_swp_states(fwp, pw_full, Vu, Vc, tf, ib)
+        os_ = osw.find_on_shell_bins(fwp.q_edges, swp.E_bound, c.T)
+        deut = osw.find_deuteron_channels(pw)
+        G = np.stack([rs.resolvent_array(e, pw, swp, tf, ib) for e in os_.E_com + swp.E_bound])
+        A = so.KernelA(pw, swp, Vu, Vc, Pm, tf, ib).dense()
+        return so.dense_solve(A, G, deut, os_.q_idx, rp.Nq_WP, Np)
+
+    U0 = U_oracle(c.theta)
+    assert np.abs(U0 - c.U).max() <= 1e-9 * np.abs(U0).max()
+    for i in (0, 3, 4, 9):   # a Yukawa, the 1S0 and 3S1 contacts, a separable
+        h = 1e-5 * abs(c.theta[i])
+        tp, tm = c.theta.copy(), c.theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (U_oracle(tp) - U_oracle(tm)) / (2 * h)
+        assert np.abs(c.dU[i] - fd).max() <= 1e-5 * np.abs(fd).max(), i
+
+
+def test_gradient_torch_autograd_gradcheck(ctx, grad_case):
+    """torch.autograd.gradcheck of the autograd.Function over relative parameter changes
+    (theta = theta0 (1 + s)), so one eps suits strengths of every size."""
+    import torch
+    from tictac.adjoint import ChannelGradient, u_matrix
+    c = grad_case
+    solver = ChannelGradient(c.prob, ctx, 0, c.P, c.T, basis=c.basis)
+    th0 = torch.tensor(c.theta, dtype=torch.float64)
+    s = torch.zeros(12, dtype=torch.float64, requires_grad=True)
+    fn = lambda s_: u_matrix(th0 * (1 + s_), solver).reshape(-1)
+    assert torch.autograd.gradcheck(fn, (s,), eps=1e-6, atol=1e-7, rtol=1e-4, fast_mode=False)
+
+
+def helpers_pw_full(prob):
+    from helpers import oracle_pw_full
+    return oracle_pw_full(prob)
 
 
 def test_error_paths_on_device(ctx, c1):

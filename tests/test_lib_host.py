@@ -122,3 +122,18 @@ def test_error_statuses():
     with pytest.raises(tt.TictacError) as e:
         tt.Problem(rp, tt.Potential("malfliet_tjon"))
     assert e.value.code == L.TT_ERR_ARG
+
+
+def test_synthetic_potential_is_linear_in_its_strengths():
+    """V_WP(theta) = sum_i theta_i V_i with the basis of tt_potential_basis_matrices (the
+    premise of tt_solve_gradient)."""
+    rp, pot, _ = synthetic_config(N=8, J_2N_max=3, two_J_3N_max=1)
+    prob = tt.Problem(rp, pot)
+    Vb_u, Vb_c = prob.basis_matrices(12)
+    th = tt.synthetic_params()[:12]
+    n = rp.Np_WP
+    Vu = prob.V_unco[:prob.nu * n * n]
+    Vc = prob.V_coup[:prob.nc * 4 * n * n]
+    assert np.abs(np.tensordot(th, Vb_u, 1) - Vu).max() <= 1e-14 * np.abs(Vu).max()
+    assert np.abs(np.tensordot(th, Vb_c, 1) - Vc).max() <= 1e-14 * np.abs(Vc).max()
+    assert np.array_equal(prob.pot.params, tt.synthetic_params())   # restored

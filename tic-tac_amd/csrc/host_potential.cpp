@@ -291,4 +291,31 @@ tt_status tt_calculate_potential_matrices(const tt_run_params *rp, const tt_fwp_
     TT_GUARD_END
 }
 
+tt_status tt_potential_basis_matrices(const tt_run_params *rp, const tt_fwp_states *fwp, const tt_pw_states *pw,
+                                      tt_potential *pot, const double *params, int n_params, int n_theta,
+                                      double *Vb_unco, double *Vb_coup) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(rp && fwp && pw && pot && params && Vb_unco && n_theta > 0 && n_theta <= n_params, "null argument");
+    int nu, nc;
+    num_2N_states(*rp, nu, nc);
+    TT_REQUIRE(nc == 0 || Vb_coup, "null argument");
+    const size_t Np = (size_t)fwp->Np_WP, su = (size_t)nu * Np * Np, sc = (size_t)nc * 4 * Np * Np;
+    std::vector<double> p(params, params + n_params);
+    try {
+        for (int i = 0; i < n_theta; ++i) {
+            std::fill(p.begin(), p.begin() + n_theta, 0.0);
+            p[i] = 1.0;
+            pot->impl->update(p.data(), n_params);
+            tt_status st = tt_calculate_potential_matrices(rp, fwp, pw, pot, Vb_unco + (size_t)i * su,
+                                                           nc ? Vb_coup + (size_t)i * sc : nullptr);
+            if (st != TT_OK) fail(st, tt_last_error());
+        }
+    } catch (...) {
+        pot->impl->update(params, n_params);
+        throw;
+    }
+    pot->impl->update(params, n_params);
+    TT_GUARD_END
+}
+
 }  // extern "C"

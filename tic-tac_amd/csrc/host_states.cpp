@@ -158,6 +158,52 @@ std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_un
     return B;
 }
 
+// Partner lists of the embedded operator B = embed_2N(M) without forming it: for each alpha
+// a the partners a' whose Np x Np block is not all zero, block[p][p'] = B[(a,p),(a',p')]
+// (transpose: B[(a',p'),(a,p)]).
+void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, const double *M_coup,
+                     const tt_run_params &rp, bool transpose, std::vector<int> &ptr, std::vector<int> &alpha,
+                     std::vector<double> &blocks) {
+    const int Na = chn.Nalpha;
+    ptr.assign(Na + 1, 0);
+    alpha.clear();
+    blocks.clear();
+    std::vector<double> blk((size_t)Np * Np);
+    for (int a = 0; a < Na; ++a) {
+        for (int b = 0; b < Na; ++b) {
+            const int ar = transpose ? b : a, ac = transpose ? a : b;  // element B[(ar,.),(ac,.)]
+            int Lr = chn.L_2N[ar], Sr = chn.S_2N[ar], Jr = chn.J_2N[ar], Tr = chn.T_2N[ar];
+            int Lc = chn.L_2N[ac];
+            bool checkL = (rp.tensor_force && std::abs(Lr - Lc) <= 2) || Lr == Lc;
+            if (!(Tr == chn.T_2N[ac] && Jr == chn.J_2N[ac] && Sr == chn.S_2N[ac] && checkL &&
+                  chn.L_1N[ar] == chn.L_1N[ac] && chn.two_J_1N[ar] == chn.two_J_1N[ac]))
+                continue;
+            bool via_L, via_T;
+            coupling_kind(Lr, Sr, Jr, Lc, rp, via_L, via_T);
+            bool nz = false;
+            for (int p = 0; p < Np; ++p)
+                for (int pp = 0; pp < Np; ++pp) {
+                    const int pr = transpose ? pp : p, pc = transpose ? p : pp;
+                    double v;
+                    if (via_L || via_T) {
+                        int idx = unique_2N_idx(Lr, Sr, Jr, Tr, true, rp);
+                        int hr = half_of(chn, ar, via_L), hc = half_of(chn, ac, via_L);
+                        v = M_coup[(size_t)idx * 4 * Np * Np + (size_t)(hr * Np + pr) * 2 * Np + hc * Np + pc];
+                    } else {
+                        int idx = unique_2N_idx(Lr, Sr, Jr, Tr, false, rp);
+                        v = M_unco[(size_t)idx * Np * Np + (size_t)pr * Np + pc];
+                    }
+                    blk[(size_t)p * Np + pp] = v;
+                    nz = nz || v != 0.0;
+                }
+            if (!nz) continue;
+            alpha.push_back(b);
+            blocks.insert(blocks.end(), blk.begin(), blk.end());
+        }
+        ptr[a + 1] = (int)alpha.size();
+    }
+}
+
 void swp_edge_offsets(const tt_pw_states &chn, const tt_swp_states &swp, const tt_run_params &rp,
                       std::vector<int> &which, std::vector<int> &offset) {
     const int Np = swp.Np_WP;

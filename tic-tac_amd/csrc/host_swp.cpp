@@ -59,8 +59,9 @@ static void jacobi_eigh(int n, std::vector<double> A, std::vector<double> &ev, s
     }
 }
 
-// CPP/make_swp_states.cpp:146-183
-static void swp_edges(const std::vector<double> &ev, int Np, bool coupled, bool chn_3S1, double *e) {
+// CPP/make_swp_states.cpp:146-183.  Linear in ev (all constants are zero), so applied to
+// eigenvalue derivatives it gives the edge derivatives.
+void swp_edges(const std::vector<double> &ev, int Np, bool coupled, bool chn_3S1, double *e) {
     e[0] = 0;
     if (coupled) e[Np + 1] = 0;
     for (int i = 0; i < Np - 1; ++i) {

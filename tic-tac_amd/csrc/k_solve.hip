@@ -63,6 +63,58 @@ DeviceCSC::~DeviceCSC() {
     if (cell_bra) (void)hipFree(cell_bra);
     if (blk) (void)hipFree(blk);
     if (blk32) (void)hipFree(blk32);
+    if (t_ptr) (void)hipFree(t_ptr);
+    if (t_partner) (void)hipFree(t_partner);
+    if (t_blk) (void)hipFree(t_blk);
+    if (t_blk32) (void)hipFree(t_blk32);
+}
+
+// tblk[c'][i][j] = blk[src[c']][j][i]
+__global__ void k_transpose_cells(const int *__restrict__ src, const double *__restrict__ blk,
+                                  double *__restrict__ tblk, int64_t ncell, int Na) {
+    const int64_t NaNa = (int64_t)Na * Na;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * NaNa) return;
+    const int64_t c = t / NaNa;
+    const int e = (int)(t - c * NaNa), i = e / Na, j = e - i * Na;
+    tblk[t] = blk[(int64_t)src[c] * NaNa + (int64_t)j * Na + i];
+}
+
+void build_transposed_cells(DeviceCSC &P, hipStream_t s) {
+    TT_REQUIRE(P.Na > 0 && P.ket_ptr && P.cell_bra && P.blk, "transposed cells need the cell-block form");
+    if (P.t_blk) return;
+    const int NqNp = P.NqNp;
+    const int64_t ncell = P.ncell;
+    std::vector<int> kp(NqNp + 1), bra(std::max<int64_t>(ncell, 1));
+    TT_HIP(hipMemcpyAsync(kp.data(), P.ket_ptr, sizeof(int) * (NqNp + 1), hipMemcpyDeviceToHost, s));
+    if (ncell) TT_HIP(hipMemcpyAsync(bra.data(), P.cell_bra, sizeof(int) * ncell, hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    // counting sort of the cells by bra; kets come out ascending inside each bra
+    std::vector<int> tp(NqNp + 1, 0), partner(std::max<int64_t>(ncell, 1)), src(std::max<int64_t>(ncell, 1));
+    for (int64_t c = 0; c < ncell; ++c) ++tp[bra[c] + 1];
+    for (int b = 0; b < NqNp; ++b) tp[b + 1] += tp[b];
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for (int k = 0; k < NqNp; ++k)
+        for (int c = kp[k]; c < kp[k + 1]; ++c) {
+            const int pos = fill[bra[c]]++;
+            partner[pos] = k;
+            src[pos] = c;
+        }
+    const int64_t nval = ncell * P.Na * P.Na;
+    TT_HIP(hipMalloc(&P.t_ptr, sizeof(int) * (NqNp + 1)));
+    TT_HIP(hipMalloc(&P.t_partner, sizeof(int) * std::max<int64_t>(ncell, 1)));
+    TT_HIP(hipMalloc(&P.t_blk, sizeof(double) * (nval + BLK_SLACK)));
+    TT_HIP(hipMemcpyAsync(P.t_ptr, tp.data(), sizeof(int) * (NqNp + 1), hipMemcpyHostToDevice, s));
+    TT_HIP(hipMemcpyAsync(P.t_partner, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice, s));
+    TT_HIP(hipMemsetAsync(P.t_blk + nval, 0, sizeof(double) * BLK_SLACK, s));
+    int *d_src = nullptr;
+    TT_HIP(hipMalloc(&d_src, sizeof(int) * src.size()));
+    TT_HIP(hipMemcpyAsync(d_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice, s));
+    if (nval)
+        hipLaunchKernelGGL(k_transpose_cells, dim3((unsigned)((nval + 255) / 256)), dim3(256), 0, s, d_src, P.blk,
+                           P.t_blk, ncell, P.Na);
+    TT_HIP(hipStreamSynchronize(s));
+    (void)hipFree(d_src);
 }
 
 // CSC -> cell blocks: column c = (alpha, ket), row = (alpha', bra); every (bra, ket) pair
@@ -489,15 +541,38 @@ __global__ void k_cell_reduce(const int *__restrict__ ket_item_ptr, const T *__r
     U[t] = scale * acc;
 }
 
-// Work items of the cell-block product: runs of at most item_len (<= 64) cells of one ket.
+// One orientation of the cell-block form: cells grouped by output bin (kets for the product
+// with P123^T, bras for the product with P123), the partner bin of each cell and the blocks
+// (rows: input alpha, columns: output alpha).
+struct CellView {
+    int Na = 0, NqNp = 0;
+    int64_t ncell = 0;
+    const int *grp_ptr = nullptr;
+    const int *partner = nullptr;
+    const double *blk = nullptr;
+    const float *blk32 = nullptr;
+};
+static CellView cells_of(const DeviceCSC &P, bool transposed) {
+    CellView v;
+    v.Na = P.Na;
+    v.NqNp = P.NqNp;
+    v.ncell = P.ncell;
+    v.grp_ptr = transposed ? P.t_ptr : P.ket_ptr;
+    v.partner = transposed ? P.t_partner : P.cell_bra;
+    v.blk = transposed ? P.t_blk : P.blk;
+    v.blk32 = transposed ? P.t_blk32 : P.blk32;
+    return v;
+}
+
+// Work items of the cell-block product: runs of at most item_len (<= 64) cells of one group.
 struct CellItems {
     int n = 0;
     DevBuf<int2> items;
     DevBuf<int> ket_item_ptr;
-    void build(const DeviceCSC &P, hipStream_t s, int item_len = CELL_ITEM) {
+    void build(const CellView &P, hipStream_t s, int item_len = CELL_ITEM) {
         TT_REQUIRE(item_len >= 1 && item_len <= 64, "work item length must be in [1, 64]");
         std::vector<int> kp(P.NqNp + 1);
-        TT_HIP(hipMemcpyAsync(kp.data(), P.ket_ptr, sizeof(int) * (P.NqNp + 1), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipMemcpyAsync(kp.data(), P.grp_ptr, sizeof(int) * (P.NqNp + 1), hipMemcpyDeviceToHost, s));
         TT_HIP(hipStreamSynchronize(s));
         std::vector<int2> it;
         std::vector<int> kip(P.NqNp + 1, 0);
@@ -512,17 +587,17 @@ struct CellItems {
 };
 
 template <typename T, int MT, int NT>
-static void launch_cellmfma(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S,
+static void launch_cellmfma(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
                             T *Upart, int R2) {
     const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
-    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, cell_un<T>()>), grid, block, 0, s, ci.items.p, P->cell_bra, blk, S, Upart,
+    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, cell_un<T>()>), grid, block, 0, s, ci.items.p, P->partner, blk, S, Upart,
                        P->Na, R2);
 }
 
 // N tiles per pass: up to 64 real components (32 complex right-hand sides) stream the blocks
 // once; wider batches take ceil(R2 / 64) passes.
 template <typename T, int MT>
-static void launch_cellmfma_nt(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S,
+static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
                                T *Upart, int R2) {
     if (R2 <= 16)
         launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
@@ -535,7 +610,7 @@ static void launch_cellmfma_nt(hipStream_t s, const DeviceCSC *P, const CellItem
 // Upart = the work items' partial panels (items * Nalpha * R2); when U is given also
 // U = scale * (sum of each ket's item panels), the full product in cell-major order.
 template <typename T>
-static void launch_cellmm(hipStream_t s, const DeviceCSC *P, const CellItems &ci, const T *blk, const T *S, T *U,
+static void launch_cellmm(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S, T *U,
                           T *Upart, int R2, T scale) {
     TT_REQUIRE(P->Na >= 1 && P->Na <= 64, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
     if (ci.n > 0) {
@@ -764,6 +839,57 @@ __global__ void k_scale_by_norm(double2 *__restrict__ Y, const double2 *__restri
     Y[t] = make_double2(f * x.x, f * x.y);
 }
 
+// out[idx][k] = in[idx][k] * G[gk][idx] + (idx == unit[k])   (e + x .* G for the adjoint)
+__global__ void k_unit_plus_G(const double2 *__restrict__ in, double2 *__restrict__ out, const double2 *__restrict__ G,
+                              const int *__restrict__ gidx, const int64_t *__restrict__ unit, int64_t dim, int r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * r) return;
+    const int64_t idx = t / r;
+    const int k = (int)(t - idx * r);
+    const double2 x = in[t], g = G[(int64_t)gidx[k] * dim + idx];
+    double2 y = make_double2(x.x * g.x - x.y * g.y, x.x * g.y + x.y * g.x);
+    if (idx == unit[k]) y.x += 1.0;
+    out[t] = y;
+}
+
+// partial[pair][b] = sum over block b's slice of x of A[x][k] B[x][k'] (W[j][x]), the bilinear
+// (unconjugated) products of the adjoint; x runs over the reference index idx = (a, qp), and A
+// or B is read in the cell-major order [(qp)][a] when its flag is set.  Summed over b by
+// k_dots_reduce (fixed order).
+__global__ void k_pair_dots(const double2 *__restrict__ A, int a_cell, const double2 *__restrict__ B, int b_cell,
+                            const double2 *__restrict__ W, const int4 *__restrict__ pairs, int Na, int NqNp, int r,
+                            double2 *__restrict__ partial) {
+    __shared__ double2 sh[256];
+    const int pr = blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+    const int4 pq = pairs[pr];
+    const int64_t dim = (int64_t)Na * NqNp;
+    double sr = 0.0, si = 0.0;
+    for (int64_t x = (int64_t)b * blockDim.x + threadIdx.x; x < dim; x += (int64_t)nb * blockDim.x) {
+        const int a = (int)(x / NqNp), qp = (int)(x - (int64_t)a * NqNp);
+        const int64_t xc = (int64_t)qp * Na + a;
+        const double2 u = A[(a_cell ? xc : x) * r + pq.x], v = B[(b_cell ? xc : x) * r + pq.y];
+        double pr_ = u.x * v.x - u.y * v.y, pi_ = u.x * v.y + u.y * v.x;
+        if (W) {
+            const double2 w = W[(int64_t)pq.z * dim + x];
+            const double tr = pr_ * w.x - pi_ * w.y;
+            pi_ = pr_ * w.y + pi_ * w.x;
+            pr_ = tr;
+        }
+        sr += pr_;
+        si += pi_;
+    }
+    sh[threadIdx.x] = make_double2(sr, si);
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sh[threadIdx.x].x += sh[threadIdx.x + st].x;
+            sh[threadIdx.x].y += sh[threadIdx.x + st].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)pr * nb + b] = sh[0];
+}
+
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------------ the operator
@@ -789,6 +915,10 @@ struct RowOperator {
     int r_cap = 0;
     int c_npart_max = 1, vc_npart_max = 1;
     bool cells = true;   // cell-block product (default) or gather over the CSC
+    // col: the column operator z -> A z = C_op^T (2 P123) VC_op z (products with P123 itself,
+    // through the transposed cells) instead of the row form y -> y A
+    bool col = false;
+    CellView cv;
     CellItems items;
     DevBuf<double> upart;  // partial panels of the work items (FP64 and FP32 share it)
     // bumped whenever a buffer the kernels address is (re)allocated: captured graphs hold
@@ -825,37 +955,13 @@ struct RowOperator {
         spmm_bytes_of.clear();
     }
 
-    static void block_list(const std::vector<double> &B, int Na, int Np, bool transpose, std::vector<int> &ptr,
-                           std::vector<int> &alpha, std::vector<double> &blocks) {
-        const int64_t n = (int64_t)Na * Np;
-        ptr.assign(Na + 1, 0);
-        alpha.clear();
-        blocks.clear();
-        for (int a = 0; a < Na; ++a) {
-            for (int ap = 0; ap < Na; ++ap) {
-                bool nz = false;
-                for (int p = 0; p < Np && !nz; ++p)
-                    for (int pp = 0; pp < Np && !nz; ++pp) {
-                        double v = transpose ? B[(size_t)(ap * Np + pp) * n + a * Np + p]
-                                             : B[(size_t)(a * Np + p) * n + ap * Np + pp];
-                        nz = v != 0.0;
-                    }
-                if (!nz) continue;
-                alpha.push_back(ap);
-                for (int p = 0; p < Np; ++p)
-                    for (int pp = 0; pp < Np; ++pp)
-                        blocks.push_back(transpose ? B[(size_t)(ap * Np + pp) * n + a * Np + p]
-                                                   : B[(size_t)(a * Np + p) * n + ap * Np + pp]);
-            }
-            ptr[a + 1] = (int)alpha.size();
-        }
-    }
-
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
                const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn,
-               int p123_format = TT_P123_CELLS) {
+               int p123_format = TT_P123_CELLS, bool col_ = false) {
         s = stream;
         cells = p123_format == TT_P123_CELLS;
+        col = col_;
+        TT_REQUIRE(!col || cells, "the column operator uses the cell-block form");
         Na = chn.Nalpha;
         Nq = swp.Nq_WP;
         Np = swp.Np_WP;
@@ -865,7 +971,11 @@ struct RowOperator {
         TT_REQUIRE(csc && csc->dim == dim, "P123 dimension does not match Nalpha*Nq*Np");
         if (cells && csc->Na == 0) build_cell_blocks(*const_cast<DeviceCSC *>(csc), Na, Nq, Np, s);
         TT_REQUIRE(!cells || (csc->Na == Na && csc->NqNp == Nq * Np), "P123 cell blocks do not match the channel");
-        if (cells) items.build(*csc, s);
+        if (cells) {
+            if (col) build_transposed_cells(*const_cast<DeviceCSC *>(csc), s);
+            cv = cells_of(*csc, col);
+            items.build(cv, s);
+        }
         int nu, nc;
         num_2N_states(rp, nu, nc);
         // V.C per 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array, lines 1828-1855)
@@ -883,12 +993,16 @@ struct RowOperator {
         for (int c = 0; c < nc; ++c)
             mm(V_coup + (size_t)c * 4 * Np * Np, swp.C_SWP_coup + (size_t)c * 4 * Np * Np,
                &VCc[(size_t)c * 4 * Np * Np], 2 * Np);
-        std::vector<double> Cop = embed_2N(chn, Np, swp.C_SWP_unco, swp.C_SWP_coup, rp);
-        std::vector<double> VCop = embed_2N(chn, Np, VCu.data(), VCc.data(), rp);
         std::vector<int> p1, a1, p2, a2;
         std::vector<double> b1, b2;
-        block_list(Cop, Na, Np, false, p1, a1, b1);   // S = C_op (X G)
-        block_list(VCop, Na, Np, true, p2, a2, b2);   // Y = VC_op^T U
+        const double *Cu = swp.C_SWP_unco, *Cc = swp.C_SWP_coup;
+        if (!col) {
+            embed_2N_blocks(chn, Np, Cu, Cc, rp, false, p1, a1, b1);                 // S = C_op (X G)
+            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, true, p2, a2, b2);   // Y = VC_op^T U
+        } else {
+            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, false, p1, a1, b1);  // S = VC_op (X G)
+            embed_2N_blocks(chn, Np, Cu, Cc, rp, true, p2, a2, b2);                   // Y = C_op^T U
+        }
         c_npart_max = vc_npart_max = 1;
         for (int a = 0; a < Na; ++a) {
             c_npart_max = std::max(c_npart_max, p1[a + 1] - p1[a]);
@@ -959,12 +1073,14 @@ struct RowOperator {
     void ensure_f32() {
         DeviceCSC *m = const_cast<DeviceCSC *>(P);
         if (cells) {
-            if (!m->blk32) {
+            float *&b32 = col ? m->t_blk32 : m->blk32;
+            if (!b32) {
                 const int64_t n = m->ncell * m->Na * m->Na;
-                TT_HIP(hipMalloc(&m->blk32, sizeof(float) * (n + BLK_SLACK)));
-                hipLaunchKernelGGL(k_to_f32, dim3(nblk(n + BLK_SLACK, 256)), dim3(256), 0, s, m->blk, m->blk32,
-                                   n + BLK_SLACK);
+                TT_HIP(hipMalloc(&b32, sizeof(float) * (n + BLK_SLACK)));
+                hipLaunchKernelGGL(k_to_f32, dim3(nblk(n + BLK_SLACK, 256)), dim3(256), 0, s, col ? m->t_blk : m->blk,
+                                   b32, n + BLK_SLACK);
             }
+            cv.blk32 = b32;
         } else if (!m->val32) {
             TT_HIP(hipMalloc(&m->val32, sizeof(float) * std::max<int64_t>(m->nnz, 1)));
             if (m->nnz) hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz, 256)), dim3(256), 0, s, m->val, m->val32, m->nnz);
@@ -1020,7 +1136,7 @@ struct RowOperator {
                                            gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
             if (timed) record(e1);
             if (cells)
-                launch_cellmm<double>(s, P, items, P->blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
+                launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
                                       2.0);
             else
                 launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
@@ -1036,7 +1152,7 @@ struct RowOperator {
                                           use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
             if (timed) record(e1);
             if (cells)
-                launch_cellmm<float>(s, P, items, P->blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                launch_cellmm<float>(s, &cv, items, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                      reinterpret_cast<float *>(upart.p), R2, 2.0f);
             else
                 launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
@@ -1052,6 +1168,17 @@ struct RowOperator {
             record(e3);
             if (!ext) spmm_bytes_of.push_back(product_bytes(r, fp32));
         }
+    }
+
+    // Ucell = 2 P^T (C_op X) for the row operator, 2 P (VC_op X) for the column operator: the
+    // first block op (no G) and the cell product, as a full cell-major panel [(q,p)][alpha][k].
+    void stage_product(const double2 *X, double2 *Ucell, int r) {
+        TT_REQUIRE(cells, "stage products use the cell-block form");
+        prepare(r, false);
+        launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, nullptr, gidx.p, Na,
+                                       Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);
+        launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p),
+                              reinterpret_cast<double *>(Ucell), upart.p, 2 * r, 2.0);
     }
 
     // Adds the timing of one application recorded into caller events (after they completed).
@@ -1488,6 +1615,7 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
 struct SolvePlan {
     uint64_t p_gen = 0, key = 0;
     int format = TT_P123_CELLS;
+    bool col = false;   // column operator (adjoint solves)
     uint64_t last_use = 0;
     RowOperator op;
     std::unique_ptr<Gmres> gm;
@@ -1544,12 +1672,13 @@ static uint64_t plan_key(const tt_run_params &rp, const double *V_unco, const do
 // The context's plan for these inputs: reused when the P123 object and every input hash
 // match, else built (evicting the least recently used plan beyond max_plans).
 static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC *P, const double *V_unco,
-                           const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format) {
+                           const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format,
+                           bool col = false) {
     if (!ctx->plans) ctx->plans = new PlanCache;
     PlanCache &c = *ctx->plans;
     const uint64_t key = plan_key(rp, V_unco, V_coup, swp, chn, format);
     for (auto &pl : c.plans)
-        if (pl->p_gen == P->gen && pl->key == key && pl->format == format) {
+        if (pl->p_gen == P->gen && pl->key == key && pl->format == format && pl->col == col) {
             pl->last_use = ++c.tick;
             return *pl;
         }
@@ -1562,10 +1691,11 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
         c.plans.erase(lru);
     }
     std::unique_ptr<SolvePlan> pl(new SolvePlan);
-    pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format);
+    pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format, col);
     pl->p_gen = P->gen;
     pl->key = key;
     pl->format = format;
+    pl->col = col;
     pl->last_use = ++c.tick;
     c.plans.push_back(std::move(pl));
     return *c.plans.back();
@@ -1896,6 +2026,205 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
         fail(TT_ERR_NOT_CONVERGED, "Krylov solve stopped at relative residual " + std::to_string(st.residual) +
                                        " above the requested tolerance (U holds the last iterate)");
 }
+// ------------------------------------------------------------------ gradient
+// U(theta) and dU/dtheta for one channel, V = sum_i theta_i V_i (no reference counterpart).
+//   forward:  V, SWP states, on-shell bins, G on the device, row GMRES y_k (1 - G_j A) = e_k^T A
+//   adjoint:  column GMRES (1 - A G_j) t_k = A e_k, z_k = e_k + G_j t_k, w_k = e_k + y_k G_j, and
+//     dU[dr][dc][j] = (dC_op w_k).(2P VC_op z_k') + (2P^T C_op w_k).(dVC_op z_k') + y_k.(dG_j t_k')
+//   with k = (dr, j), k' = (dc, j) and dVC = dV C + V dC per 2N channel; dC, the SWP edge and
+//   deuteron-energy changes come from first-order perturbation of the SWP states, dG from the
+//   closed forms.  The on-shell bins are held fixed (their choice is piecewise constant).
+// Cost: the forward solve, one column solve, two stage products and per-theta block ops.
+static void matmul_add(const double *A, const double *B, double *C, int n) {
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            const double a = A[(size_t)i * n + k];
+            if (a == 0.0) continue;
+            for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+        }
+}
+
+static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_states &fwp,
+                           const tt_pw_states &pw_full, int channel, const tt_csc *P123, const double *Vb_unco,
+                           const double *Vb_coup, int n_theta, const double *theta, const double *T_lab, int nT,
+                           double *U, double *dU, int capacity, int *n_out, const tt_solve_opts &o_in,
+                           tt_grad_stats *gs) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto ms_since = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+    int nu, nc;
+    num_2N_states(rp, nu, nc);
+    const int Np = fwp.Np_WP, Nq = fwp.Nq_WP;
+    const size_t su = (size_t)nu * Np * Np, sc = (size_t)nc * 4 * Np * Np;
+    std::vector<double> Vu(std::max<size_t>(su, 1), 0.0), Vc(std::max<size_t>(sc, 1), 0.0);
+    for (int i = 0; i < n_theta; ++i) {
+        for (size_t e = 0; e < su; ++e) Vu[e] += theta[i] * Vb_unco[(size_t)i * su + e];
+        for (size_t e = 0; e < sc; ++e) Vc[e] += theta[i] * Vb_coup[(size_t)i * sc + e];
+    }
+    std::vector<double> eu(std::max<size_t>((size_t)nu * (Np + 1), 1)), ec(std::max<size_t>((size_t)nc * 2 * (Np + 1), 1));
+    std::vector<double> Cu(std::max<size_t>(su, 1)), Cc(std::max<size_t>(sc, 1));
+    tt_swp_states swp;
+    std::memset(&swp, 0, sizeof(swp));
+    swp.e_SWP_unco = eu.data();
+    swp.e_SWP_coup = ec.data();
+    swp.C_SWP_unco = Cu.data();
+    swp.C_SWP_coup = Cc.data();
+    auto check = [](tt_status st) {
+        if (st != TT_OK) fail(st, tt_last_error());
+    };
+    check(tt_make_swp_states(&rp, &fwp, &pw_full, Vu.data(), Vc.data(), &swp));
+    tt_pw_states chn;
+    check(tt_pw_channel(&pw_full, channel, &chn));
+    tt_channel_os os;
+    check(tt_find_on_shell(&swp, &chn, T_lab, nT, &os));
+    struct OsGuard {
+        tt_channel_os *o;
+        ~OsGuard() { tt_channel_os_free(o); }
+    } os_guard{&os};
+    const int nd = os.num_deuteron_states, nE = os.num_T_lab, n = nd * nd * nE;
+    if (n_out) *n_out = n;
+    TT_REQUIRE(nd > 0 && nE > 0, "no on-shell elements to solve for");
+    TT_REQUIRE(capacity >= n, "U / dU capacity is smaller than nd*nd*nE");
+    tt_solve_opts o = o_in;
+    o.method = TT_SOLVE_GMRES;
+    o.resolvent_on_device = 1;
+    o.p123_format = TT_P123_CELLS;
+    tt_solve_stats st_f;
+    solve_channel(ctx, &rp, U, nullptr, P123, Vu.data(), Vc.data(), &swp, &os, &chn, o, &st_f);
+    tt_grad_stats g;
+    std::memset(&g, 0, sizeof(g));
+    g.ms_forward = ms_since();
+    g.n_apply_forward = st_f.n_apply;
+    g.residual_forward = st_f.residual;
+    if (!dU) {
+        g.ms_total = g.ms_forward;
+        if (gs) *gs = g;
+        return;
+    }
+    const DeviceCSC *P = dev_of(P123);
+    SolvePlan &rowp = get_plan(ctx, rp, P, Vu.data(), Vc.data(), swp, chn, TT_P123_CELLS, false);
+    SolvePlan &colp = get_plan(ctx, rp, P, Vu.data(), Vc.data(), swp, chn, TT_P123_CELLS, true);
+    RowOperator &R = rowp.op, &Cl = colp.op;
+    hipStream_t s = ctx->stream;
+    const OnshellLayout L = onshell_layout(os, Nq, Np);
+    const int r = L.r, Na = chn.Nalpha;
+    const int64_t dim = R.dim, ne = dim * r;
+    // the column operator takes the same G and energy of each system
+    double2 *Gc = Cl.G_buffer(nE);
+    TT_HIP(hipMemcpyAsync(Gc, R.G.p, sizeof(double2) * (size_t)nE * dim, hipMemcpyDeviceToDevice, s));
+    Cl.set_gidx(L.row_g);
+    Cl.collect();
+    Cl.n_apply = 0;
+    auto fit = [&](DevBuf<double2> &v) {
+        if (v.n < (size_t)ne) v.alloc(ne);
+    };
+    fit(colp.b);
+    fit(colp.x);
+    on_shell_rows_of_A(Cl, L, colp.b.p);   // A e_c
+    const int m = o.restart > 0 ? o.restart : 40;
+    Gmres &gc = colp.gmres(r, m, o.use_graphs != 0);
+    std::vector<double> bnorm = gc.norms(colp.b.p);
+    TT_HIP(hipMemsetAsync(colp.x.p, 0, sizeof(double2) * ne, s));
+    double res_a = 0.0;
+    gc.solve(colp.b.p, colp.x.p, o.tol, o.max_iter, false, res_a, bnorm);
+    g.residual_adjoint = res_a;
+    g.n_apply_adjoint = Cl.n_apply;
+    if (!(res_a <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "adjoint column solve stopped at relative residual " + std::to_string(res_a));
+    DevBuf<int64_t> unit;
+    unit.upload(L.row_idx.data(), r, s);
+    DevBuf<double2> z(ne), w(ne), Uw(ne), bz(ne), Dw(ne), Dz(ne);
+    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, colp.x.p, z.p, R.G.p, R.gidx.p, unit.p,
+                       dim, r);
+    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, rowp.x.p, w.p, R.G.p, R.gidx.p, unit.p,
+                       dim, r);
+    R.stage_product(w.p, Uw.p, r);    // 2 P^T C_op w   (cell-major)
+    Cl.stage_product(z.p, bz.p, r);   // 2 P VC_op z    (cell-major)
+    std::vector<int4> pairs;
+    for (int dr = 0; dr < nd; ++dr)
+        for (int dc = 0; dc < nd; ++dc)
+            for (int j = 0; j < nE; ++j) pairs.push_back(make_int4(dr * nE + j, dc * nE + j, j, 0));
+    const int npairs = (int)pairs.size();
+    DevBuf<int4> dpairs;
+    dpairs.upload(pairs.data(), npairs, s);
+    const int nbp = (int)std::min<int64_t>(256, nblk(dim, 256));
+    DevBuf<double2> part((size_t)npairs * nbp), tri((size_t)3 * npairs), dG((size_t)nE * dim);
+    std::vector<double> E(nE), fE(nE);
+    const double Md = kMn + kMp + swp.E_bound;
+    for (int j = 0; j < nE; ++j) {
+        E[j] = os.E_com[j] + swp.E_bound;
+        // E_com = (q_l^2 + q_u^2) / (4 mu1(E_bound)) at the bin midpoint: dE/dE_bound
+        fE[j] = 1.0 - os.E_com[j] * kMn / (Md * (kMn + Md));
+    }
+    std::vector<double> dGh((size_t)2 * nE * dim), dE(nE);
+    std::vector<double2> trih((size_t)3 * npairs);
+    std::vector<double> dVCu(std::max<size_t>(su, 1)), dVCc(std::max<size_t>(sc, 1));
+    DevBuf<int> bp1, ba1, bp2, ba2;
+    DevBuf<double> bb1, bb2;
+    for (int i = 0; i < n_theta; ++i) {
+        const double *dVu = Vb_unco + (size_t)i * su, *dVc = Vb_coup + (size_t)i * sc;
+        SwpDerivative d;
+        swp_derivative(rp, fwp, pw_full, Vu.data(), Vc.data(), swp, dVu, dVc, d);
+        std::fill(dVCu.begin(), dVCu.end(), 0.0);
+        std::fill(dVCc.begin(), dVCc.end(), 0.0);
+        for (int c = 0; c < nu; ++c) {
+            const size_t o1 = (size_t)c * Np * Np;
+            matmul_add(dVu + o1, Cu.data() + o1, dVCu.data() + o1, Np);
+            matmul_add(Vu.data() + o1, d.dC_unco.data() + o1, dVCu.data() + o1, Np);
+        }
+        for (int c = 0; c < nc; ++c) {
+            const size_t o1 = (size_t)c * 4 * Np * Np;
+            matmul_add(dVc + o1, Cc.data() + o1, dVCc.data() + o1, 2 * Np);
+            matmul_add(Vc.data() + o1, d.dC_coup.data() + o1, dVCc.data() + o1, 2 * Np);
+        }
+        std::vector<int> p1, a1, p2, a2;
+        std::vector<double> b1, b2;
+        embed_2N_blocks(chn, Np, d.dC_unco.data(), d.dC_coup.data(), rp, false, p1, a1, b1);
+        embed_2N_blocks(chn, Np, dVCu.data(), dVCc.data(), rp, false, p2, a2, b2);
+        int ne1 = 0, ne2 = 0;
+        for (int a = 0; a < Na; ++a) {
+            ne1 = std::max(ne1, p1[a + 1] - p1[a]);
+            ne2 = std::max(ne2, p2[a + 1] - p2[a]);
+        }
+        a1.push_back(0);
+        a2.push_back(0);
+        b1.push_back(0.0);
+        b2.push_back(0.0);
+        bp1.upload(p1.data(), p1.size(), s);
+        ba1.upload(a1.data(), a1.size(), s);
+        bb1.upload(b1.data(), b1.size(), s);
+        bp2.upload(p2.data(), p2.size(), s);
+        ba2.upload(a2.data(), a2.size(), s);
+        bb2.upload(b2.data(), b2.size(), s);
+        launch_block<double2, double2>(s, w.p, Dw.p, bp1.p, ba1.p, bb1.p, ne1, nullptr, R.gidx.p, Na, Nq, Np, r,
+                                       nullptr, 0, 1, nullptr, 1.0);   // dC_op w
+        launch_block<double2, double2>(s, z.p, Dz.p, bp2.p, ba2.p, bb2.p, ne2, nullptr, R.gidx.p, Na, Nq, Np, r,
+                                       nullptr, 0, 1, nullptr, 1.0);   // dVC_op z
+        for (int j = 0; j < nE; ++j) dE[j] = d.dE_bound * fE[j];
+        resolvent_derivative(rp, swp, chn, d, E.data(), dE.data(), nE, dGh.data());
+        dG.upload(reinterpret_cast<const double2 *>(dGh.data()), (size_t)nE * dim, s);
+        const dim3 pg((unsigned)nbp, (unsigned)npairs);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Dw.p, 1, bz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
+                           part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Uw.p, 1, Dz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
+                           part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p + npairs);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, rowp.x.p, 0, colp.x.p, 0, dG.p, dpairs.p, Na, Nq * Np,
+                           r, part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p + 2 * npairs);
+        TT_HIP(hipMemcpyAsync(trih.data(), tri.p, sizeof(double2) * 3 * npairs, hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < npairs; ++q) {
+            const int dr = pairs[q].x / nE, dc = pairs[q].y / nE, j = pairs[q].z;
+            const size_t e = (size_t)i * n + ((size_t)dr * nd + dc) * nE + j;
+            dU[2 * e] = trih[q].x + trih[npairs + q].x + trih[2 * npairs + q].x;
+            dU[2 * e + 1] = trih[q].y + trih[npairs + q].y + trih[2 * npairs + q].y;
+        }
+    }
+    Cl.collect();
+    g.ms_total = ms_since();
+    if (gs) *gs = g;
+}
 }  // namespace tt
 
 extern "C" {
@@ -1939,6 +2268,23 @@ tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_u
     TT_GUARD_END
 }
 
+tt_status tt_solve_gradient(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_states *fwp,
+                            const tt_pw_states *pw_full, int channel, const tt_csc *P123, const double *Vb_unco,
+                            const double *Vb_coup, int n_theta, const double *theta, const double *T_lab, int n_T_lab,
+                            double *U, double *dU_dtheta, int capacity, int *n_out, const tt_solve_opts *opts,
+                            tt_grad_stats *stats) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && fwp && pw_full && P123 && Vb_unco && theta && T_lab && U && n_theta > 0 && n_T_lab > 0,
+               "null argument");
+    TT_REQUIRE(channel >= 0 && channel < pw_full->N_chn_3N, "channel index out of range");
+    tt_solve_opts o;
+    tt_solve_opts_default(&o);
+    if (opts) o = *opts;
+    solve_gradient(ctx, *rp, *fwp, *pw_full, channel, P123, Vb_unco, Vb_coup, n_theta, theta, T_lab, n_T_lab, U,
+                   dU_dtheta, capacity, n_out, o, stats);
+    TT_GUARD_END
+}
+
 tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123, const double *V_unco,
                         const double *V_coup, const tt_swp_states *swp, const tt_pw_states *chn, const double *G,
                         int nE, const int *g_index, int r, const double *X, double *Y, int flags) {
@@ -1946,7 +2292,8 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
     TT_HIP(hipSetDevice(ctx->device));
     RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, (flags & 2) ? TT_P123_CSC : TT_P123_CELLS);
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, (flags & 2) ? TT_P123_CSC : TT_P123_CELLS,
+             (flags & 4) != 0);
     op.set_G(G, nE);
     std::vector<int> g(r, -1);
     bool useG = false;

@@ -64,6 +64,9 @@ void coupling_kind(int Lr, int Sr, int Jr, int Lc, const tt_run_params &rp, bool
 // CPP/solve_faddeev.cpp:1588-1927).  Dense (Nalpha*Np)^2 layout, row-major.
 std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_unco,
                              const double *M_coup, const tt_run_params &rp);
+void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, const double *M_coup,
+                     const tt_run_params &rp, bool transpose, std::vector<int> &ptr, std::vector<int> &alpha,
+                     std::vector<double> &blocks);
 
 // Per-alpha offset of the SWP energy edges used by the resolvent (CPP/make_resolvent.cpp:118-156):
 // returns for each alpha a pointer offset into either e_SWP_unco (flag 0) or e_SWP_coup (flag 1).
@@ -119,11 +122,37 @@ struct DeviceCSC {
     int *cell_bra = nullptr;         // ncell
     double *blk = nullptr;           // ncell * Na * Na + BLK_SLACK (zeroed slack)
     float *blk32 = nullptr;          // lazily made FP32 copy, same length
+    // transposed cell-block form for products with P123 itself (instead of its transpose):
+    // cells grouped by bra (bras ascending, kets ascending inside), partner = ket, blocks
+    // tblk[cell][alpha][alpha'] = blk[same cell][alpha'][alpha]; built on first use
+    int *t_ptr = nullptr;            // NqNp + 1
+    int *t_partner = nullptr;        // ncell
+    double *t_blk = nullptr;         // ncell * Na * Na + BLK_SLACK
+    float *t_blk32 = nullptr;
     ~DeviceCSC();
     static uint64_t next_gen();
 };
 // Builds the cell-block form of a CSC P123 (host pass over a device->host copy).
 void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s);
+// Builds the transposed cell-block form from the cell-block form (device transpose).
+void build_transposed_cells(DeviceCSC &P, hipStream_t s);
+
+// SWP bin edges from the (reordered) eigenvalues of one 2N channel, CPP/make_swp_states.cpp:146-183.
+void swp_edges(const std::vector<double> &ev, int Np, bool coupled, bool chn_3S1, double *e);
+
+// First-order change of the SWP states under a change dV of the potential matrices (same
+// layouts as V, C and e of tt_swp_states), with H = H0 + V diagonalised as in tt_make_swp_states.
+struct SwpDerivative {
+    std::vector<double> dC_unco, dC_coup, de_unco, de_coup;
+    double dE_bound = 0.0;
+};
+void swp_derivative(const tt_run_params &rp, const tt_fwp_states &fwp, const tt_pw_states &pw_full,
+                    const double *V_unco, const double *V_coup, const tt_swp_states &swp, const double *dV_unco,
+                    const double *dV_coup, SwpDerivative &d);
+// dG[j][idx] (interleaved complex, nE*dim) of the channel resolvent for edge changes d.de_*,
+// deuteron-energy change d.dE_bound and energy changes dE[j] at energies E[j].
+void resolvent_derivative(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
+                          const SwpDerivative &d, const double *E, const double *dE, int nE, double *dG);
 
 // Solve plans cached per context (operator set-up, Krylov workspaces, captured graphs).
 struct PlanCache;

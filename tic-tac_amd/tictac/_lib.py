@@ -71,6 +71,11 @@ class SolveStats(C.Structure):
                 ("n_spmm", C.c_int), ("ms_spmm", C.c_double), ("spmm_bytes", C.c_double)]
 
 
+class GradStats(C.Structure):
+    _fields_ = [("ms_forward", C.c_double), ("ms_total", C.c_double), ("n_apply_forward", C.c_int),
+                ("n_apply_adjoint", C.c_int), ("residual_forward", C.c_double), ("residual_adjoint", C.c_double)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("chn", C.POINTER(PwStates)), ("os", C.POINTER(ChannelOS)), ("P123", C.POINTER(CSC)),
                 ("U", _dp), ("stats", C.POINTER(SolveStats)), ("status", C.c_int)]
@@ -117,6 +122,11 @@ PROTOTYPES = {
                                              C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "tt_solve_batch": (C.c_int, [C.c_void_p, C.POINTER(RunParams), _dp, _dp, C.POINTER(SwpStates),
                                  C.POINTER(BatchItem), C.c_int, C.POINTER(SolveOpts)]),
+    "tt_potential_basis_matrices": (C.c_int, [C.POINTER(RunParams), C.POINTER(FwpStates), C.POINTER(PwStates),
+                                              C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]),
+    "tt_solve_gradient": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(FwpStates), C.POINTER(PwStates),
+                                    C.c_int, C.POINTER(CSC), _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp,
+                                    C.c_int, _ip, C.POINTER(SolveOpts), C.POINTER(GradStats)]),
     "tt_apply_rows": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(CSC), _dp, _dp, C.POINTER(SwpStates),
                                 C.POINTER(PwStates), _dp, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int]),
 }

@@ -156,6 +156,55 @@ class Problem:
         return [(U.view(np.complex128).reshape(ch.nd, ch.nd, ch.nE), st, items[i].status)
                 for i, (U, st, ch) in enumerate(zip(Us, stats, channels))]
 
+    def basis_matrices(self, n_theta=12):
+        """V_i = V_WP(theta = e_i) for the potential's linear strengths (tt_potential_basis_matrices):
+        arrays (n_theta, nu*Np*Np) and (n_theta, nc*4*Np*Np)."""
+        Np = self.rp.Np_WP
+        su, sc = self.nu * Np * Np, self.nc * 4 * Np * Np
+        Vb_u = np.zeros((n_theta, max(su, 1)))
+        Vb_c = np.zeros((n_theta, max(sc, 1)))
+        if su == 0 or (sc and Vb_c.shape[1] != sc):
+            raise ValueError("unexpected 2N sizes")
+        Vb_u = np.zeros(n_theta * su)
+        Vb_c = np.zeros(max(n_theta * sc, 1))
+        p = np.ascontiguousarray(self.pot.params, np.float64)
+        check(L.lib().tt_potential_basis_matrices(C.byref(self.rp), C.byref(self.fwp), C.byref(self.pw), self.pot._h,
+                                                  dptr(p), len(p), n_theta, dptr(Vb_u), dptr(Vb_c)))
+        return Vb_u.reshape(n_theta, su), Vb_c[:n_theta * sc].reshape(n_theta, sc)
+
+    def gradient(self, ctx, channel, P, theta, T_lab, basis=None, grad=True, tol=1e-12):
+        """U(theta) of channel `channel` at the on-shell bins of T_lab and, with grad, dU/dtheta
+        (tt_solve_gradient).  Returns (U[nd, nd, nE], dU[n_theta, nd, nd, nE] or None, stats)."""
+        theta = np.ascontiguousarray(theta, np.float64)
+        n_theta = len(theta)
+        Vb_u, Vb_c = basis if basis is not None else self.basis_matrices(n_theta)
+        Vb_u = np.ascontiguousarray(Vb_u, np.float64).ravel()
+        Vb_c = np.ascontiguousarray(Vb_c, np.float64).ravel()
+        if Vb_c.size == 0:
+            Vb_c = np.zeros(1)
+        T = np.ascontiguousarray(T_lab, np.float64)
+        cap = 3 * 3 * 2 * (self.rp.Nq_WP + 1)
+        U = np.zeros(2 * cap)
+        dU = np.zeros(2 * cap * n_theta) if grad else None
+        n_out = C.c_int(0)
+        o = L.SolveOpts()
+        L.lib().tt_solve_opts_default(C.byref(o))
+        o.tol = tol
+        st = L.GradStats()
+        check(L.lib().tt_solve_gradient(ctx.handle, C.byref(self.rp), C.byref(self.fwp), C.byref(self.pw), int(channel),
+                                        C.byref(P.csc), dptr(Vb_u), dptr(Vb_c), n_theta, dptr(theta), dptr(T), len(T),
+                                        dptr(U), dptr(dU) if grad else None, cap, C.byref(n_out), C.byref(o),
+                                        C.byref(st)))
+        n = n_out.value
+        ch = self.channel(channel, T_lab)
+        shape = (ch.nd, ch.nd, ch.nE)
+        assert n == ch.nd * ch.nd * ch.nE
+        Uc = U[:2 * n].view(np.complex128).reshape(shape)
+        dUc = None
+        if grad:
+            dUc = dU.view(np.complex128)[:n_theta * n].reshape((n_theta,) + shape)
+        return Uc, dUc, st
+
     def pw_arrays(self):
         n = self.pw.Nalpha
         names = ["L_2N", "S_2N", "J_2N", "T_2N", "L_1N", "two_J_1N", "two_J_3N", "two_T_3N", "P_3N"]
@@ -293,8 +342,8 @@ class Channel:
             return U, st, a.view(np.complex128).reshape(self.nd, self.nd, self.nE, nterms)
         return U, st
 
-    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False, csc=False):
-        """Y = (X .* G_sel) A for rows X (r, dim) complex."""
+    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False, csc=False, column=False):
+        """Y = (X .* G_sel) A for rows X (r, dim) complex; with column=True Y_k = A (G_k .* X_k)."""
         X = np.ascontiguousarray(X, np.complex128)
         r = X.shape[0]
         Y = np.zeros_like(X)
@@ -306,7 +355,8 @@ class Channel:
         check(L.lib().tt_apply_rows(ctx.handle, C.byref(self.prob.rp), C.byref(P.csc), dptr(self.prob.V_unco),
                                     dptr(self.prob.V_coup), C.byref(self.prob.swp), C.byref(self.view),
                                     dptr(Gf), nE, iptr(gi) if gi is not None else None, r,
-                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)), int(fp32) | (2 if csc else 0)))
+                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)),
+                                    int(fp32) | (2 if csc else 0) | (4 if column else 0)))
         return Y
 
     def __del__(self):
