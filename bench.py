@@ -94,17 +94,66 @@ def batched(ctx, reps=2):
             "p123_build_s_all_channels": round(t_build, 2)}
 
 
+GRAD_WORKLOAD = ("same synthetic NN, channels 2J=1,3 both parities (N_alpha 27, 47; dim 24300, 42300), Np=Nq=30, "
+                 "T_lab 5, 50, 150 MeV, 12 strengths theta (3 Yukawas, 6 contacts, 3 separables); FP64")
+
+
+def gradient_ratio(ctx, reps=3):
+    """Cost of U(theta) with dU/dtheta (tt_solve_gradient: forward row solve + adjoint column
+    solve + assembly) over the forward alone (same call without the derivative: V, SWP
+    states, G and the row solve), summed over the 4 channels, best of reps each."""
+    T = [5.0, 50.0, 150.0]
+    rp = tt.run_params(two_J_3N_max=3, J_2N_max=3, Np_WP=30, Nq_WP=30, chebyshev_s=tt.chebyshev_scale(30))
+    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
+    basis = prob.basis_matrices(12)
+    th = tt.synthetic_params()[:12]
+    t_f = t_g = 0.0
+    for c in range(prob.n_channels):
+        P = prob.channel(c, T).build_p123(ctx)
+        prob.gradient(ctx, c, P, th, T, basis=basis, grad=True)   # warm-up: plans and step graphs
+        bf = bg = None
+        for rep in range(reps):
+            theta = th * (1.0 + 1e-3 * (rep + 1))                  # a new theta each time
+            t0 = time.perf_counter()
+            prob.gradient(ctx, c, P, theta, T, basis=basis, grad=False)
+            t1 = time.perf_counter()
+            prob.gradient(ctx, c, P, theta, T, basis=basis, grad=True)
+            t2 = time.perf_counter()
+            bf = (t1 - t0) if bf is None else min(bf, t1 - t0)
+            bg = (t2 - t1) if bg is None else min(bg, t2 - t1)
+        t_f += bf
+        t_g += bg
+    return {"workload": GRAD_WORKLOAD, "grad_over_fwd": round(t_g / t_f, 3),
+            "adjoint_over_fwd": round((t_g - t_f) / t_f, 3), "ms_forward": round(1e3 * t_f, 2),
+            "ms_forward_plus_gradient": round(1e3 * t_g, 2)}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_cellmfma_synthetic_np30.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/ (scaled by a copy-kernel calibration), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch " + kernel + " = "):
+                return int(line.rsplit("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def product_roofline(stats, kernel):
     """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
     algorithmic bytes per launch (every block value once, the S panel read and the U panel
     written once, the cell indexes) over the launch time from the HIP events that bracket each
-    launch on the solve stream during the timed steps."""
+    launch on the solve stream during the timed steps; traffic = measured HBM bytes per launch."""
     ms = sum(s.ms_spmm for s in stats)
     nb = sum(s.spmm_bytes for s in stats)
     n = sum(s.n_spmm for s in stats)
     achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
             "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
             "bytes_per_launch": round(nb / max(n, 1))}
 
@@ -153,6 +202,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the all-channel batched line")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
@@ -195,6 +245,9 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(ctx)
+    if not args.no_grad:
+        out["gradient"] = gradient_ratio(ctx)
+        out["grad_over_fwd"] = out["gradient"]["grad_over_fwd"]
     if not args.no_cpu:
         G = ch.resolvent(ctx)
         cb, U_cpu = cpu_baseline(prob, ch, P, G)

@@ -1,7 +1,7 @@
 // Host side of dU/dtheta (no reference counterpart; the reference has no derivatives): the
-// first-order change of the SWP basis (C, bin edges, deuteron energy) and of the channel
-// resolvent G under a change of the potential matrices.  The device side (column solve,
-// products, pair dots) is in k_solve.hip.
+// first-order change of the SWP basis (C, bin edges, deuteron energy) under a change of the
+// potential matrices.  The device side (resolvent partials, column solve, products, pair
+// dots) is in k_solve.hip.
 #include <algorithm>
 #include <cmath>
 
@@ -62,6 +62,9 @@ void swp_derivative(const tt_run_params &rp, const tt_fwp_states &fwp, const tt_
                 dC = d.dC_unco.data() + (size_t)idx * n * n;
                 de = d.de_unco.data() + (size_t)idx * (Np + 1);
             }
+            bool any = false;
+            for (size_t e = 0; e < (size_t)n * n && !any; ++e) any = dV[e] != 0.0;
+            if (!any) continue;   // untouched channel: dC, d lambda and d e stay zero
             std::vector<double> H((size_t)n * n), dH((size_t)n * n);
             for (int i = 0; i < n; ++i)
                 for (int j = i; j < n; ++j) {
@@ -109,73 +112,6 @@ void swp_derivative(const tt_run_params &rp, const tt_fwp_states &fwp, const tt_
             const bool chn_3S1 = (Lr == 0 && Sr == 1 && Jr == 1 && Tr == 0);
             swp_edges(dl, Np, cm, chn_3S1, de);
             if (chn_3S1) d.dE_bound = dl[0];
-        }
-    }
-}
-
-// ------------------------------------------------------------------ resolvent derivative
-namespace {
-struct Dual {
-    double v, d;
-};
-inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
-inline Dual operator*(double c, Dual a) { return {c * a.v, c * a.d}; }
-inline Dual logabs(Dual a) { return {std::log(std::fabs(a.v)), a.d / a.v}; }
-inline double heav(Dual a) { return a.v < 0 ? 0.0 : 1.0; }
-// x ln|x| (its derivative ln|x| + 1 is finite except at 0, where the value is not used)
-inline Dual xlogx(Dual a) { return a * logabs(a); }
-// x H(x), derivative H(x)
-inline Dual xheav(Dual a) { return {a.v * heav(a), a.d * heav(a)}; }
-inline Dual mu1_dual(Dual Eb) {
-    // mu1 = Mn (Mn + Mp + Eb) / (2 Mn + Mp + Eb)
-    const Dual Md = Dual{kMn + kMp, 0.0} + Eb;
-    return Dual{kMn, 0.0} * Md / (Dual{kMn, 0.0} + Md);
-}
-}  // namespace
-
-// The closed forms of k_resolvent (CPP/make_resolvent.cpp:14-80) in forward-mode duals.
-void resolvent_derivative(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
-                          const SwpDerivative &d, const double *E, const double *dE, int nE, double *dG) {
-    std::vector<int> which, offset;
-    swp_edge_offsets(chn, swp, rp, which, offset);
-    const int Na = chn.Nalpha, Nq = swp.Nq_WP, Np = swp.Np_WP;
-    const int64_t dim = (int64_t)Na * Nq * Np;
-    for (int j = 0; j < nE; ++j) {
-        const Dual Ej{E[j], dE[j]};
-        for (int a = 0; a < Na; ++a) {
-            const double *e = (which[a] ? swp.e_SWP_coup : swp.e_SWP_unco) + offset[a];
-            const double *de = (which[a] ? d.de_coup.data() : d.de_unco.data()) + offset[a];
-            for (int q = 0; q < Nq; ++q) {
-                const double ql = swp.q_WP[q], qu = swp.q_WP[q + 1];
-                for (int p = 0; p < Np; ++p) {
-                    const Dual el{e[p], de[p]}, eu{e[p + 1], de[p + 1]};
-                    Dual re, im;
-                    if (el.v < 0) {
-                        const Dual Eb = el;
-                        const Dual m1 = mu1_dual(Eb);
-                        const Dual Eql = Dual{ql * ql, 0.0} / (2.0 * m1), Equ = Dual{qu * qu, 0.0} / (2.0 * m1);
-                        const Dual Dq = Equ - Eql;
-                        re = logabs((Eql + Eb - Ej) / (Equ + Eb - Ej)) / Dq;
-                        const Dual hh{(heav(Equ + Eb - Ej) - heav(Eql + Eb - Ej)) * (-M_PI), 0.0};
-                        im = hh / Dq;
-                    } else {
-                        const Dual m1 = mu1_dual(Dual{0.0, 0.0});
-                        const Dual Eql = Dual{ql * ql, 0.0} / (2.0 * m1), Equ = Dual{qu * qu, 0.0} / (2.0 * m1);
-                        const Dual Ep = 0.5 * (el + eu), Eq = 0.5 * (Eql + Equ);
-                        const Dual Dq = Equ - Eql, Dp = eu - el;
-                        const Dual DM = 0.5 * (Dp - Dq), DP = 0.5 * (Dp + Dq);
-                        const Dual D = Ep + Eq - Ej;
-                        re = (xlogx(D + DM) + xlogx(D - DM) - xlogx(D + DP) - xlogx(D - DP)) / (Dp * Dq);
-                        im = M_PI * (xheav(D + DM) + xheav(D - DM) - xheav(D + DP) - xheav(D - DP)) / (Dp * Dq);
-                    }
-                    const int64_t idx = ((int64_t)a * Nq + q) * Np + p;
-                    dG[2 * ((size_t)j * dim + idx)] = re.d;
-                    dG[2 * ((size_t)j * dim + idx) + 1] = im.d;
-                }
-            }
         }
     }
 }

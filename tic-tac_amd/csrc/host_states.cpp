@@ -159,11 +159,12 @@ std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_un
 }
 
 // Partner lists of the embedded operator B = embed_2N(M) without forming it: for each alpha
-// a the partners a' whose Np x Np block is not all zero, block[p][p'] = B[(a,p),(a',p')]
+// a the partners a' whose Np x Np block is not all zero (keep_zero: every structurally
+// coupled partner, so the lists depend on the channel only), block[p][p'] = B[(a,p),(a',p')]
 // (transpose: B[(a',p'),(a,p)]).
 void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, const double *M_coup,
                      const tt_run_params &rp, bool transpose, std::vector<int> &ptr, std::vector<int> &alpha,
-                     std::vector<double> &blocks) {
+                     std::vector<double> &blocks, bool keep_zero) {
     const int Na = chn.Nalpha;
     ptr.assign(Na + 1, 0);
     alpha.clear();
@@ -196,7 +197,7 @@ void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, cons
                     blk[(size_t)p * Np + pp] = v;
                     nz = nz || v != 0.0;
                 }
-            if (!nz) continue;
+            if (!nz && !keep_zero) continue;
             alpha.push_back(b);
             blocks.insert(blocks.end(), blk.begin(), blk.end());
         }

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <thread>
 
 #include "tt_internal.h"
 
@@ -178,9 +179,62 @@ void build_cell_blocks(DeviceCSC &P, int Na, int Nq, int Np, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ resolvent
-__device__ inline double heaviside(double v) { return v < 0 ? 0.0 : 1.0; }
 
 // One thread per (energy j, alpha, q, p); formulas of CPP/make_resolvent.cpp:14-80.
+// Forward-mode dual number (value, derivative) for the derivative of the closed forms.
+struct Dual {
+    double v, d;
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__host__ __device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__host__ __device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__host__ __device__ inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__host__ __device__ inline double value(double a) { return a; }
+__host__ __device__ inline double value(Dual a) { return a.v; }
+__host__ __device__ inline double logabs(double a) { return log(fabs(a)); }
+__host__ __device__ inline Dual logabs(Dual a) { return {log(fabs(a.v)), a.d / a.v}; }
+template <typename T>
+__host__ __device__ inline T cst(double c) {
+    return T(c);
+}
+template <>
+__host__ __device__ inline Dual cst<Dual>(double c) {
+    return {c, 0.0};
+}
+// x H(x) (derivative H(x))
+__host__ __device__ inline double xheav(double a) { return a < 0 ? 0.0 : a; }
+__host__ __device__ inline Dual xheav(Dual a) { return a.v < 0 ? Dual{0.0, 0.0} : a; }
+
+// One element of the channel resolvent at total energy E for the SWP bin [el, eu] and q bin
+// [ql, qu] (CPP/make_resolvent.cpp:14-80): bound-continuum when el < 0 (Eb = el, which enters
+// mu1), continuum-continuum otherwise (Eb = 0).
+template <typename T>
+__host__ __device__ inline void resolvent_element(T E, T el, T eu, double ql, double qu, T &re, T &im) {
+    const T one = cst<T>(1.0);
+    const T Mn = cst<T>(939.565), Mp = cst<T>(938.272);
+    if (value(el) < 0) {
+        const T Eb = el;
+        const T m1 = Mn * (Mn + Mp + Eb) / (Mn + Mn + Mp + Eb);
+        const T Eql = cst<T>(ql * ql) / (cst<T>(2.0) * m1), Equ = cst<T>(qu * qu) / (cst<T>(2.0) * m1);
+        const T Dq = Equ - Eql;
+        re = logabs((Eql + Eb - E) / (Equ + Eb - E)) / Dq;
+        const double hh = ((value(Equ + Eb - E) < 0 ? 0.0 : 1.0) - (value(Eql + Eb - E) < 0 ? 0.0 : 1.0)) * (-M_PI);
+        im = cst<T>(hh) / Dq;
+    } else {
+        const T m1 = Mn * (Mn + Mp) / (Mn + Mn + Mp);
+        const T Eql = cst<T>(ql * ql) / (cst<T>(2.0) * m1), Equ = cst<T>(qu * qu) / (cst<T>(2.0) * m1);
+        const T half = cst<T>(0.5);
+        const T Ep = half * (el + eu), Eq = half * (Eql + Equ);
+        const T Dq = Equ - Eql, Dp = eu - el;
+        const T DM = half * (Dp - Dq), DP = half * (Dp + Dq);
+        const T D = Ep + Eq - E;
+        const T a = D + DM, b = D - DM, c = D + DP, d = D - DP;
+        re = (a * logabs(a) + b * logabs(b) - c * logabs(c) - d * logabs(d)) / (Dp * Dq);
+        im = cst<T>(M_PI) * (xheav(a) + xheav(b) - xheav(c) - xheav(d)) / (Dp * Dq);
+    }
+    (void)one;
+}
+
 __global__ void k_resolvent(double2 *__restrict__ G, const double *__restrict__ e_unco,
                             const double *__restrict__ e_coup, const int *__restrict__ which,
                             const int *__restrict__ offset, const double *__restrict__ qe,
@@ -194,32 +248,51 @@ __global__ void k_resolvent(double2 *__restrict__ G, const double *__restrict__ 
     const int q = (int)((idx / Np) % Nq);
     const int p = (int)(idx % Np);
     const double *e = (which[a] ? e_coup : e_unco) + offset[a];
-    const double el = e[p], eu = e[p + 1];
-    const double Ej = E[j];
-    const double ql = qe[q], qu = qe[q + 1];
     double re, im;
-    if (el < 0) {
-        const double Eb = el;
-        const double m1 = 939.565 * (939.565 + 938.272 + Eb) / (939.565 + 939.565 + 938.272 + Eb);
-        const double Eql = ql * ql / (2 * m1), Equ = qu * qu / (2 * m1);
-        const double Dq = Equ - Eql;
-        re = log(fabs((Eql + Eb - Ej) / (Equ + Eb - Ej))) / Dq;
-        im = (heaviside(Equ + Eb - Ej) - heaviside(Eql + Eb - Ej)) * (-M_PI) / Dq;
-    } else {
-        const double m1 = 939.565 * (939.565 + 938.272) / (939.565 + 939.565 + 938.272);
-        const double Eql = ql * ql / (2 * m1), Equ = qu * qu / (2 * m1);
-        const double Ep = 0.5 * (el + eu), Eq = 0.5 * (Eql + Equ);
-        const double Dq = Equ - Eql, Dp = eu - el;
-        const double DM = 0.5 * (Dp - Dq), DP = 0.5 * (Dp + Dq);
-        const double D = Ep + Eq - Ej;
-        re = ((D + DM) * log(fabs(D + DM)) + (D - DM) * log(fabs(D - DM)) - (D + DP) * log(fabs(D + DP)) -
-              (D - DP) * log(fabs(D - DP))) /
-             (Dp * Dq);
-        im = ((D + DM) * heaviside(D + DM) + (D - DM) * heaviside(D - DM) - (D + DP) * heaviside(D + DP) -
-              (D - DP) * heaviside(D - DP)) *
-             M_PI / (Dp * Dq);
-    }
+    resolvent_element<double>(E[j], e[p], e[p + 1], qe[q], qe[q + 1], re, im);
     G[t] = make_double2(re, im);
+}
+
+// Partial derivatives of every G element with respect to its lower edge, upper edge and
+// the energy: Gp[0][j][idx], Gp[1][j][idx], Gp[2][j][idx] (the lower edge of a bound bin is
+// E_b, so its partial includes the change of mu1).
+__global__ void k_resolvent_partials(double2 *__restrict__ Gp, const double *__restrict__ e_unco,
+                                     const double *__restrict__ e_coup, const int *__restrict__ which,
+                                     const int *__restrict__ offset, const double *__restrict__ qe,
+                                     const double *__restrict__ E, int Na, int Nq, int Np, int nE) {
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * nE) return;
+    const int j = (int)(t / dim);
+    const int64_t idx = t - (int64_t)j * dim;
+    const int a = (int)(idx / (Nq * Np));
+    const int q = (int)((idx / Np) % Nq);
+    const int p = (int)(idx % Np);
+    const double *e = (which[a] ? e_coup : e_unco) + offset[a];
+    for (int w = 0; w < 3; ++w) {
+        Dual re, im;
+        resolvent_element<Dual>(Dual{E[j], w == 2 ? 1.0 : 0.0}, Dual{e[p], w == 0 ? 1.0 : 0.0},
+                                Dual{e[p + 1], w == 1 ? 1.0 : 0.0}, qe[q], qe[q + 1], re, im);
+        Gp[(int64_t)w * dim * nE + t] = make_double2(re.d, im.d);
+    }
+}
+
+// dG[j][idx] = dG/del de(a, p) + dG/deu de(a, p+1) + dG/dE dE[j]
+__global__ void k_resolvent_change(double2 *__restrict__ dG, const double2 *__restrict__ Gp,
+                                   const double *__restrict__ de_unco, const double *__restrict__ de_coup,
+                                   const int *__restrict__ which, const int *__restrict__ offset,
+                                   const double *__restrict__ dE, int Na, int Nq, int Np, int nE) {
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * nE) return;
+    const int j = (int)(t / dim);
+    const int64_t idx = t - (int64_t)j * dim;
+    const int a = (int)(idx / (Nq * Np));
+    const int p = (int)(idx % Np);
+    const double *de = (which[a] ? de_coup : de_unco) + offset[a];
+    const double2 gl = Gp[t], gu = Gp[dim * nE + t], gE = Gp[2 * dim * nE + t];
+    const double dl = de[p], du = de[p + 1], dj = dE[j];
+    dG[t] = make_double2(gl.x * dl + gu.x * du + gE.x * dj, gl.y * dl + gu.y * du + gE.y * dj);
 }
 
 // ------------------------------------------------------------------ block ops
@@ -913,7 +986,7 @@ struct RowOperator {
     double ms_apply = 0.0, ms_spmm = 0.0, spmm_bytes = 0.0;
     int n_apply = 0, n_apply32 = 0, n_spmm = 0;
     int r_cap = 0;
-    int c_npart_max = 1, vc_npart_max = 1;
+    int c_npart_max = 0, vc_npart_max = 0;
     bool cells = true;   // cell-block product (default) or gather over the CSC
     // col: the column operator z -> A z = C_op^T (2 P123) VC_op z (products with P123 itself,
     // through the transposed cells) instead of the row form y -> y A
@@ -976,16 +1049,24 @@ struct RowOperator {
             cv = cells_of(*csc, col);
             items.build(cv, s);
         }
+        set_blocks(rp, V_unco, V_coup, swp, chn);
+    }
+
+    // C_op / VC_op partner blocks from V and the SWP eigenvectors.  The partner lists are
+    // structural (zero blocks kept), so new V or C (another theta) only rewrite the block
+    // contents in the same buffers and captured graphs stay valid.
+    void set_blocks(const tt_run_params &rp, const double *V_unco, const double *V_coup, const tt_swp_states &swp,
+                    const tt_pw_states &chn) {
         int nu, nc;
         num_2N_states(rp, nu, nc);
         // V.C per 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array, lines 1828-1855)
         std::vector<double> VCu((size_t)nu * Np * Np, 0.0), VCc((size_t)nc * 4 * Np * Np, 0.0);
         auto mm = [](const double *A, const double *B, double *C, int n) {
             for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j) {
-                    double acc = 0.0;
-                    for (int k = 0; k < n; ++k) acc += A[(size_t)i * n + k] * B[(size_t)k * n + j];
-                    C[(size_t)i * n + j] = acc;
+                for (int k = 0; k < n; ++k) {
+                    const double a = A[(size_t)i * n + k];
+                    if (a == 0.0) continue;
+                    for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
                 }
         };
         for (int c = 0; c < nu; ++c)
@@ -997,23 +1078,35 @@ struct RowOperator {
         std::vector<double> b1, b2;
         const double *Cu = swp.C_SWP_unco, *Cc = swp.C_SWP_coup;
         if (!col) {
-            embed_2N_blocks(chn, Np, Cu, Cc, rp, false, p1, a1, b1);                 // S = C_op (X G)
-            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, true, p2, a2, b2);   // Y = VC_op^T U
+            embed_2N_blocks(chn, Np, Cu, Cc, rp, false, p1, a1, b1, true);                 // S = C_op (X G)
+            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, true, p2, a2, b2, true);   // Y = VC_op^T U
         } else {
-            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, false, p1, a1, b1);  // S = VC_op (X G)
-            embed_2N_blocks(chn, Np, Cu, Cc, rp, true, p2, a2, b2);                   // Y = C_op^T U
+            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, false, p1, a1, b1, true);  // S = VC_op (X G)
+            embed_2N_blocks(chn, Np, Cu, Cc, rp, true, p2, a2, b2, true);                   // Y = C_op^T U
         }
-        c_npart_max = vc_npart_max = 1;
+        int m1 = 1, m2 = 1;
         for (int a = 0; a < Na; ++a) {
-            c_npart_max = std::max(c_npart_max, p1[a + 1] - p1[a]);
-            vc_npart_max = std::max(vc_npart_max, p2[a + 1] - p2[a]);
+            m1 = std::max(m1, p1[a + 1] - p1[a]);
+            m2 = std::max(m2, p2[a + 1] - p2[a]);
         }
-        c_ptr.upload(p1.data(), p1.size(), s);
-        c_alpha.upload(a1.data(), std::max<size_t>(a1.size(), 1), s);
-        c_blocks.upload(b1.data(), std::max<size_t>(b1.size(), 1), s);
-        vc_ptr.upload(p2.data(), p2.size(), s);
-        vc_alpha.upload(a2.data(), std::max<size_t>(a2.size(), 1), s);
-        vc_blocks.upload(b2.data(), std::max<size_t>(b2.size(), 1), s);
+        auto up = [&](auto &buf, const auto &v) {
+            const size_t n = std::max<size_t>(v.size(), 1);
+            if (n > buf.n) ++layout_gen;
+            buf.upload(v.data(), n, s);
+        };
+        if (m1 != c_npart_max || m2 != vc_npart_max) ++layout_gen;   // a launch parameter of the graphs
+        c_npart_max = m1;
+        vc_npart_max = m2;
+        a1.push_back(0);
+        a2.push_back(0);
+        b1.push_back(0.0);
+        b2.push_back(0.0);
+        up(c_ptr, p1);
+        up(c_alpha, a1);
+        up(c_blocks, b1);
+        up(vc_ptr, p2);
+        up(vc_alpha, a2);
+        up(vc_blocks, b2);
     }
 
     // Device buffer for the resolvent of nE energies (contents written by the caller).
@@ -1584,9 +1677,11 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
     st.iterations = n_done + 1;
 }
 
-// G[j][idx] at the total energies E[0..nE) (host), written to device memory dG (nE*dim).
+// G[j][idx] at the total energies E[0..nE) (host), written to device memory dG (nE*dim);
+// with partials, the three partial-derivative arrays of k_resolvent_partials (3*nE*dim).
 static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt_swp_states &swp,
-                                const tt_pw_states &chn, const double *E, int nE, double2 *dG) {
+                                const tt_pw_states &chn, const double *E, int nE, double2 *dG,
+                                bool partials = false) {
     std::vector<int> which, offset;
     swp_edge_offsets(chn, swp, rp, which, offset);
     const int Na = chn.Nalpha, Nq = swp.Nq_WP, Np = swp.Np_WP;
@@ -1600,8 +1695,12 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
     dE.upload(E, nE, s);
     dw.upload(which.data(), Na, s);
     doff.upload(offset.data(), Na, s);
-    hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p, doff.p, qe.p,
-                       dE.p, Na, Nq, Np, nE);
+    if (partials)
+        hipLaunchKernelGGL(k_resolvent_partials, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p,
+                           doff.p, qe.p, dE.p, Na, Nq, Np, nE);
+    else
+        hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p, doff.p,
+                           qe.p, dE.p, Na, Nq, Np, nE);
     TT_HIP(hipGetLastError());
     // the staging buffers are freed on return: let the kernel finish with them first
     TT_HIP(hipStreamSynchronize(s));
@@ -1613,7 +1712,7 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
 // solution vectors.  Cached per context, keyed by the P123 object and a hash of the inputs
 // the operator is built from; only G (the energies) changes between reuses.
 struct SolvePlan {
-    uint64_t p_gen = 0, key = 0;
+    uint64_t p_gen = 0, key = 0, content = 0;
     int format = TT_P123_CELLS;
     bool col = false;   // column operator (adjoint solves)
     uint64_t last_use = 0;
@@ -1651,35 +1750,49 @@ static uint64_t hash_bytes(uint64_t h, const void *data, size_t n) {
     return h;
 }
 
-static uint64_t plan_key(const tt_run_params &rp, const double *V_unco, const double *V_coup,
-                         const tt_swp_states &swp, const tt_pw_states &chn, int format) {
-    int nu, nc;
-    num_2N_states(rp, nu, nc);
-    const size_t Np = (size_t)swp.Np_WP;
+// Structure of a plan: everything that fixes the operator's layout (flags, sizes, the
+// channel's quantum numbers).  Its contents (V and the SWP eigenvectors) are hashed apart.
+static uint64_t plan_struct_key(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
+                                int format) {
     uint64_t h = 0xcbf29ce484222325ull;
     const int flags[6] = {rp.tensor_force, rp.isospin_breaking_1S0, swp.Np_WP, swp.Nq_WP, chn.Nalpha, format};
     h = hash_bytes(h, flags, sizeof(flags));
-    h = hash_bytes(h, V_unco, sizeof(double) * nu * Np * Np);
-    if (nc) h = hash_bytes(h, V_coup, sizeof(double) * nc * 4 * Np * Np);
-    h = hash_bytes(h, swp.C_SWP_unco, sizeof(double) * nu * Np * Np);
-    if (nc) h = hash_bytes(h, swp.C_SWP_coup, sizeof(double) * nc * 4 * Np * Np);
     const int *arrs[] = {chn.L_2N, chn.S_2N, chn.J_2N, chn.T_2N, chn.L_1N, chn.two_J_1N, chn.two_J_3N,
                          chn.two_T_3N, chn.P_3N};
     for (const int *a : arrs) h = hash_bytes(h, a, sizeof(int) * chn.Nalpha);
     return h;
 }
 
-// The context's plan for these inputs: reused when the P123 object and every input hash
-// match, else built (evicting the least recently used plan beyond max_plans).
+static uint64_t plan_content_key(const tt_run_params &rp, const double *V_unco, const double *V_coup,
+                                 const tt_swp_states &swp) {
+    int nu, nc;
+    num_2N_states(rp, nu, nc);
+    const size_t Np = (size_t)swp.Np_WP;
+    uint64_t h = 0x84222325cbf29ce4ull;
+    h = hash_bytes(h, V_unco, sizeof(double) * nu * Np * Np);
+    if (nc) h = hash_bytes(h, V_coup, sizeof(double) * nc * 4 * Np * Np);
+    h = hash_bytes(h, swp.C_SWP_unco, sizeof(double) * nu * Np * Np);
+    if (nc) h = hash_bytes(h, swp.C_SWP_coup, sizeof(double) * nc * 4 * Np * Np);
+    return h;
+}
+
+// The context's plan for these inputs: found by the P123 object and the structure, its block
+// contents refreshed when V or C changed (parameter sweeps, gradients), else built (evicting
+// the least recently used plan beyond max_plans).
 static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC *P, const double *V_unco,
                            const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format,
                            bool col = false) {
     if (!ctx->plans) ctx->plans = new PlanCache;
     PlanCache &c = *ctx->plans;
-    const uint64_t key = plan_key(rp, V_unco, V_coup, swp, chn, format);
+    const uint64_t key = plan_struct_key(rp, swp, chn, format);
+    const uint64_t content = plan_content_key(rp, V_unco, V_coup, swp);
     for (auto &pl : c.plans)
         if (pl->p_gen == P->gen && pl->key == key && pl->format == format && pl->col == col) {
             pl->last_use = ++c.tick;
+            if (pl->content != content) {
+                pl->op.set_blocks(rp, V_unco, V_coup, swp, chn);
+                pl->content = content;
+            }
             return *pl;
         }
     if (c.plans.size() >= PlanCache::max_plans) {
@@ -1694,6 +1807,7 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
     pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format, col);
     pl->p_gen = P->gen;
     pl->key = key;
+    pl->content = content;
     pl->format = format;
     pl->col = col;
     pl->last_use = ++c.tick;
@@ -2147,7 +2261,7 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     DevBuf<int4> dpairs;
     dpairs.upload(pairs.data(), npairs, s);
     const int nbp = (int)std::min<int64_t>(256, nblk(dim, 256));
-    DevBuf<double2> part((size_t)npairs * nbp), tri((size_t)3 * npairs), dG((size_t)nE * dim);
+    DevBuf<double2> part((size_t)npairs * nbp), dG((size_t)nE * dim);
     std::vector<double> E(nE), fE(nE);
     const double Md = kMn + kMp + swp.E_bound;
     for (int j = 0; j < nE; ++j) {
@@ -2155,17 +2269,27 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
         // E_com = (q_l^2 + q_u^2) / (4 mu1(E_bound)) at the bin midpoint: dE/dE_bound
         fE[j] = 1.0 - os.E_com[j] * kMn / (Md * (kMn + Md));
     }
-    std::vector<double> dGh((size_t)2 * nE * dim), dE(nE);
-    std::vector<double2> trih((size_t)3 * npairs);
-    std::vector<double> dVCu(std::max<size_t>(su, 1)), dVCc(std::max<size_t>(sc, 1));
-    DevBuf<int> bp1, ba1, bp2, ba2;
-    DevBuf<double> bb1, bb2;
-    for (int i = 0; i < n_theta; ++i) {
+    DevBuf<double2> Gp((size_t)3 * nE * dim);   // dG/d(lower edge), dG/d(upper edge), dG/dE
+    resolvent_on_device(s, rp, swp, chn, E.data(), nE, Gp.p, true);
+    std::vector<int> which, offset;
+    swp_edge_offsets(chn, swp, rp, which, offset);
+    DevBuf<int> dwhich, doffset;
+    dwhich.upload(which.data(), Na, s);
+    doffset.upload(offset.data(), Na, s);
+    // host phase, one theta per task on a few threads: SWP perturbation, dVC = dV C + V dC,
+    // and the partner lists of dC_op and dVC_op
+    struct ThetaWork {
+        std::vector<int> p1, a1, p2, a2;
+        std::vector<double> b1, b2, de_u, de_c, dE;
+        int ne1 = 0, ne2 = 0;
+    };
+    std::vector<ThetaWork> tw(n_theta);
+    auto work = [&](int i) {
+        ThetaWork &t = tw[i];
         const double *dVu = Vb_unco + (size_t)i * su, *dVc = Vb_coup + (size_t)i * sc;
         SwpDerivative d;
         swp_derivative(rp, fwp, pw_full, Vu.data(), Vc.data(), swp, dVu, dVc, d);
-        std::fill(dVCu.begin(), dVCu.end(), 0.0);
-        std::fill(dVCc.begin(), dVCc.end(), 0.0);
+        std::vector<double> dVCu(std::max<size_t>(su, 1), 0.0), dVCc(std::max<size_t>(sc, 1), 0.0);
         for (int c = 0; c < nu; ++c) {
             const size_t o1 = (size_t)c * Np * Np;
             matmul_add(dVu + o1, Cu.data() + o1, dVCu.data() + o1, Np);
@@ -2176,49 +2300,92 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
             matmul_add(dVc + o1, Cc.data() + o1, dVCc.data() + o1, 2 * Np);
             matmul_add(Vc.data() + o1, d.dC_coup.data() + o1, dVCc.data() + o1, 2 * Np);
         }
-        std::vector<int> p1, a1, p2, a2;
-        std::vector<double> b1, b2;
-        embed_2N_blocks(chn, Np, d.dC_unco.data(), d.dC_coup.data(), rp, false, p1, a1, b1);
-        embed_2N_blocks(chn, Np, dVCu.data(), dVCc.data(), rp, false, p2, a2, b2);
-        int ne1 = 0, ne2 = 0;
+        embed_2N_blocks(chn, Np, d.dC_unco.data(), d.dC_coup.data(), rp, false, t.p1, t.a1, t.b1);
+        embed_2N_blocks(chn, Np, dVCu.data(), dVCc.data(), rp, false, t.p2, t.a2, t.b2);
         for (int a = 0; a < Na; ++a) {
-            ne1 = std::max(ne1, p1[a + 1] - p1[a]);
-            ne2 = std::max(ne2, p2[a + 1] - p2[a]);
+            t.ne1 = std::max(t.ne1, t.p1[a + 1] - t.p1[a]);
+            t.ne2 = std::max(t.ne2, t.p2[a + 1] - t.p2[a]);
         }
-        a1.push_back(0);
-        a2.push_back(0);
-        b1.push_back(0.0);
-        b2.push_back(0.0);
-        bp1.upload(p1.data(), p1.size(), s);
-        ba1.upload(a1.data(), a1.size(), s);
-        bb1.upload(b1.data(), b1.size(), s);
-        bp2.upload(p2.data(), p2.size(), s);
-        ba2.upload(a2.data(), a2.size(), s);
-        bb2.upload(b2.data(), b2.size(), s);
-        launch_block<double2, double2>(s, w.p, Dw.p, bp1.p, ba1.p, bb1.p, ne1, nullptr, R.gidx.p, Na, Nq, Np, r,
+        t.a1.push_back(0);
+        t.a2.push_back(0);
+        t.b1.push_back(0.0);
+        t.b2.push_back(0.0);
+        t.de_u = d.de_unco;
+        t.de_c = d.de_coup;
+        t.de_c.push_back(0.0);
+        t.dE.resize(nE);
+        for (int j = 0; j < nE; ++j) t.dE[j] = d.dE_bound * fE[j];
+    };
+    {
+        const int nthr = (int)std::max(1u, std::min<unsigned>((unsigned)n_theta, std::min(8u, std::thread::hardware_concurrency())));
+        std::vector<std::exception_ptr> errs(nthr);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthr; ++t)
+            th.emplace_back([&, t]() {
+                try {
+                    for (int i = t; i < n_theta; i += nthr) work(i);
+                } catch (...) {
+                    errs[t] = std::current_exception();
+                }
+            });
+        for (auto &x : th) x.join();
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    }
+    // device phase: buffers sized for every theta first (no reallocation behind pending work),
+    // then all thetas back to back and one read-back of the 3 * npairs terms per theta
+    size_t m_p = 1, m_a1 = 1, m_b1 = 1, m_a2 = 1, m_b2 = 1, m_u = 1, m_c = 1;
+    for (const ThetaWork &t : tw) {
+        m_p = std::max(m_p, t.p1.size());
+        m_a1 = std::max(m_a1, t.a1.size());
+        m_b1 = std::max(m_b1, t.b1.size());
+        m_a2 = std::max(m_a2, t.a2.size());
+        m_b2 = std::max(m_b2, t.b2.size());
+        m_u = std::max(m_u, t.de_u.size());
+        m_c = std::max(m_c, t.de_c.size());
+    }
+    DevBuf<int> bp1(m_p), ba1(m_a1), bp2(m_p), ba2(m_a2);
+    DevBuf<double> bb1(m_b1), bb2(m_b2), dde_u(m_u), dde_c(m_c), ddE(nE);
+    DevBuf<double2> tri_all((size_t)3 * npairs * n_theta);
+    std::vector<double2> tri_h((size_t)3 * npairs * n_theta);
+    for (int i = 0; i < n_theta; ++i) {
+        const ThetaWork &t = tw[i];
+        bp1.upload(t.p1.data(), t.p1.size(), s);
+        ba1.upload(t.a1.data(), t.a1.size(), s);
+        bb1.upload(t.b1.data(), t.b1.size(), s);
+        bp2.upload(t.p2.data(), t.p2.size(), s);
+        ba2.upload(t.a2.data(), t.a2.size(), s);
+        bb2.upload(t.b2.data(), t.b2.size(), s);
+        dde_u.upload(t.de_u.data(), std::max<size_t>(t.de_u.size(), 1), s);
+        dde_c.upload(t.de_c.data(), t.de_c.size(), s);
+        ddE.upload(t.dE.data(), nE, s);
+        launch_block<double2, double2>(s, w.p, Dw.p, bp1.p, ba1.p, bb1.p, t.ne1, nullptr, R.gidx.p, Na, Nq, Np, r,
                                        nullptr, 0, 1, nullptr, 1.0);   // dC_op w
-        launch_block<double2, double2>(s, z.p, Dz.p, bp2.p, ba2.p, bb2.p, ne2, nullptr, R.gidx.p, Na, Nq, Np, r,
+        launch_block<double2, double2>(s, z.p, Dz.p, bp2.p, ba2.p, bb2.p, t.ne2, nullptr, R.gidx.p, Na, Nq, Np, r,
                                        nullptr, 0, 1, nullptr, 1.0);   // dVC_op z
-        for (int j = 0; j < nE; ++j) dE[j] = d.dE_bound * fE[j];
-        resolvent_derivative(rp, swp, chn, d, E.data(), dE.data(), nE, dGh.data());
-        dG.upload(reinterpret_cast<const double2 *>(dGh.data()), (size_t)nE * dim, s);
+        hipLaunchKernelGGL(k_resolvent_change, dim3(nblk((int64_t)nE * dim, 256)), dim3(256), 0, s, dG.p, Gp.p,
+                           dde_u.p, dde_c.p, dwhich.p, doffset.p, ddE.p, Na, Nq, Np, nE);
+        double2 *tri_i = tri_all.p + (size_t)3 * npairs * i;
         const dim3 pg((unsigned)nbp, (unsigned)npairs);
         hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Dw.p, 1, bz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
                            part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i);
         hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Uw.p, 1, Dz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
                            part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p + npairs);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i + npairs);
         hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, rowp.x.p, 0, colp.x.p, 0, dG.p, dpairs.p, Na, Nq * Np,
                            r, part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri.p + 2 * npairs);
-        TT_HIP(hipMemcpyAsync(trih.data(), tri.p, sizeof(double2) * 3 * npairs, hipMemcpyDeviceToHost, s));
-        TT_HIP(hipStreamSynchronize(s));
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i + 2 * npairs);
+    }
+    TT_HIP(hipMemcpyAsync(tri_h.data(), tri_all.p, sizeof(double2) * tri_h.size(), hipMemcpyDeviceToHost, s));
+    TT_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n_theta; ++i) {
+        const double2 *tr = tri_h.data() + (size_t)3 * npairs * i;
         for (int q = 0; q < npairs; ++q) {
             const int dr = pairs[q].x / nE, dc = pairs[q].y / nE, j = pairs[q].z;
             const size_t e = (size_t)i * n + ((size_t)dr * nd + dc) * nE + j;
-            dU[2 * e] = trih[q].x + trih[npairs + q].x + trih[2 * npairs + q].x;
-            dU[2 * e + 1] = trih[q].y + trih[npairs + q].y + trih[2 * npairs + q].y;
+            dU[2 * e] = tr[q].x + tr[npairs + q].x + tr[2 * npairs + q].x;
+            dU[2 * e + 1] = tr[q].y + tr[npairs + q].y + tr[2 * npairs + q].y;
         }
     }
     Cl.collect();

@@ -66,7 +66,7 @@ std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_un
                              const double *M_coup, const tt_run_params &rp);
 void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, const double *M_coup,
                      const tt_run_params &rp, bool transpose, std::vector<int> &ptr, std::vector<int> &alpha,
-                     std::vector<double> &blocks);
+                     std::vector<double> &blocks, bool keep_zero = false);
 
 // Per-alpha offset of the SWP energy edges used by the resolvent (CPP/make_resolvent.cpp:118-156):
 // returns for each alpha a pointer offset into either e_SWP_unco (flag 0) or e_SWP_coup (flag 1).
@@ -149,10 +149,6 @@ struct SwpDerivative {
 void swp_derivative(const tt_run_params &rp, const tt_fwp_states &fwp, const tt_pw_states &pw_full,
                     const double *V_unco, const double *V_coup, const tt_swp_states &swp, const double *dV_unco,
                     const double *dV_coup, SwpDerivative &d);
-// dG[j][idx] (interleaved complex, nE*dim) of the channel resolvent for edge changes d.de_*,
-// deuteron-energy change d.dE_bound and energy changes dE[j] at energies E[j].
-void resolvent_derivative(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
-                          const SwpDerivative &d, const double *E, const double *dE, int nE, double *dG);
 
 // Solve plans cached per context (operator set-up, Krylov workspaces, captured graphs).
 struct PlanCache;
