@@ -143,6 +143,42 @@ def measured_traffic(kernel):
     return None
 
 
+N40_WORKLOAD = ("same synthetic NN at Np=Nq=40: channels J^Pi=1/2+ (N_alpha 27, dim 43200) and 7/2+ (N_alpha 63, "
+                "dim 100800), T_lab = 10 MeV (2 on-shell bins)")
+
+
+def precision_n40(ctx, reps=3):
+    """Mixed precision at Np=Nq=40: channel-solves/s (mixed and FP64, best of reps, G on the
+    device) and the relative error of every on-shell U element of the mixed solve against the
+    converged FP64 solve, per channel and energy (target <= 1e-8)."""
+    N = 40
+    rp = tt.run_params(two_J_3N_max=7, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N))
+    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
+    out = []
+    for c in range(prob.n_channels):
+        ch = prob.channel(c, [10.0])
+        if not ((ch.two_J == 1 and ch.parity == 1) or (ch.two_J == 7 and ch.parity == 1)):
+            continue
+        P = ch.build_p123(ctx)
+        res = {"channel": "%d/2%s" % (ch.two_J, "+" if ch.parity > 0 else "-"), "dim": ch.dim,
+               "p123_build_ms": round(P.stats.ms_total, 1)}
+        U = {}
+        for method, name in ((tt.SOLVE_MIXED, "mixed"), (tt.SOLVE_GMRES, "fp64")):
+            ch.solve(ctx, P, None, method=method, device_resolvent=True)
+            best = None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                U[name], st = ch.solve(ctx, P, None, method=method, device_resolvent=True)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            res[name] = round(1.0 / best, 2)
+        err = np.abs(U["mixed"] - U["fp64"]) / np.abs(U["fp64"])
+        res["mixed_rel_err_per_energy"] = [float(err[..., j].max()) for j in range(ch.nE)]
+        out.append(res)
+        del P
+    return {"workload": N40_WORKLOAD, "unit": "channel-solves/s", "channels": out}
+
+
 def product_roofline(stats, kernel):
     """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
     algorithmic bytes per launch (every block value once, the S panel read and the U panel
@@ -203,6 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the all-channel batched line")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient line")
+    ap.add_argument("--no-n40", action="store_true", help="skip the Np=Nq=40 precision line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
@@ -245,6 +282,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(ctx)
+    if not args.no_n40:
+        out["precision_n40"] = precision_n40(ctx)
     if not args.no_grad:
         out["gradient"] = gradient_ratio(ctx)
         out["grad_over_fwd"] = out["gradient"]["grad_over_fwd"]
