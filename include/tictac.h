@@ -245,6 +245,11 @@ typedef struct {
     int n_spmm;
     double ms_spmm;
     double spmm_bytes;
+    /* the FP32 launches among them (mixed precision); in graph replays only the first
+     * Arnoldi step of each cycle is timed */
+    int n_spmm_fp32;
+    double ms_spmm_fp32;
+    double spmm_bytes_fp32;
 } tt_solve_stats;
 
 /* Replaces solve_faddeev_equations (CPP/solve_faddeev.h:25-38).

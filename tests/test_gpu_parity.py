@@ -240,7 +240,9 @@ def test_solves_c2_against_oracle_gmres(ctx, c2):
 def test_graph_replays_match_eager_launches(ctx, c2):
     """Each Arnoldi step is captured once as a HIP graph and replayed by later solves of the
     channel (the context keeps the channel's plan).  Every replay, with eager work and
-    synchronisation in between, gives the result of plain stream launches."""
+    synchronisation in between, gives the result of plain stream launches to the solve
+    tolerance (the Gram-Schmidt sums add block partials by device atomics, whose order, and
+    so the last bits, vary from run to run; the iteration counts are equal)."""
     ch = c2.ch
     for method in (tt.SOLVE_GMRES, tt.SOLVE_MIXED):
         U_eager, st_e = ch.solve(ctx, c2.P, c2.G, method=method, graphs=False)
@@ -248,7 +250,7 @@ def test_graph_replays_match_eager_launches(ctx, c2):
             G = ch.resolvent(ctx)   # eager work between the replays
             U_g, st_g = ch.solve(ctx, c2.P, G, method=method, graphs=True)
             assert st_g.iterations == st_e.iterations
-            assert np.abs(U_g - U_eager).max() <= 1e-14 * np.abs(U_eager).max(), (method, rep)
+            assert np.abs(U_g - U_eager).max() <= 1e-12 * np.abs(U_eager).max(), (method, rep)
             assert st_g.n_spmm <= st_g.n_apply
 
 

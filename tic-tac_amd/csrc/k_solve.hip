@@ -311,8 +311,10 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
                                                const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
                                                int ne_max, const double2 *__restrict__ Zsub, int in_cell,
                                                int out_cell, const int *__restrict__ ket_item_ptr,
-                                               double in_scale) {
+                                               double in_scale, double2 *__restrict__ zero_buf, int zero_n) {
     extern __shared__ double smem[];
+    if (zero_buf && blockIdx.x == 0 && blockIdx.y == 0)   // a buffer a later kernel accumulates into
+        for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_buf[i] = make_double2(0.0, 0.0);
     const int64_t dim = (int64_t)Na * Nq * Np;
     const int a = blockIdx.x / Nq, q = blockIdx.x % Nq;
     const int k0 = blockIdx.y * kc, nk = min(kc, r - k0);
@@ -386,13 +388,13 @@ template <typename TI, typename TO>
 static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
                          const double *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
                          int r, const double2 *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
-                         double in_scale) {
+                         double in_scale, double2 *zero_buf = nullptr, int zero_n = 0) {
     const int kc = std::min(r, 16);
     const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(double) + (size_t)ne_max * Np * kc * sizeof(double2);
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
     hipLaunchKernelGGL((k_block<TI, TO>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na, Nq,
-                       Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale);
+                       Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n);
 }
 
 // ------------------------------------------------------------------ gather SpMM
@@ -746,14 +748,16 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, in
 }
 
 // ------------------------------------------------------------------ Krylov vector ops
-// partial[i][b][k] = sum over block b's slice of conj(V_i[e]) W[e] in column k.  The sum over
-// the blocks is a separate launch (k_dots_reduce): a device-scope fence per block, which a
-// single-launch "last block finishes" scheme needs, writes back the whole L2 of the block's
-// XCD and costs several times the dot products themselves.
+// partial[i][b][k] = sum over block b's slice of conj(V_i[e]) W[e] in column k, summed over the
+// blocks either by a separate launch (k_dots_reduce, fixed order) or, with accum, by device
+// atomics into accum[i][k] (zeroed beforehand; the order of the additions then varies from run
+// to run, at the level of rounding).  A single-launch "last block finishes" scheme would need a
+// device-scope fence per block, which writes back the whole L2 of the block's XCD and costs
+// several times the dot products themselves; atomics need none.
 constexpr int DOT_UN = 8;
 
 __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const double2 *__restrict__ W,
-                       double2 *__restrict__ partial, int64_t n_elem, int r) {
+                       double2 *__restrict__ partial, int64_t n_elem, int r, double2 *__restrict__ accum) {
     extern __shared__ double2 sh[];
     const int i = blockIdx.y;
     const int b = blockIdx.x, nb = gridDim.x;
@@ -791,7 +795,12 @@ __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const dou
             ar += sh[s2 * r + threadIdx.x].x;
             ai += sh[s2 * r + threadIdx.x].y;
         }
-        partial[((int64_t)i * nb + b) * r + threadIdx.x] = make_double2(ar, ai);
+        if (accum) {   // straight into the (zeroed) result: one launch, no reduction pass
+            atomicAdd(&accum[(int64_t)i * r + threadIdx.x].x, ar);
+            atomicAdd(&accum[(int64_t)i * r + threadIdx.x].y, ai);
+        } else {
+            partial[((int64_t)i * nb + b) * r + threadIdx.x] = make_double2(ar, ai);
+        }
     }
 }
 
@@ -838,8 +847,8 @@ __global__ __launch_bounds__(256) void k_dots_reduce(const double2 *__restrict__
 // W[e] -= sum_i h[i][k] V_i[e]   (sign = -1) or X[e] += sum_i z[i][k] V_i[e]   (sign = +1).
 // The vectors are taken in batches of COMBO_UN with all loads issued before use (clamped and
 // masked past nvec), so each thread keeps COMBO_UN loads in flight.  With norm_partial the
-// block also writes the squared norms of its slice of the result per column k (blockDim a
-// multiple of r), summed over the blocks by k_dots_reduce.
+// block also adds the squared norms of its slice of the result per column k (blockDim a
+// multiple of r) into norm_partial[k] (zeroed beforehand) by device atomics.
 constexpr int COMBO_UN = 8;
 __global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
                         const double2 *__restrict__ h, int64_t n_elem, int r, double sign,
@@ -875,7 +884,7 @@ __global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, 
         if ((int)threadIdx.x < r) {
             double acc = 0.0;
             for (int q = (int)threadIdx.x; q < (int)blockDim.x; q += r) acc += nsh[q];
-            norm_partial[(int64_t)blockIdx.x * r + threadIdx.x] = make_double2(acc, 0.0);
+            atomicAdd(&norm_partial[threadIdx.x].x, acc);
         }
     }
 }
@@ -983,8 +992,10 @@ struct RowOperator {
     std::vector<hipEvent_t> evs;
     size_t ev_used = 0;
     std::vector<double> spmm_bytes_of;   // algorithmic bytes of each timed SpMM launch
+    std::vector<char> spmm_fp32_of;       // and whether it was an FP32 launch
     double ms_apply = 0.0, ms_spmm = 0.0, spmm_bytes = 0.0;
-    int n_apply = 0, n_apply32 = 0, n_spmm = 0;
+    double ms_spmm32 = 0.0, spmm_bytes32 = 0.0;
+    int n_apply = 0, n_apply32 = 0, n_spmm = 0, n_spmm32 = 0;
     int r_cap = 0;
     int c_npart_max = 0, vc_npart_max = 0;
     bool cells = true;   // cell-block product (default) or gather over the CSC
@@ -997,6 +1008,9 @@ struct RowOperator {
     // bumped whenever a buffer the kernels address is (re)allocated: captured graphs hold
     // these addresses and are rebuilt when it changes
     uint64_t layout_gen = 0;
+    // a buffer the first block op of the next application zeroes (for accumulations that follow)
+    double2 *zero_next = nullptr;
+    int zero_next_n = 0;
 
     ~RowOperator() {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
@@ -1020,12 +1034,11 @@ struct RowOperator {
             TT_HIP(hipEventElapsedTime(&a, evs[i], evs[i + 3]));
             TT_HIP(hipEventElapsedTime(&b, evs[i + 1], evs[i + 2]));
             ms_apply += a;
-            ms_spmm += b;
-            spmm_bytes += spmm_bytes_of[i / 4];
-            ++n_spmm;
+            add_product_time(b, spmm_bytes_of[i / 4], spmm_fp32_of[i / 4] != 0);
         }
         ev_used = 0;
         spmm_bytes_of.clear();
+        spmm_fp32_of.clear();
     }
 
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
@@ -1224,9 +1237,13 @@ struct RowOperator {
         // into the input staging of the V.C block op
         const int *kip = cells ? items.ket_item_ptr.p : nullptr;
         const double in_scale = cells ? 2.0 : 1.0;
+        double2 *zb = zero_next;
+        const int zn = zero_next_n;
+        zero_next = nullptr;
+        zero_next_n = 0;
         if (!fp32) {
             launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, use_G ? G.p : nullptr,
-                                           gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
+                                           gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0, zb, zn);
             if (timed) record(e1);
             if (cells)
                 launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
@@ -1242,7 +1259,8 @@ struct RowOperator {
         } else {
             ensure_f32();
             launch_block<double2, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
-                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0);
+                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
+                                          zb, zn);
             if (timed) record(e1);
             if (cells)
                 launch_cellmm<float>(s, &cv, items, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
@@ -1259,7 +1277,10 @@ struct RowOperator {
         }
         if (timed) {
             record(e3);
-            if (!ext) spmm_bytes_of.push_back(product_bytes(r, fp32));
+            if (!ext) {
+                spmm_bytes_of.push_back(product_bytes(r, fp32));
+                spmm_fp32_of.push_back(fp32 ? 1 : 0);
+            }
         }
     }
 
@@ -1276,17 +1297,26 @@ struct RowOperator {
 
     // Adds the timing of one application recorded into caller events (after they completed).
     // Returns false, adding nothing, where the runtime keeps no timestamps for them.
-    bool add_timing(const hipEvent_t *ev, double bytes) {
+    bool add_timing(const hipEvent_t *ev, double bytes, bool fp32) {
         float a = 0.f, b = 0.f;
         if (hipEventElapsedTime(&a, ev[0], ev[3]) != hipSuccess || hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess) {
             (void)hipGetLastError();
             return false;
         }
         ms_apply += a;
-        ms_spmm += b;
+        add_product_time(b, bytes, fp32);
+        return true;
+    }
+
+    void add_product_time(double ms, double bytes, bool fp32) {
+        ms_spmm += ms;
         spmm_bytes += bytes;
         ++n_spmm;
-        return true;
+        if (fp32) {
+            ms_spmm32 += ms;
+            spmm_bytes32 += bytes;
+            ++n_spmm32;
+        }
     }
 };
 
@@ -1387,14 +1417,15 @@ struct Gmres {
     }
 
     // out[i][k] = <Vb_i, Wv>_k for i < nvec, on the device
+    // out (zeroed beforehand) += <Vb_i, Wv>_k for i < nvec, on the device
     void dots_dev(const double2 *Vb, int nvec, const double2 *Wv, double2 *out) {
         int bs = 256 - 256 % r;
         hipLaunchKernelGGL(k_dots, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, partial.p, ne,
-                           r);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(nvec), dim3(256), 0, op.s, partial.p, nb_dots, r, out);
+                           r, out);
     }
 
     std::vector<double> norms(const double2 *Wv) {
+        TT_HIP(hipMemsetAsync(n2d.p, 0, sizeof(double2) * r, op.s));   // eager, never captured
         dots_dev(Wv, 1, Wv, n2d.p);
         std::vector<double2> h(r);
         TT_HIP(hipMemcpyAsync(h.data(), n2d.p, sizeof(double2) * r, hipMemcpyDeviceToHost, op.s));
@@ -1405,15 +1436,19 @@ struct Gmres {
     }
 
     // out = M(y) = y - (y .* G) A
-    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr) {
-        op.apply(y, out, r, true, fp32, true, y, ev);
+    // Timed through ev when given; untimed inside a step graph without ev; through the event
+    // pool otherwise (eager launches).
+    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr, bool capturing = false) {
+        op.apply(y, out, r, true, fp32, ev != nullptr || !capturing, y, ev);
     }
 
     // The work of Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
-    void enqueue_step(int j, bool fp32, const hipEvent_t *ev) {
+    void enqueue_step(int j, bool fp32, const hipEvent_t *ev, bool capturing = false) {
         double2 *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
-        apply_M(vj, W.p, fp32, ev);
+        op.zero_next = h1;   // h1, h2 and n2 accumulate by atomics: the step's first kernel zeroes them
+        op.zero_next_n = slot_len;
+        apply_M(vj, W.p, fp32, ev, capturing);
         dots_dev(V.p, j + 1, W.p, h1);
         hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r, -1.0,
                            nullptr);
@@ -1421,8 +1456,7 @@ struct Gmres {
         const int bs = 256 - 256 % r;  // the norm partials need whole columns per block
         const unsigned nbc = nblk(ne, bs);
         hipLaunchKernelGGL(k_combo, dim3(nbc), dim3(bs), bs * sizeof(double), op.s, W.p, V.p, ne, j + 1, h2, ne, r,
-                           -1.0, partial.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(256), 0, op.s, partial.p, (int)nbc, r, n2);
+                           -1.0, n2);
         hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, n2, ne, r);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
     }
@@ -1445,7 +1479,9 @@ struct Gmres {
             TT_HIP(hipStreamBeginCapture(op.s, hipStreamCaptureModeThreadLocal));
             hipGraph_t graph = nullptr;
             try {
-                enqueue_step(j, fp32, g.ev);
+                // timing events (external event nodes, a few microseconds of gap each) only in
+                // the first step of a cycle: the product's launch time is sampled there
+                enqueue_step(j, fp32, j == 0 ? g.ev : nullptr, true);
             } catch (...) {
                 (void)hipStreamEndCapture(op.s, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1468,12 +1504,13 @@ struct Gmres {
         if (fp32) ++op.n_apply32;
     }
 
-    // Timing of a completed step (graph mode records the application into the graph's events
-    // as external event nodes); stops asking once the runtime reports no timestamps for them.
+    // Timing of a completed step (graph mode records the application of step 0 into the
+    // graph's events as external event nodes); stops asking once the runtime reports no
+    // timestamps for them.
     bool graph_timing = true;
     void read_step_timing(int j, bool fp32) {
-        if (use_graphs && graph_timing)
-            graph_timing = op.add_timing(graphs[fp32 ? 1 : 0][j].ev, op.product_bytes(r, fp32));
+        if (use_graphs && graph_timing && j == 0)
+            graph_timing = op.add_timing(graphs[fp32 ? 1 : 0][j].ev, op.product_bytes(r, fp32), fp32);
     }
 
     // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).
@@ -2097,8 +2134,8 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
     SolvePlan &pl = get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format);
     RowOperator &op = pl.op;
     op.collect();
-    op.ms_apply = op.ms_spmm = op.spmm_bytes = 0.0;
-    op.n_apply = op.n_apply32 = op.n_spmm = 0;
+    op.ms_apply = op.ms_spmm = op.spmm_bytes = op.ms_spmm32 = op.spmm_bytes32 = 0.0;
+    op.n_apply = op.n_apply32 = op.n_spmm = op.n_spmm32 = 0;
     if (o.resolvent_on_device) {
         // E_com + E_bound of the on-shell bins (CPP/main.cpp:391), straight into the plan's G
         TT_REQUIRE(os->E_com, "resolvent_on_device needs the on-shell energies (os->E_com)");
@@ -2134,6 +2171,9 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
     st.n_spmm = op.n_spmm;
     st.ms_spmm = op.ms_spmm;
     st.spmm_bytes = op.spmm_bytes;
+    st.n_spmm_fp32 = op.n_spmm32;
+    st.ms_spmm_fp32 = op.ms_spmm32;
+    st.spmm_bytes_fp32 = op.spmm_bytes32;
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
     if ((o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED) && !(st.residual <= o.tol))

@@ -68,7 +68,8 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("n_apply", C.c_int), ("n_apply_fp32", C.c_int), ("iterations", C.c_int),
                 ("residual", C.c_double), ("ms_total", C.c_double), ("ms_apply", C.c_double),
-                ("n_spmm", C.c_int), ("ms_spmm", C.c_double), ("spmm_bytes", C.c_double)]
+                ("n_spmm", C.c_int), ("ms_spmm", C.c_double), ("spmm_bytes", C.c_double),
+                ("n_spmm_fp32", C.c_int), ("ms_spmm_fp32", C.c_double), ("spmm_bytes_fp32", C.c_double)]
 
 
 class GradStats(C.Structure):
