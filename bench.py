@@ -77,7 +77,7 @@ def batched(ctx, reps=2):
     res = {}
     U = {}
     for method, name in ((tt.SOLVE_MIXED, "mixed"), (tt.SOLVE_GMRES, "fp64")):
-        prob.solve_batch(ctx, chans, Ps, method=method)   # warm-up: plans and step graphs
+        prob.solve_batch(ctx, chans, Ps, method=method)   # warm-up: plans and Krylov workspaces
         best = None
         for _ in range(reps):
             ctx.synchronize()
@@ -110,7 +110,7 @@ def gradient_ratio(ctx, reps=3):
     t_f = t_g = 0.0
     for c in range(prob.n_channels):
         P = prob.channel(c, T).build_p123(ctx)
-        prob.gradient(ctx, c, P, th, T, basis=basis, grad=True)   # warm-up: plans and step graphs
+        prob.gradient(ctx, c, P, th, T, basis=basis, grad=True)   # warm-up: plans and Krylov workspaces
         bf = bg = None
         for rep in range(reps):
             theta = th * (1.0 + 1e-3 * (rep + 1))                  # a new theta each time

@@ -219,7 +219,9 @@ typedef struct {
     double *neumann_terms;
     int p123_format;            /* tt_p123_format                           */
     int use_graphs;             /* 1: each Arnoldi step is captured once as a HIP graph and
-                                 * replayed (default); 0: plain stream launches.  The context
+                                 * replayed; 0: plain stream launches (default: on MI355X a
+                                 * replay of the step's half-dozen kernels costs more device
+                                 * time than launching them).  The context
                                  * keeps the operator and Krylov workspace of recent channels
                                  * (keyed by the P123 object and a hash of V, C and the channel),
                                  * so repeated solves of a channel only upload G. */

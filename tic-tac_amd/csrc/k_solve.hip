@@ -846,47 +846,58 @@ __global__ __launch_bounds__(256) void k_dots_reduce(const double2 *__restrict__
 
 // W[e] -= sum_i h[i][k] V_i[e]   (sign = -1) or X[e] += sum_i z[i][k] V_i[e]   (sign = +1).
 // The vectors are taken in batches of COMBO_UN with all loads issued before use (clamped and
-// masked past nvec), so each thread keeps COMBO_UN loads in flight.  With norm_partial the
-// block also adds the squared norms of its slice of the result per column k (blockDim a
-// multiple of r) into norm_partial[k] (zeroed beforehand) by device atomics.
+// masked past nvec), so each thread keeps COMBO_UN loads in flight.
 constexpr int COMBO_UN = 8;
+__device__ __forceinline__ double2 combo_element(double2 w, const double2 *__restrict__ V, int64_t vstride, int nvec,
+                                                 const double2 *__restrict__ h, int64_t t, int k, int r,
+                                                 double sign, double &hh) {
+    for (int i0 = 0; i0 < nvec; i0 += COMBO_UN) {
+        double2 c[COMBO_UN], v[COMBO_UN];
+#pragma unroll
+        for (int u = 0; u < COMBO_UN; ++u) {
+            const int i = min(i0 + u, nvec - 1);
+            c[u] = h[i * r + k];
+            v[u] = V[(int64_t)i * vstride + t];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their use
+#pragma unroll
+        for (int u = 0; u < COMBO_UN; ++u) {
+            const double f = i0 + u < nvec ? sign : 0.0;
+            w.x += f * (c[u].x * v[u].x - c[u].y * v[u].y);
+            w.y += f * (c[u].x * v[u].y + c[u].y * v[u].x);
+            hh += f * f * (c[u].x * c[u].x + c[u].y * c[u].y);
+        }
+    }
+    return w;
+}
+
 __global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
-                        const double2 *__restrict__ h, int64_t n_elem, int r, double sign,
-                        double2 *__restrict__ norm_partial) {
-    extern __shared__ double nsh[];
+                        const double2 *__restrict__ h, int64_t n_elem, int r, double sign) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = t < n_elem;
+    if (t >= n_elem) return;
+    double hh = 0.0;
+    W[t] = combo_element(W[t], V, vstride, nvec, h, t, (int)(t % r), r, sign, hh);
+}
+
+// The last pass of a reorthogonalised Gram-Schmidt step, with the normalisation folded in:
+// W[e] <- (W[e] - sum_{i<nvec} h[i][k] V_i[e]) / nu_k.  h = V^H W from the pass before, whose
+// entry h[nvec][k] is <W, W>_k (W was its last vector); with the columns of V orthonormal,
+// nu_k^2 = <W, W>_k - sum_i |h[i][k]|^2 (Pythagoras), so no pass over the result is needed.
+// nu_k^2 goes to n2[k] (written by the first r threads) for the Hessenberg matrix.  After the
+// first projection h is of the order of the rounding of that projection, so the difference
+// cancels only when W itself has collapsed into span(V), where GMRES has converged.
+__global__ void k_combo_normalize(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride,
+                                  int nvec, const double2 *__restrict__ h, int64_t n_elem, int r,
+                                  double2 *__restrict__ n2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem) return;
     const int k = (int)(t % r);
-    double2 w = make_double2(0.0, 0.0);
-    if (act) {
-        w = W[t];
-        for (int i0 = 0; i0 < nvec; i0 += COMBO_UN) {
-            double2 c[COMBO_UN], v[COMBO_UN];
-#pragma unroll
-            for (int u = 0; u < COMBO_UN; ++u) {
-                const int i = min(i0 + u, nvec - 1);
-                c[u] = h[i * r + k];
-                v[u] = V[(int64_t)i * vstride + t];
-            }
-            __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their use
-#pragma unroll
-            for (int u = 0; u < COMBO_UN; ++u) {
-                const double f = i0 + u < nvec ? sign : 0.0;
-                w.x += f * (c[u].x * v[u].x - c[u].y * v[u].y);
-                w.y += f * (c[u].x * v[u].y + c[u].y * v[u].x);
-            }
-        }
-        W[t] = w;
-    }
-    if (norm_partial) {
-        nsh[threadIdx.x] = w.x * w.x + w.y * w.y;
-        __syncthreads();
-        if ((int)threadIdx.x < r) {
-            double acc = 0.0;
-            for (int q = (int)threadIdx.x; q < (int)blockDim.x; q += r) acc += nsh[q];
-            atomicAdd(&norm_partial[threadIdx.x].x, acc);
-        }
-    }
+    double hh = 0.0;
+    const double2 w = combo_element(W[t], V, vstride, nvec, h, t, k, r, -1.0, hh);
+    const double nu2 = fmax(h[nvec * r + k].x - hh, 0.0);
+    const double f = nu2 > 0.0 ? 1.0 / sqrt(nu2) : 0.0;
+    W[t] = make_double2(f * w.x, f * w.y);
+    if (t < r) n2[t] = make_double2(nu2, 0.0);
 }
 
 __global__ void k_axpby(double2 *__restrict__ Y, const double2 *__restrict__ A, const double2 *__restrict__ B,
@@ -1376,7 +1387,7 @@ struct Gmres {
     // blocks per vector of k_dots: each thread takes about one batch of DOT_UN rows
     int nb_dots = 1;
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
-    DevBuf<double2> V, W, R, zdev, partial, Hd, n2d;
+    DevBuf<double2> V, R, zdev, partial, Hd, n2d;
     double2 *Hh = nullptr;              // pinned mirror of Hd
     std::vector<hipEvent_t> step_ev;
     struct StepGraph {
@@ -1391,7 +1402,6 @@ struct Gmres {
         ne = o.dim * r;
         slot_len = (2 * (m + 1) + 1) * r;
         V.alloc((size_t)(m + 1) * ne);
-        W.alloc(ne);
         R.alloc(ne);
         zdev.alloc((size_t)(m + 1) * r);
         const int tpr = (256 - 256 % r) / r;
@@ -1436,28 +1446,26 @@ struct Gmres {
     }
 
     // out = M(y) = y - (y .* G) A
-    // Timed through ev when given; untimed inside a step graph without ev; through the event
-    // pool otherwise (eager launches).
-    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr, bool capturing = false) {
-        op.apply(y, out, r, true, fp32, ev != nullptr || !capturing, y, ev);
+    // Timed through ev when given, else through the event pool when timed.
+    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr, bool timed = true) {
+        op.apply(y, out, r, true, fp32, ev != nullptr || timed, y, ev);
     }
 
     // The work of Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
+    // The product's launch time is sampled in the first step of a cycle only: timing events
+    // are not free (a few microseconds of gap each).
     void enqueue_step(int j, bool fp32, const hipEvent_t *ev, bool capturing = false) {
         double2 *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
         op.zero_next = h1;   // h1, h2 and n2 accumulate by atomics: the step's first kernel zeroes them
         op.zero_next_n = slot_len;
-        apply_M(vj, W.p, fp32, ev, capturing);
-        dots_dev(V.p, j + 1, W.p, h1);
-        hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, W.p, V.p, ne, j + 1, h1, ne, r, -1.0,
-                           nullptr);
-        dots_dev(V.p, j + 1, W.p, h2);
-        const int bs = 256 - 256 % r;  // the norm partials need whole columns per block
-        const unsigned nbc = nblk(ne, bs);
-        hipLaunchKernelGGL(k_combo, dim3(nbc), dim3(bs), bs * sizeof(double), op.s, W.p, V.p, ne, j + 1, h2, ne, r,
-                           -1.0, n2);
-        hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, W.p, n2, ne, r);
+        // the new vector is formed in place, in column j + 1 of the basis
+        apply_M(vj, vj1, fp32, ev, !capturing && j == 0);
+        dots_dev(V.p, j + 1, vj1, h1);
+        hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h1, ne, r, -1.0);
+        dots_dev(V.p, j + 2, vj1, h2);   // h2[j + 1] = <w, w>
+        hipLaunchKernelGGL(k_combo_normalize, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h2, ne, r,
+                           n2);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
     }
 
@@ -1513,16 +1521,27 @@ struct Gmres {
             graph_timing = op.add_timing(graphs[fp32 ? 1 : 0][j].ev, op.product_bytes(r, fp32), fp32);
     }
 
-    // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).
-    // Returns iterations; res_out = final true relative residual (max over systems).
+    // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).  x_zero: the
+    // initial guess is zero (the first residual is b, no product).  verify: a cycle that
+    // converged by the GMRES residual estimate is confirmed by the true residual b - M x (one
+    // more product); without it the solve ends on the estimate (the inner solves of iterative
+    // refinement, whose caller forms the true residual in higher precision anyway).
+    // Returns iterations; res_out = final relative residual, max over systems (true residual
+    // with verify, else the estimate).
     int solve(const double2 *b, double2 *x, double tol, int max_iter, bool fp32, double &res_out,
-              const std::vector<double> &bnorm) {
+              const std::vector<double> &bnorm, bool x_zero = false, bool verify = true) {
         int it_total = 0;
         std::vector<cdouble> H((size_t)(m + 1) * m * r), cs((size_t)m * r), sn((size_t)m * r), g((size_t)(m + 1) * r);
         double res_max = 0.0;
+        bool first = true;
         while (true) {
-            apply_M(x, R.p, fp32);  // R = M x, then R = b - M x
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, R.p, b, R.p, 1.0, -1.0, ne);
+            if (first && x_zero) {
+                TT_HIP(hipMemcpyAsync(R.p, b, sizeof(double2) * ne, hipMemcpyDeviceToDevice, op.s));
+            } else {
+                apply_M(x, R.p, fp32);  // R = M x, then R = b - M x
+                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, R.p, b, R.p, 1.0, -1.0, ne);
+            }
+            first = false;
             std::vector<double> beta = norms(R.p);
             res_max = 0.0;
             bool all = true;
@@ -1536,18 +1555,22 @@ struct Gmres {
             std::fill(g.begin(), g.end(), 0.0);
             for (int k = 0; k < r; ++k) g[k] = beta[k];
             int used = 0;
+            bool converged = false;
             const int jmax = std::min(m, max_iter - it_total);
-            step(0, fp32);
-            int launched = 1;
+            // The device runs one step ahead of the host's Givens rotations, except where the
+            // residual history predicts that the step being waited for converges: that
+            // speculative step would be wasted, a wrong guess costs one host round trip.
+            int launched = 0;
+            bool hold = false;
+            std::vector<double> prev_rel(r);
+            for (int k = 0; k < r; ++k) prev_rel[k] = beta[k] / std::max(bnorm[k], 1e-300);
             for (int j = 0; j < jmax; ++j) {
-                if (j + 1 < jmax) {
-                    step(j + 1, fp32);   // keep the device busy while the host rotates column j
-                    launched = j + 2;
-                }
+                if (launched <= j) step(launched++, fp32);
+                if (!hold && launched == j + 1 && launched < jmax) step(launched++, fp32);
                 TT_HIP(hipEventSynchronize(step_ev[j]));
                 read_step_timing(j, fp32);
                 const double2 *hj = Hh + (size_t)j * slot_len;
-                bool conv_all = true;
+                bool conv_all = true, hold_next = true;
                 for (int k = 0; k < r; ++k) {
                     auto Hk = [&](int i, int jj) -> cdouble & { return H[((size_t)i * m + jj) * r + k]; };
                     for (int i = 0; i <= j; ++i) {
@@ -1574,10 +1597,18 @@ struct Gmres {
                     g[(size_t)(j + 1) * r + k] = -sv * gj;
                     double rel = std::abs(g[(size_t)(j + 1) * r + k]) / std::max(bnorm[k], 1e-300);
                     if (rel > 0.5 * tol) conv_all = false;
+                    // geometric extrapolation of this system's residual to the next step
+                    const double pred = prev_rel[k] > 0.0 ? rel * std::min(1.0, rel / prev_rel[k]) : 0.0;
+                    if (pred > 0.5 * tol) hold_next = false;
+                    prev_rel[k] = rel;
                 }
                 used = j + 1;
                 ++it_total;
-                if (conv_all) break;
+                if (conv_all) {
+                    converged = true;
+                    break;
+                }
+                hold = hold_next;
             }
             // x += V z from the triangular least-squares solution
             std::vector<cdouble> z((size_t)(m + 1) * r, 0.0);
@@ -1589,11 +1620,16 @@ struct Gmres {
                     z[(size_t)i * r + k] = Hk(i, i) != cdouble(0.0) ? sacc / Hk(i, i) : 0.0;
                 }
             }
-            TT_HIP(hipStreamSynchronize(op.s));  // a speculative step may still use zdev's neighbours
-            if (launched > used) read_step_timing(used, fp32);
+            TT_HIP(hipStreamSynchronize(op.s));  // speculative steps may still use zdev's neighbours
+            for (int j = used; j < launched; ++j) read_step_timing(j, fp32);
             zdev.upload(reinterpret_cast<const double2 *>(z.data()), z.size(), op.s);
-            hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r, 1.0,
-                               nullptr);
+            hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r, 1.0);
+            if (!verify && (converged || it_total >= max_iter)) {
+                res_max = 0.0;
+                for (int k = 0; k < r; ++k)
+                    res_max = std::max(res_max, std::abs(g[(size_t)used * r + k]) / std::max(bnorm[k], 1e-300));
+                break;
+            }
         }
         res_out = res_max;
         return it_total;
@@ -1871,7 +1907,7 @@ static void solve_krylov(SolvePlan &pl, const tt_channel_os &os, const tt_solve_
     double res = 0.0;
     int iters = 0;
     if (!mixed) {
-        iters = gm.solve(pl.b.p, pl.x.p, o.tol, o.max_iter, false, res, bnorm);
+        iters = gm.solve(pl.b.p, pl.x.p, o.tol, o.max_iter, false, res, bnorm, true);
     } else {
         // iterative refinement: FP64 residual, FP32-product GMRES corrections
         fit(pl.rres);
@@ -1884,13 +1920,13 @@ static void solve_krylov(SolvePlan &pl, const tt_channel_os &os, const tt_solve_
             res = 0.0;
             for (int k = 0; k < L.r; ++k) res = std::max(res, rn[k] / std::max(bnorm[k], 1e-300));
             if (res <= o.tol || iters >= o.max_iter) break;
-            TT_HIP(hipMemsetAsync(pl.d.p, 0, sizeof(double2) * ne, op.s));
+            TT_HIP(hipMemsetAsync(pl.d.p, 0, sizeof(double2) * ne, op.s));   // the solve adds its update to d
             double inner_res = 0.0;
             // inner tolerance relative to this residual
             std::vector<double> rnorm = rn;
             double inner_tol = std::max(1e-6, o.tol / std::max(res, 1e-300));
             inner_tol = std::min(inner_tol, 1e-4);
-            iters += gm.solve(pl.rres.p, pl.d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm);
+            iters += gm.solve(pl.rres.p, pl.d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm, true, false);
             hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.x.p, pl.x.p, pl.d.p, 1.0, 1.0, ne);
         }
     }
@@ -2034,7 +2070,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->NM_max = 14;
     o->neumann_terms = nullptr;
     o->p123_format = TT_P123_CELLS;
-    o->use_graphs = 1;
+    o->use_graphs = 0;
     o->resolvent_on_device = 0;
 }
 
@@ -2279,7 +2315,7 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     std::vector<double> bnorm = gc.norms(colp.b.p);
     TT_HIP(hipMemsetAsync(colp.x.p, 0, sizeof(double2) * ne, s));
     double res_a = 0.0;
-    gc.solve(colp.b.p, colp.x.p, o.tol, o.max_iter, false, res_a, bnorm);
+    gc.solve(colp.b.p, colp.x.p, o.tol, o.max_iter, false, res_a, bnorm, true);
     g.residual_adjoint = res_a;
     g.n_apply_adjoint = Cl.n_apply;
     if (!(res_a <= o.tol))

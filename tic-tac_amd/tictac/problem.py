@@ -130,7 +130,7 @@ class Problem:
         return Channel(self, c, T_lab)
 
     def solve_batch(self, ctx, channels, P123s, method=L.SOLVE_MIXED, tol=1e-12, max_iter=2000, restart=40,
-                    graphs=True):
+                    graphs=False):
         """tt_solve_batch over channels (each with all its on-shell bins), G on the device.
         Returns [(U[d_row, d_col, j], stats, status)] per channel; raises on the first failure
         after every channel was attempted."""
@@ -307,7 +307,7 @@ class Channel:
         return G.view(np.complex128).reshape(len(E), self.dim)
 
     def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
-              neumann=False, p123_format=L.P123_CELLS, graphs=True, device_resolvent=False):
+              neumann=False, p123_format=L.P123_CELLS, graphs=False, device_resolvent=False):
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
 
         With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n].
