@@ -748,16 +748,18 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, in
 }
 
 // ------------------------------------------------------------------ Krylov vector ops
-// partial[i][b][k] = sum over block b's slice of conj(V_i[e]) W[e] in column k, summed over the
-// blocks either by a separate launch (k_dots_reduce, fixed order) or, with accum, by device
-// atomics into accum[i][k] (zeroed beforehand; the order of the additions then varies from run
-// to run, at the level of rounding).  A single-launch "last block finishes" scheme would need a
+// accum[i][k] += sum over block b's slice of conj(V_i[e]) W[e] in column k (accum zeroed
+// beforehand), one workgroup per (slice, vector), the blocks' sums added by device atomics (the
+// order of the additions then varies from run to run, at the level of rounding).  Few slices
+// (each thread walking several batches of DOT_UN rows) run faster than one batch per thread:
+// on MI355X the start and end of a workgroup (its reduction, its atomics) cost more than the
+// loads it saves.  A single-launch "last block finishes" scheme would need a
 // device-scope fence per block, which writes back the whole L2 of the block's XCD and costs
 // several times the dot products themselves; atomics need none.
 constexpr int DOT_UN = 8;
 
 __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const double2 *__restrict__ W,
-                       double2 *__restrict__ partial, int64_t n_elem, int r, double2 *__restrict__ accum) {
+                       int64_t n_elem, int r, double2 *__restrict__ accum) {
     extern __shared__ double2 sh[];
     const int i = blockIdx.y;
     const int b = blockIdx.x, nb = gridDim.x;
@@ -795,12 +797,8 @@ __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const dou
             ar += sh[s2 * r + threadIdx.x].x;
             ai += sh[s2 * r + threadIdx.x].y;
         }
-        if (accum) {   // straight into the (zeroed) result: one launch, no reduction pass
-            atomicAdd(&accum[(int64_t)i * r + threadIdx.x].x, ar);
-            atomicAdd(&accum[(int64_t)i * r + threadIdx.x].y, ai);
-        } else {
-            partial[((int64_t)i * nb + b) * r + threadIdx.x] = make_double2(ar, ai);
-        }
+        atomicAdd(&accum[(int64_t)i * r + threadIdx.x].x, ar);
+        atomicAdd(&accum[(int64_t)i * r + threadIdx.x].y, ai);
     }
 }
 
@@ -1384,10 +1382,10 @@ struct Gmres {
     int r, m;
     int64_t ne;
     bool use_graphs;
-    // blocks per vector of k_dots: each thread takes about one batch of DOT_UN rows
+    // row slices of k_dots per vector (at most 64)
     int nb_dots = 1;
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
-    DevBuf<double2> V, R, zdev, partial, Hd, n2d;
+    DevBuf<double2> V, R, zdev, Hd, n2d;
     double2 *Hh = nullptr;              // pinned mirror of Hd
     std::vector<hipEvent_t> step_ev;
     struct StepGraph {
@@ -1405,8 +1403,7 @@ struct Gmres {
         R.alloc(ne);
         zdev.alloc((size_t)(m + 1) * r);
         const int tpr = (256 - 256 % r) / r;
-        nb_dots = (int)std::min<int64_t>(512, std::max<int64_t>(1, (o.dim + (int64_t)tpr * DOT_UN - 1) / ((int64_t)tpr * DOT_UN)));
-        partial.alloc(std::max<size_t>((size_t)nb_dots * (m + 1) * r, (size_t)nblk(ne, 256 - 256 % r) * r));
+        nb_dots = (int)std::min<int64_t>(64, std::max<int64_t>(1, (o.dim + (int64_t)tpr * DOT_UN - 1) / ((int64_t)tpr * DOT_UN)));
         Hd.alloc((size_t)m * slot_len);
         n2d.alloc(r);
         TT_HIP(hipHostMalloc(&Hh, sizeof(double2) * (size_t)m * slot_len, hipHostMallocDefault));
@@ -1430,8 +1427,7 @@ struct Gmres {
     // out (zeroed beforehand) += <Vb_i, Wv>_k for i < nvec, on the device
     void dots_dev(const double2 *Vb, int nvec, const double2 *Wv, double2 *out) {
         int bs = 256 - 256 % r;
-        hipLaunchKernelGGL(k_dots, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, partial.p, ne,
-                           r, out);
+        hipLaunchKernelGGL(k_dots, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, ne, r, out);
     }
 
     std::vector<double> norms(const double2 *Wv) {
@@ -1913,9 +1909,13 @@ static void solve_krylov(SolvePlan &pl, const tt_channel_os &os, const tt_solve_
         fit(pl.rres);
         fit(pl.d);
         for (int outer = 0; outer < 20; ++outer) {
-            gm.apply_M(pl.x.p, pl.rres.p, false);
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.rres.p, pl.b.p, pl.rres.p, 1.0,
-                               -1.0, ne);
+            if (outer == 0) {   // x = 0: the residual is b
+                TT_HIP(hipMemcpyAsync(pl.rres.p, pl.b.p, sizeof(double2) * ne, hipMemcpyDeviceToDevice, op.s));
+            } else {
+                gm.apply_M(pl.x.p, pl.rres.p, false);
+                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.rres.p, pl.b.p, pl.rres.p,
+                                   1.0, -1.0, ne);
+            }
             std::vector<double> rn = gm.norms(pl.rres.p);
             res = 0.0;
             for (int k = 0; k < L.r; ++k) res = std::max(res, rn[k] / std::max(bnorm[k], 1e-300));
