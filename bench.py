@@ -1,15 +1,18 @@
 """Benchmark of the AGS/WPCD solve path on one MI355X.
 
-Step = one channel-solve: the resolvent G of the on-shell energies plus the
-on-shell U-matrix solve, for the 3N channel J^Pi = 1/2+ of the seeded synthetic
-NN potential at Np = Nq = 30 (N_alpha = 27, dim = 24300), T_lab = 10 MeV
-(2 on-shell q bins x 2 deuteron channels = 4 coupled systems).  P123, V and the
-SWP basis are built once before timing and stay resident in HBM.
+Step = one pass over the batched workload: every 3N channel 2J = 1..17 of both parities (18
+channels, N_alpha 27..63, dim 24300..56700) of the seeded synthetic NN potential at
+Np = Nq = 30, at the 8 lab energies linspace(1, 200, 8) MeV, i.e. 144 channel-solves (one
+channel-solve = every on-shell U element of one channel at one requested T_lab).  A step computes
+the resolvent of every on-shell bin on the device and solves all channels through tt_solve_batch;
+P123, V and the SWP basis are built once before timing and stay resident in HBM.
 
-The headline value is the mixed-precision solve (FP32 P.V.G products with FP64
-iterative refinement to |dU/U| <= 1e-8); the plain FP64 GMRES path is timed on
-the same inputs and reported beside it, with the accuracy of the mixed result
-against it.  One JSON line on stdout.
+The headline value is that batched solve in mixed precision (FP32 P.V.G products, FP64 iterative
+refinement).  Beside it, on the same 144 channel-solves: the batched solve in FP64, and the plain
+FP64 path (one tt_solve_faddeev_equations call per channel and T_lab, FP64 GMRES), with the
+relative error of every mixed U element against the plain FP64 one (target <= 1e-8).  Further
+lines: the single-channel J^Pi = 1/2+ workload, Np = Nq = 40 precision, the gradient cost and the
+CPU baseline.  One JSON line on stdout.
 """
 import argparse
 import json
@@ -25,73 +28,142 @@ import numpy as np  # noqa: E402
 import tictac as tt  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = ("synthetic NN (3 Yukawas + contacts, seed 7), J^Pi=1/2+, Np=Nq=30, J2max=3, N_alpha=27, "
-            "dim=24300, T_lab=10 MeV -> 2 on-shell bins x 2 deuteron channels")
+METRIC = "U-matrix channel-solves/s batched-mixed vs plain FP64 @|ΔU/U|≤1e-8; grad/fwd time"
+SOLVE_TOL = 1e-12
+BATCH_T = [float(t) for t in np.linspace(1.0, 200.0, 8)]
+BATCH_WORKLOAD = ("synthetic NN (3 Yukawas + contacts + 3 separables, seed 7), all 3N channels 2J=1..17 both "
+                  "parities (18 channels, N_alpha 27..63, dim 24300..56700), Np=Nq=30, J2max=3, T_lab = "
+                  "linspace(1, 200, 8) MeV -> 144 channel-solves (6 distinct on-shell bins per channel)")
+SINGLE_WORKLOAD = ("same synthetic NN, J^Pi=1/2+, Np=Nq=30, J2max=3, N_alpha=27, dim=24300, T_lab=10 MeV -> "
+                   "2 on-shell bins x 2 deuteron channels")
 
 
-def build(N=30, seed=7, T_lab=(10.0,)):
-    rp = tt.run_params(two_J_3N_max=1, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N))
-    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(seed)))
-    ch = prob.channel(0, list(T_lab))
-    return prob, ch
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def timed_steps(ctx, ch, P, method, steps, warmup, host_G=False):
-    """One step = resolvent of the on-shell energies + the on-shell U solve.  By default G is
-    computed on the device inside the solve (inputs resident in HBM); host_G routes it through
-    host memory as the reference's G_array does (tt_calculate_resolvent, then the solve uploads
-    it), i.e. the PCIe-inclusive rate."""
-    def one():
-        if host_G:
-            return ch.solve(ctx, P, ch.resolvent(ctx), method=method)
-        return ch.solve(ctx, P, None, method=method, device_resolvent=True)
-    stats = []
-    U = None
+def wp_params(N, two_J_max):
+    return tt.run_params(two_J_3N_max=two_J_max, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N))
+
+
+class BatchWorkload:
+    """The 18 channels at the 8 energies: one Channel per channel holding all energies (the batched
+    solve) and one per (channel, T_lab) (the plain path), with one P123 per channel."""
+
+    def __init__(self, ctx):
+        self.prob = tt.Problem(wp_params(30, 17), tt.Potential("synthetic", tt.synthetic_params(7)))
+        t0 = time.perf_counter()
+        self.chans = [self.prob.channel(c, BATCH_T) for c in range(self.prob.n_channels)]
+        self.Ps = [ch.build_p123(ctx) for ch in self.chans]
+        ctx.synchronize()
+        self.build_s = time.perf_counter() - t0
+        self.plain = [[self.prob.channel(c, [T]) for T in BATCH_T] for c in range(self.prob.n_channels)]
+        self.n_solves = len(self.chans) * len(BATCH_T)
+
+    def solve(self, ctx, method):
+        return self.prob.solve_batch(ctx, self.chans, self.Ps, method=method, tol=SOLVE_TOL)
+
+    def solve_plain(self, ctx, method=tt.SOLVE_GMRES):
+        return [[ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)[0] for ch in row]
+                for row, P in zip(self.plain, self.Ps)]
+
+    def accuracy(self, U_batch, U_plain):
+        """Relative error of every on-shell element of the batched U against the plain one of the same
+        channel and bin: max |dU|/|U| over elements, and max |dU| / max |U| per channel."""
+        elem = norm = 0.0
+        for c, (Ub, row) in enumerate(zip(U_batch, U_plain)):
+            qb = list(self.chans[c].q_idx)
+            for ch, Up in zip(self.plain[c], row):
+                for jp, q in enumerate(ch.q_idx):
+                    a, b = Ub[:, :, qb.index(q)], Up[:, :, jp]
+                    elem = max(elem, float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))))
+                    norm = max(norm, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+        return elem, norm
+
+
+def time_batches(ctx, wl, method, steps, warmup):
     for _ in range(warmup):
-        U, st = one()
+        out = wl.solve(ctx, method)
     ctx.synchronize()
+    stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        U, st = one()
-        stats.append(st)
+        out = wl.solve(ctx, method)
+        stats.extend(st for (_, st, _) in out)
     ctx.synchronize()
-    return time.perf_counter() - t0, U, stats
+    return time.perf_counter() - t0, [u for (u, _, _) in out], stats
 
 
-BATCH_WORKLOAD = ("same synthetic NN, all 3N channels 2J=1..17 both parities (18 channels, N_alpha 27..63, "
-                  "dim 24300..56700), Np=Nq=30, T_lab = linspace(1, 200, 8) MeV -> 6 on-shell bins per channel, "
-                  "each channel one block Krylov system (tt_solve_batch)")
+def product_roofline(stats, kernel, fp32=False, traffic=None):
+    """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
+    algorithmic bytes per launch (every block value once, the S panel read and the U panel
+    written once, the cell indexes) over the launch time from the HIP events that bracket each
+    launch on the solve stream during the timed steps; traffic = measured HBM bytes per launch."""
+    if fp32:   # only the FP32 launches of the mixed solves (its FP64 residual applications are apart)
+        ms = sum(s.ms_spmm_fp32 for s in stats)
+        nb = sum(s.spmm_bytes_fp32 for s in stats)
+        n = sum(s.n_spmm_fp32 for s in stats)
+    else:
+        ms = sum(s.ms_spmm for s in stats)
+        nb = sum(s.spmm_bytes for s in stats)
+        n = sum(s.n_spmm for s in stats)
+    achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+            "bytes_per_launch": round(nb / max(n, 1))}
 
 
-def batched(ctx, reps=2):
-    """All channels x 8 energies through tt_solve_batch (G on the device, P123 resident):
-    channel-solves/s = 18 * 8 / wall time of one batch, mixed and FP64, best of reps."""
-    T = list(np.linspace(1.0, 200.0, 8))
-    rp = tt.run_params(two_J_3N_max=17, J_2N_max=3, Np_WP=30, Nq_WP=30, chebyshev_s=tt.chebyshev_scale(30))
-    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
-    t0 = time.perf_counter()
-    chans = [prob.channel(c, T) for c in range(prob.n_channels)]
-    Ps = [ch.build_p123(ctx) for ch in chans]
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("pmc_cellmfma_synthetic_np30_batch.md", "pmc_cellmfma_synthetic_np30.md")]
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised
+    in profiles/ (scaled by a copy-kernel calibration), or None."""
+    for path in TRAFFIC_FILES:
+        try:
+            for line in open(path):
+                if line.startswith("traffic_bytes_per_launch " + kernel + " = "):
+                    return int(line.rsplit("=", 1)[1])
+        except OSError:
+            pass
+    return None
+
+
+def single_channel(ctx, steps, warmup):
+    """The J^Pi = 1/2+ channel at 10 MeV alone: mixed, FP64 and PCIe-inclusive channel-solves/s."""
+    prob = tt.Problem(wp_params(30, 1), tt.Potential("synthetic", tt.synthetic_params(7)))
+    ch = prob.channel(0, [10.0])
+    P = ch.build_p123(ctx)
     ctx.synchronize()
-    t_build = time.perf_counter() - t0
-    res = {}
-    U = {}
-    for method, name in ((tt.SOLVE_MIXED, "mixed"), (tt.SOLVE_GMRES, "fp64")):
-        prob.solve_batch(ctx, chans, Ps, method=method)   # warm-up: plans and Krylov workspaces
-        best = None
-        for _ in range(reps):
-            ctx.synchronize()
-            t1 = time.perf_counter()
-            out = prob.solve_batch(ctx, chans, Ps, method=method)
-            ctx.synchronize()
-            dt = time.perf_counter() - t1
-            best = dt if best is None else min(best, dt)
-        res[name] = round(len(chans) * len(T) / best, 2)
-        U[name] = [u for (u, _, _) in out]
-    rel = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(U["mixed"], U["fp64"]))
-    return {"workload": BATCH_WORKLOAD, "unit": "channel-solves/s", "channel_solves": len(chans) * len(T),
-            "mixed": res["mixed"], "fp64": res["fp64"], "mixed_vs_fp64_max_rel": rel,
-            "p123_build_s_all_channels": round(t_build, 2)}
+
+    def run(method, host_G, n, w):
+        def one():
+            if host_G:
+                return ch.solve(ctx, P, ch.resolvent(ctx), method=method, tol=SOLVE_TOL)
+            return ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)
+        for _ in range(w):
+            one()
+        ctx.synchronize()
+        st, t0 = [], time.perf_counter()
+        for _ in range(n):
+            U, s = one()
+            st.append(s)
+        ctx.synchronize()
+        return n / (time.perf_counter() - t0), U, st
+
+    v_mix, U_mix, st_mix = run(tt.SOLVE_MIXED, False, steps, warmup)
+    v_64, U_64, st_64 = run(tt.SOLVE_GMRES, False, steps, warmup)
+    v_pcie, _, _ = run(tt.SOLVE_MIXED, True, steps, 1)
+    return {"workload": SINGLE_WORKLOAD, "unit": "channel-solves/s", "nnz_P123": int(P.nnz),
+            "p123_build_ms": round(P.stats.ms_total, 2), "mixed": round(v_mix, 2), "fp64": round(v_64, 2),
+            "mixed_pcie_inclusive": round(v_pcie, 2),
+            "mixed_vs_fp64_max_rel": float(np.max(np.abs(U_mix - U_64)) / np.max(np.abs(U_64))),
+            "roofline_fp32": product_roofline(st_mix, "k_cellmfma<float, 2, 1, 4>", True,
+                                              measured_traffic("k_cellmfma<float, 2, 1, 4>")),
+            "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, 2, 1, 2>", False,
+                                              measured_traffic("k_cellmfma<double, 2, 1, 2>"))}
 
 
 GRAD_WORKLOAD = ("same synthetic NN, channels 2J=1,3 both parities (N_alpha 27, 47; dim 24300, 42300), Np=Nq=30, "
@@ -103,8 +175,7 @@ def gradient_ratio(ctx, reps=3):
     solve + assembly) over the forward alone (same call without the derivative: V, SWP
     states, G and the row solve), summed over the 4 channels, best of reps each."""
     T = [5.0, 50.0, 150.0]
-    rp = tt.run_params(two_J_3N_max=3, J_2N_max=3, Np_WP=30, Nq_WP=30, chebyshev_s=tt.chebyshev_scale(30))
-    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
+    prob = tt.Problem(wp_params(30, 3), tt.Potential("synthetic", tt.synthetic_params(7)))
     basis = prob.basis_matrices(12)
     th = tt.synthetic_params()[:12]
     t_f = t_g = 0.0
@@ -128,21 +199,6 @@ def gradient_ratio(ctx, reps=3):
             "ms_forward_plus_gradient": round(1e3 * t_g, 2)}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_cellmfma_synthetic_np30.md")
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` at this workload from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes summarised in profiles/ (scaled by a copy-kernel calibration), or None."""
-    try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch " + kernel + " = "):
-                return int(line.rsplit("=", 1)[1])
-    except OSError:
-        pass
-    return None
-
-
 N40_WORKLOAD = ("same synthetic NN at Np=Nq=40: channels J^Pi=1/2+ (N_alpha 27, dim 43200) and 7/2+ (N_alpha 63, "
                 "dim 100800), T_lab = 10 MeV (2 on-shell bins)")
 
@@ -151,9 +207,7 @@ def precision_n40(ctx, reps=3):
     """Mixed precision at Np=Nq=40: channel-solves/s (mixed and FP64, best of reps, G on the
     device) and the relative error of every on-shell U element of the mixed solve against the
     converged FP64 solve, per channel and energy (target <= 1e-8)."""
-    N = 40
-    rp = tt.run_params(two_J_3N_max=7, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N))
-    prob = tt.Problem(rp, tt.Potential("synthetic", tt.synthetic_params(7)))
+    prob = tt.Problem(wp_params(40, 7), tt.Potential("synthetic", tt.synthetic_params(7)))
     out = []
     for c in range(prob.n_channels):
         ch = prob.channel(c, [10.0])
@@ -164,11 +218,11 @@ def precision_n40(ctx, reps=3):
                "p123_build_ms": round(P.stats.ms_total, 1)}
         U = {}
         for method, name in ((tt.SOLVE_MIXED, "mixed"), (tt.SOLVE_GMRES, "fp64")):
-            ch.solve(ctx, P, None, method=method, device_resolvent=True)
+            ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)
             best = None
             for _ in range(reps):
                 t0 = time.perf_counter()
-                U[name], st = ch.solve(ctx, P, None, method=method, device_resolvent=True)
+                U[name], st = ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)
                 dt = time.perf_counter() - t0
                 best = dt if best is None else min(best, dt)
             res[name] = round(1.0 / best, 2)
@@ -179,43 +233,29 @@ def precision_n40(ctx, reps=3):
     return {"workload": N40_WORKLOAD, "unit": "channel-solves/s", "channels": out}
 
 
-def product_roofline(stats, kernel, fp32=False):
-    """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
-    algorithmic bytes per launch (every block value once, the S panel read and the U panel
-    written once, the cell indexes) over the launch time from the HIP events that bracket each
-    launch on the solve stream during the timed steps; traffic = measured HBM bytes per launch."""
-    if fp32:   # only the FP32 launches of the mixed solves (its FP64 residual applications are the other kernel)
-        ms = sum(s.ms_spmm_fp32 for s in stats)
-        nb = sum(s.spmm_bytes_fp32 for s in stats)
-        n = sum(s.n_spmm_fp32 for s in stats)
-    else:
-        ms = sum(s.ms_spmm for s in stats)
-        nb = sum(s.spmm_bytes for s in stats)
-        n = sum(s.n_spmm for s in stats)
-    achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
-            "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
-            "bytes_per_launch": round(nb / max(n, 1))}
-
-
-def cpu_baseline(prob, ch, P, G, budget_s=20.0):
-    """Oracle (numpy/scipy, 1 thread) converged GMRES on the same inputs: channel-solves/s."""
+def cpu_baseline(wl, ctx, U_plain, budget_s=20.0):
+    """Oracle (numpy/scipy, 1 thread) converged GMRES on the same inputs: channel-solves of the
+    workload's first channel (J^Pi = 1/2+), one requested T_lab at a time, until the budget is
+    spent.  Returns channel-solves/s and the largest relative difference to the GPU plain FP64 U."""
     sys.path.insert(0, ROOT)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from threadpoolctl import threadpool_limits
     import scipy.sparse.linalg as sla
     from oracle import solve as so
     from helpers import oracle_pw, oracle_swp, oracle_V, flags
+    prob, P = wl.prob, wl.Ps[0]
     val, row, ptr = P.to_host()
     tf, ib = flags(prob)
     Vu, Vc = oracle_V(prob)
     N = prob.rp.Np_WP
+    done, diff = 0, 0.0
     with threadpool_limits(limits=1):
-        K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
-        done, t0 = 0, time.perf_counter()
-        U = np.zeros((ch.nd, ch.nd, ch.nE), complex)
-        while True:
+        ch0 = wl.plain[0][0]
+        K = so.KernelA(oracle_pw(ch0), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch0.dim), tf, ib)
+        t0 = time.perf_counter()
+        for i, ch in enumerate(wl.plain[0]):
+            G = ch.resolvent(ctx)
+            U = np.zeros((ch.nd, ch.nd, ch.nE), complex)
             for dr in range(ch.nd):
                 for j in range(ch.nE):
                     e = np.zeros(ch.dim)
@@ -224,80 +264,109 @@ def cpu_baseline(prob, ch, P, G, budget_s=20.0):
                     g = G[j]
                     op = sla.LinearOperator((ch.dim, ch.dim), dtype=complex,
                                             matvec=lambda v, g=g: v - K.row_apply((v * g)[None, :])[0])
-                    y, _ = sla.gmres(op, b, rtol=1e-12, restart=60, maxiter=50)
+                    y, _ = sla.gmres(op, b, rtol=SOLVE_TOL, restart=60, maxiter=50)
                     for dc in range(ch.nd):
                         U[dr, dc, j] = y[ch.deut[dc] * N * N + ch.q_idx[j] * N]
             done += 1
-            el = time.perf_counter() - t0
-            if el > budget_s / 2 or done >= 8:
+            ref = U_plain[0][i]
+            diff = max(diff, float(np.max(np.abs(U - ref)) / np.max(np.abs(ref))))
+            if time.perf_counter() - t0 > budget_s / 2:
                 break
+        el = time.perf_counter() - t0
     return {"value": round(done / el, 4), "unit": "channel-solves/s", "cores": 1, "kind": "port",
-            "sample": "%d full channel-solve(s) (4 GMRES systems each, rtol 1e-12) of the same workload, "
-                      "scipy.sparse on 1 thread" % done}, U
+            "sample": "%d channel-solve(s) of the workload's J^Pi=1/2+ channel (the smallest; T_lab %s MeV, "
+                      "2 on-shell bins x 2 deuteron channels each, GMRES rtol 1e-12), scipy.sparse on 1 thread"
+                      % (done, ", ".join("%.4g" % t for t in BATCH_T[:done])),
+            "max_rel_vs_gpu_plain_fp64": diff}
+
+
+LEGS = ("plain", "single", "n40", "grad", "cpu")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batched", action="store_true", help="skip the all-channel batched line")
-    ap.add_argument("--no-grad", action="store_true", help="skip the gradient line")
-    ap.add_argument("--no-n40", action="store_true", help="skip the Np=Nq=40 precision line")
+    ap.add_argument("--steps", type=int, default=10, help="timed batches (144 channel-solves each)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated extra lines beside the headline batch: " + ", ".join(LEGS) + " (or none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
-              "independent; see DESIGN.md for how the work would shard)", file=sys.stderr)
+              "independent; the design says how the work would shard)", file=sys.stderr)
         sys.exit(2)
+    legs = set() if args.legs == "none" else set(x for x in args.legs.split(",") if x)
+    bad = legs - set(LEGS)
+    if bad:
+        ap.error("unknown legs: " + ", ".join(sorted(bad)))
 
-    prob, ch = build()
     ctx = tt.Context(0)
-    P = ch.build_p123(ctx)
-    ctx.synchronize()
-
-    t_mix, U_mix, st_mix = timed_steps(ctx, ch, P, tt.SOLVE_MIXED, args.steps, args.warmup)
-    t_64, U_64, st_64 = timed_steps(ctx, ch, P, tt.SOLVE_GMRES, args.steps, args.warmup)
-    t_pcie, _, _ = timed_steps(ctx, ch, P, tt.SOLVE_MIXED, args.steps, 1, host_G=True)
-    rel = float(np.max(np.abs(U_mix - U_64)) / np.max(np.abs(U_64)))
-    value = args.steps / t_mix
-    fp64 = args.steps / t_64
-
+    log("building P123 of the 18 channels")
+    wl = BatchWorkload(ctx)
+    log("P123 built in %.1f s; timing the batched mixed solve" % wl.build_s)
+    t_mix, U_mix, st_mix = time_batches(ctx, wl, tt.SOLVE_MIXED, args.steps, args.warmup)
+    log("batched FP64")
+    n64 = max(2, args.steps // 2)
+    t_64, U_64, st_64 = time_batches(ctx, wl, tt.SOLVE_GMRES, n64, 1)
+    value = wl.n_solves * args.steps / t_mix
+    fp64 = wl.n_solves * n64 / t_64
+    k32 = "k_cellmfma<float, MT, NT, 4>"
     out = {
-        "metric": "U-matrix channel-solves/s batched-mixed vs plain FP64 @|\u0394U/U|\u22641e-8; grad/fwd time",
-        "value": round(value, 3),
+        "metric": METRIC,
+        "value": round(value, 2),
         "unit": "channel-solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_mix / args.steps, 4),
+        "ms_per_step": round(1e3 * t_mix / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 products + fp64 refinement (complex)",
+        "dtype": "complex fp32 products + fp64 refinement",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD, "nnz_P123": int(P.nnz), "p123_build_ms": round(P.stats.ms_total, 2)},
-        "fp64_value": round(fp64, 3),
-        "mixed_over_fp64": round(value / fp64, 3),
-        "value_pcie_inclusive": round(args.steps / t_pcie, 3),
-        "mixed_vs_fp64_max_rel": rel,
+        "config": {"workload": BATCH_WORKLOAD, "channel_solves_per_step": wl.n_solves,
+                   "nnz_P123_total": int(sum(P.nnz for P in wl.Ps)),
+                   "p123_build_s_all_channels": round(wl.build_s, 2), "solve_tol": SOLVE_TOL},
+        "batched_fp64": round(fp64, 2),
+        "mixed_over_batched_fp64": round(value / fp64, 3),
+        "plain_fp64": None,
+        "mixed_over_plain_fp64": None,
         "grad_over_fwd": None,
-        "roofline": product_roofline(st_mix, "k_cellmfma<float, 2, 1, 4>", fp32=True),
-        "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, 2, 1, 2>"),
+        "roofline": product_roofline(st_mix, k32, True, measured_traffic(k32)),
+        "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, MT, NT, 2>", False,
+                                          measured_traffic("k_cellmfma<double, MT, NT, 2>")),
     }
-    if not args.no_batched:
-        out["batched"] = batched(ctx)
-    if not args.no_n40:
+    U_plain = None
+    if "plain" in legs or "cpu" in legs:
+        log("plain FP64: one call per channel and T_lab")
+        wl.solve_plain(ctx)   # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        U_plain = wl.solve_plain(ctx)
+        ctx.synchronize()
+        t_plain = time.perf_counter() - t0
+        elem, norm = wl.accuracy(U_mix, U_plain)
+        elem64, norm64 = wl.accuracy(U_64, U_plain)
+        out["plain_fp64"] = round(wl.n_solves / t_plain, 2)
+        out["mixed_over_plain_fp64"] = round(value / out["plain_fp64"], 3)
+        out["mixed_vs_plain_fp64_max_rel_elem"] = elem
+        out["mixed_vs_plain_fp64_max_rel_norm"] = norm
+        out["batched_fp64_vs_plain_fp64_max_rel_elem"] = elem64
+        out["accuracy_target_met"] = bool(elem <= 1e-8)
+    if "single" in legs:
+        log("single channel 1/2+")
+        out["single_channel"] = single_channel(ctx, 20, 3)
+    if "n40" in legs:
+        log("Np=Nq=40")
         out["precision_n40"] = precision_n40(ctx)
-    if not args.no_grad:
+    if "grad" in legs:
+        log("gradient")
         out["gradient"] = gradient_ratio(ctx)
         out["grad_over_fwd"] = out["gradient"]["grad_over_fwd"]
-    if not args.no_cpu:
-        G = ch.resolvent(ctx)
-        cb, U_cpu = cpu_baseline(prob, ch, P, G)
-        cb["max_rel_vs_gpu_fp64"] = float(np.max(np.abs(U_cpu - U_64)) / np.max(np.abs(U_64)))
-        out["cpu_baseline"] = cb
-    print(json.dumps(out))
+    if "cpu" in legs:
+        log("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(wl, ctx, U_plain)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <type_traits>
 
 #include "tt_internal.h"
 
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
                                                const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
                                                const double *__restrict__ blocks, const double2 *__restrict__ G,
                                                const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
-                                               int ne_max, const double2 *__restrict__ Zsub, int in_cell,
+                                               int ne_max, const TO *__restrict__ Zsub, int in_cell,
                                                int out_cell, const int *__restrict__ ket_item_ptr,
                                                double in_scale, double2 *__restrict__ zero_buf, int zero_n) {
     extern __shared__ double smem[];
@@ -375,21 +376,23 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         }
         const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
         if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
-            sr = Zsub[dst].x - sr;
-            si = Zsub[dst].y - si;
+            sr = (double)Zsub[dst].x - sr;
+            si = (double)Zsub[dst].y - si;
         }
         Y[dst].x = sr;
         Y[dst].y = si;
     }
 }
 
-// Launch of k_block over all (a, q) and k-chunks of at most 16 columns.
+// Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
+// (every chunk stages all partner blocks, so a narrow remainder chunk costs a full one).
 template <typename TI, typename TO>
 static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
                          const double *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
-                         int r, const double2 *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
+                         int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
                          double in_scale, double2 *zero_buf = nullptr, int zero_n = 0) {
-    const int kc = std::min(r, 16);
+    const int nch = (r + 23) / 24;
+    const int kc = (r + nch - 1) / nch;
     const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(double) + (size_t)ne_max * Np * kc * sizeof(double2);
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
@@ -670,16 +673,32 @@ static void launch_cellmfma(hipStream_t s, const CellView *P, const CellItems &c
 }
 
 // N tiles per pass: up to 64 real components (32 complex right-hand sides) stream the blocks
-// once; wider batches take ceil(R2 / 64) passes.
+// once; wider batches take ceil(R2 / 64) passes.  Within that number of passes the narrowest
+// tile that covers R2 is taken: padded columns cost MFMA issue slots, and at Nalpha ~ 64 with
+// 36 components a 64-wide tile makes the FP64 product MFMA-bound (a 48-wide one covers 75%
+// useful columns instead of 56%).
+static int cell_nt(int R2) {
+    const int passes = (R2 + 63) / 64;
+    return std::min(4, (R2 + 16 * passes - 1) / (16 * passes));
+}
+
 template <typename T, int MT>
 static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
                                T *Upart, int R2) {
-    if (R2 <= 16)
-        launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
-    else if (R2 <= 32)
-        launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2);
-    else
-        launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2);
+    switch (cell_nt(R2)) {
+        case 1:
+            launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
+            break;
+        case 2:
+            launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2);
+            break;
+        case 3:
+            launch_cellmfma<T, MT, 3>(s, P, ci, blk, S, Upart, R2);
+            break;
+        default:
+            launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2);
+            break;
+    }
 }
 
 // Upart = the work items' partial panels (items * Nalpha * R2); when U is given also
@@ -758,12 +777,13 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, in
 // several times the dot products themselves; atomics need none.
 constexpr int DOT_UN = 8;
 
-__global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const double2 *__restrict__ W,
+template <typename VT>
+__global__ void k_dots(const VT *__restrict__ V, int64_t vstride, const VT *__restrict__ W,
                        int64_t n_elem, int r, double2 *__restrict__ accum) {
     extern __shared__ double2 sh[];
     const int i = blockIdx.y;
     const int b = blockIdx.x, nb = gridDim.x;
-    const double2 *Vi = V + (int64_t)i * vstride;
+    const VT *Vi = V + (int64_t)i * vstride;
     const int tpr = blockDim.x / r;  // threads per column k
     const int k = threadIdx.x % r;
     const int sub = threadIdx.x / r;
@@ -773,7 +793,7 @@ __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const dou
         // and masked, so no load sits behind a branch
         const int64_t rows = n_elem / r, step = (int64_t)nb * tpr;
         for (int64_t e = (int64_t)b * tpr + sub; e < rows; e += DOT_UN * step) {
-            double2 v[DOT_UN], w[DOT_UN];
+            VT v[DOT_UN], w[DOT_UN];
 #pragma unroll
             for (int u = 0; u < DOT_UN; ++u) {
                 const int64_t ee = min(e + u * step, rows - 1);
@@ -784,8 +804,9 @@ __global__ void k_dots(const double2 *__restrict__ V, int64_t vstride, const dou
 #pragma unroll
             for (int u = 0; u < DOT_UN; ++u) {
                 const double mk = e + u * step < rows ? 1.0 : 0.0;
-                sr += mk * (v[u].x * w[u].x + v[u].y * w[u].y);
-                si += mk * (v[u].x * w[u].y - v[u].y * w[u].x);
+                const double vx = v[u].x, vy = v[u].y, wx = w[u].x, wy = w[u].y;
+                sr += mk * (vx * wx + vy * wy);
+                si += mk * (vx * wy - vy * wx);
             }
         }
     }
@@ -846,11 +867,13 @@ __global__ __launch_bounds__(256) void k_dots_reduce(const double2 *__restrict__
 // The vectors are taken in batches of COMBO_UN with all loads issued before use (clamped and
 // masked past nvec), so each thread keeps COMBO_UN loads in flight.
 constexpr int COMBO_UN = 8;
-__device__ __forceinline__ double2 combo_element(double2 w, const double2 *__restrict__ V, int64_t vstride, int nvec,
+template <typename TV>
+__device__ __forceinline__ double2 combo_element(double2 w, const TV *__restrict__ V, int64_t vstride, int nvec,
                                                  const double2 *__restrict__ h, int64_t t, int k, int r,
                                                  double sign, double &hh) {
     for (int i0 = 0; i0 < nvec; i0 += COMBO_UN) {
-        double2 c[COMBO_UN], v[COMBO_UN];
+        double2 c[COMBO_UN];
+        TV v[COMBO_UN];
 #pragma unroll
         for (int u = 0; u < COMBO_UN; ++u) {
             const int i = min(i0 + u, nvec - 1);
@@ -861,20 +884,25 @@ __device__ __forceinline__ double2 combo_element(double2 w, const double2 *__res
 #pragma unroll
         for (int u = 0; u < COMBO_UN; ++u) {
             const double f = i0 + u < nvec ? sign : 0.0;
-            w.x += f * (c[u].x * v[u].x - c[u].y * v[u].y);
-            w.y += f * (c[u].x * v[u].y + c[u].y * v[u].x);
+            const double vx = v[u].x, vy = v[u].y;
+            w.x += f * (c[u].x * vx - c[u].y * vy);
+            w.y += f * (c[u].x * vy + c[u].y * vx);
             hh += f * f * (c[u].x * c[u].x + c[u].y * c[u].y);
         }
     }
     return w;
 }
 
-__global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride, int nvec,
+template <typename TW, typename TV>
+__global__ void k_combo(TW *__restrict__ W, const TV *__restrict__ V, int64_t vstride, int nvec,
                         const double2 *__restrict__ h, int64_t n_elem, int r, double sign) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_elem) return;
     double hh = 0.0;
-    W[t] = combo_element(W[t], V, vstride, nvec, h, t, (int)(t % r), r, sign, hh);
+    const TW w0 = W[t];
+    const double2 w = combo_element(make_double2(w0.x, w0.y), V, vstride, nvec, h, t, (int)(t % r), r, sign, hh);
+    W[t].x = w.x;
+    W[t].y = w.y;
 }
 
 // The last pass of a reorthogonalised Gram-Schmidt step, with the normalisation folded in:
@@ -884,17 +912,20 @@ __global__ void k_combo(double2 *__restrict__ W, const double2 *__restrict__ V, 
 // nu_k^2 goes to n2[k] (written by the first r threads) for the Hessenberg matrix.  After the
 // first projection h is of the order of the rounding of that projection, so the difference
 // cancels only when W itself has collapsed into span(V), where GMRES has converged.
-__global__ void k_combo_normalize(double2 *__restrict__ W, const double2 *__restrict__ V, int64_t vstride,
+template <typename VT>
+__global__ void k_combo_normalize(VT *__restrict__ W, const VT *__restrict__ V, int64_t vstride,
                                   int nvec, const double2 *__restrict__ h, int64_t n_elem, int r,
                                   double2 *__restrict__ n2) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_elem) return;
     const int k = (int)(t % r);
     double hh = 0.0;
-    const double2 w = combo_element(W[t], V, vstride, nvec, h, t, k, r, -1.0, hh);
+    const VT w0 = W[t];
+    const double2 w = combo_element(make_double2(w0.x, w0.y), V, vstride, nvec, h, t, k, r, -1.0, hh);
     const double nu2 = fmax(h[nvec * r + k].x - hh, 0.0);
     const double f = nu2 > 0.0 ? 1.0 / sqrt(nu2) : 0.0;
-    W[t] = make_double2(f * w.x, f * w.y);
+    W[t].x = f * w.x;
+    W[t].y = f * w.y;
     if (t < r) n2[t] = make_double2(nu2, 0.0);
 }
 
@@ -920,14 +951,16 @@ __global__ void k_unit_rows(double2 *__restrict__ X, const int64_t *__restrict__
 }
 
 // Y[e] = X[e] / sqrt(n2[k].x), zero where the squared norm is zero
-__global__ void k_scale_by_norm(double2 *__restrict__ Y, const double2 *__restrict__ X, const double2 *__restrict__ n2,
+template <typename TY>
+__global__ void k_scale_by_norm(TY *__restrict__ Y, const double2 *__restrict__ X, const double2 *__restrict__ n2,
                                 int64_t n_elem, int r) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_elem) return;
     const double nn = n2[t % r].x;
     const double f = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
     const double2 x = X[t];
-    Y[t] = make_double2(f * x.x, f * x.y);
+    Y[t].x = f * x.x;
+    Y[t].y = f * x.y;
 }
 
 // out[idx][k] = in[idx][k] * G[gk][idx] + (idx == unit[k])   (e + x .* G for the adjoint)
@@ -985,8 +1018,16 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 
 // ------------------------------------------------------------------ the operator
 // Matrix-free row-form application Y = (X .* G_sel) A for r rows at once.
+// Device copies of the resolvent's inputs (SWP edges, q edges, energies, per-alpha edge
+// offsets), kept by an operator so that computing its G allocates nothing and needs no wait.
+struct ResolventStaging {
+    DevBuf<double> eu, ec, qe, E;
+    DevBuf<int> which, offset;
+};
+
 struct RowOperator {
     hipStream_t s;
+    ResolventStaging rstage;
     int Na, Nq, Np, nE;
     int64_t dim;
     const DeviceCSC *P;
@@ -1215,8 +1256,13 @@ struct RowOperator {
     // With ext (four events), the timing events are the caller's (a captured graph records
     // them on every replay and its owner reads them with add_timing); otherwise timed
     // applications take four events of the pool, read back in collect().
-    void apply(const double2 *X, double2 *Y, int r, bool use_G, bool fp32, bool timed = true,
-               const double2 *Zsub = nullptr, const hipEvent_t *ext = nullptr) {
+    // VT: element type of X, Y and Zsub (double2, or float2 for the FP32 vectors of the inner
+    // solves of iterative refinement, which take the FP32 product).
+    template <typename VT>
+    void apply(const VT *X, VT *Y, int r, bool use_G, bool fp32, bool timed = true, const VT *Zsub = nullptr,
+               const hipEvent_t *ext = nullptr) {
+        constexpr bool v64 = std::is_same<VT, double2>::value;
+        TT_REQUIRE(v64 || fp32, "FP32 vectors take the FP32 product");
         prepare(r, fp32);
         if (!ext && timed && ev_used + 4 > 4096) collect();
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
@@ -1251,23 +1297,26 @@ struct RowOperator {
         zero_next = nullptr;
         zero_next_n = 0;
         if (!fp32) {
-            launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, use_G ? G.p : nullptr,
-                                           gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0, zb, zn);
-            if (timed) record(e1);
-            if (cells)
-                launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p), nullptr, upart.p, R2,
-                                      2.0);
-            else
-                launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
-                                    reinterpret_cast<double *>(U.p), dim, R2, 2.0);
-            if (timed) record(e2);
-            const double2 *Uin = cells ? reinterpret_cast<const double2 *>(upart.p) : U.p;
-            launch_block<double2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
-                                           Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
-            ++n_apply;
+            if constexpr (v64) {
+                launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
+                                               use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
+                                               1.0, zb, zn);
+                if (timed) record(e1);
+                if (cells)
+                    launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
+                                          upart.p, R2, 2.0);
+                else
+                    launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
+                                        reinterpret_cast<double *>(U.p), dim, R2, 2.0);
+                if (timed) record(e2);
+                const double2 *Uin = cells ? reinterpret_cast<const double2 *>(upart.p) : U.p;
+                launch_block<double2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr,
+                                               gidx.p, Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
+                ++n_apply;
+            }
         } else {
             ensure_f32();
-            launch_block<double2, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
+            launch_block<VT, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
                                           use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
                                           zb, zn);
             if (timed) record(e1);
@@ -1279,8 +1328,8 @@ struct RowOperator {
                                    reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
             if (timed) record(e2);
             const float2 *Uin = cells ? reinterpret_cast<const float2 *>(upart.p) : U32.p;
-            launch_block<float2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
-                                          Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
+            launch_block<float2, VT>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
+                                     Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
             ++n_apply;
             ++n_apply32;
         }
@@ -1370,114 +1419,143 @@ static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
     return det_lu(P, M + 1) / det_lu(Q, M + 1);
 }
 
-// ------------------------------------------------------------------ GMRES (batched)
-// r independent systems y (1 - G_k A) = b_k share every operator application.  The Arnoldi
-// step (operator, two classical Gram-Schmidt passes, normalisation) runs entirely on the
-// device; its Hessenberg column is copied to pinned host memory and consumed by the host's
-// Givens rotations one step later, so the host never stalls the stream inside a cycle.
-// With use_graphs each step (j, precision) is captured once into a HIP graph and replayed
-// afterwards: one launch per step instead of about a dozen.
-struct Gmres {
+// ------------------------------------------------------------------ Krylov engine
+// GMRES on r independent systems M y_k = b_k, M y = y - (y .* G_k) A, that share every
+// operator application: one Arnoldi basis per system (column k of the basis vectors).  VT is
+// the storage type of the basis: double2 for the FP64 solves, float2 for the inner solves of
+// iterative refinement (FP32 products and basis vectors; dot products, norms and the
+// Hessenberg arithmetic stay in FP64).
+// The host drives a cycle one step at a time: launch(j) enqueues Arnoldi step j (operator, two
+// classical Gram-Schmidt passes with the normalisation folded into the second, Hessenberg
+// column to pinned host memory); after its event, take_column(j) applies the Givens rotations
+// and updates the residual estimates.  The device runs one step ahead of the host except where
+// the residual history predicts that the step being waited for converges.  Several Krylov
+// objects (the channels of a batch) advance in lockstep on one stream (lockstep_cycle), so one
+// host wait serves all of them and the stream always holds every channel's next step.
+// With use_graphs each step (j) is captured once into a HIP graph and replayed afterwards.
+template <typename VT>
+struct Krylov {
     RowOperator &op;
     int r, m;
     int64_t ne;
     bool use_graphs;
-    // row slices of k_dots per vector (at most 64)
-    int nb_dots = 1;
+    int nb_dots = 1;                    // row slices of k_dots per vector (at most 64)
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
-    DevBuf<double2> V, R, zdev, Hd, n2d;
+    DevBuf<VT> V;                       // (m+1) basis vectors of ne elements
+    DevBuf<double2> zdev, Hd, n2d;
     double2 *Hh = nullptr;              // pinned mirror of Hd
+    double2 *zh = nullptr;              // pinned staging of the update coefficients
+    double2 *n2h = nullptr;             // pinned copy of n2d (squared norms)
     std::vector<hipEvent_t> step_ev;
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         uint64_t gen = 0;
     };
-    std::vector<StepGraph> graphs[2];   // [fp32][j]
+    std::vector<StepGraph> graphs;
+    static constexpr bool fp32 = std::is_same<VT, float2>::value;
 
-    Gmres(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
+    // state of the current cycle (host)
+    std::vector<cdouble> H, cs, sn, g;
+    std::vector<double> ref, prev_rel;  // residual norms are relative to ref[k]
+    double tol = 0.0;
+    int jmax = 0, used = 0, launched = 0;
+    bool hold = false, in_cycle = false, conv = false;
+    double est = 0.0;                   // largest relative residual estimate at the end of the cycle
+    bool graph_timing = true;
+
+    Krylov(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
         TT_REQUIRE(r <= 256, "at most 256 simultaneous systems");
         ne = o.dim * r;
         slot_len = (2 * (m + 1) + 1) * r;
         V.alloc((size_t)(m + 1) * ne);
-        R.alloc(ne);
         zdev.alloc((size_t)(m + 1) * r);
         const int tpr = (256 - 256 % r) / r;
         nb_dots = (int)std::min<int64_t>(64, std::max<int64_t>(1, (o.dim + (int64_t)tpr * DOT_UN - 1) / ((int64_t)tpr * DOT_UN)));
         Hd.alloc((size_t)m * slot_len);
         n2d.alloc(r);
         TT_HIP(hipHostMalloc(&Hh, sizeof(double2) * (size_t)m * slot_len, hipHostMallocDefault));
+        TT_HIP(hipHostMalloc(&zh, sizeof(double2) * (size_t)(m + 1) * r, hipHostMallocDefault));
+        TT_HIP(hipHostMalloc(&n2h, sizeof(double2) * r, hipHostMallocDefault));
         step_ev.resize(m);
         for (auto &e : step_ev) TT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        graphs[0].resize(m);
-        graphs[1].resize(m);
+        graphs.resize(m);
+        H.resize((size_t)(m + 1) * m * r);
+        cs.resize((size_t)m * r);
+        sn.resize((size_t)m * r);
+        g.resize((size_t)(m + 1) * r);
+        prev_rel.resize(r);
     }
-    ~Gmres() {
-        for (auto &gv : graphs)
-            for (auto &g : gv) {
-                if (g.exec) (void)hipGraphExecDestroy(g.exec);
-                for (auto e : g.ev)
-                    if (e) (void)hipEventDestroy(e);
-            }
+    ~Krylov() {
+        for (auto &gr : graphs) {
+            if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+            for (auto e : gr.ev)
+                if (e) (void)hipEventDestroy(e);
+        }
         for (auto e : step_ev) (void)hipEventDestroy(e);
         if (Hh) (void)hipHostFree(Hh);
+        if (zh) (void)hipHostFree(zh);
+        if (n2h) (void)hipHostFree(n2h);
     }
 
-    // out[i][k] = <Vb_i, Wv>_k for i < nvec, on the device
     // out (zeroed beforehand) += <Vb_i, Wv>_k for i < nvec, on the device
-    void dots_dev(const double2 *Vb, int nvec, const double2 *Wv, double2 *out) {
+    template <typename T>
+    void dots_dev(const T *Vb, int nvec, const T *Wv, double2 *out) {
         int bs = 256 - 256 % r;
-        hipLaunchKernelGGL(k_dots, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, ne, r, out);
+        hipLaunchKernelGGL(k_dots<T>, dim3(nb_dots, nvec), dim3(bs), bs * sizeof(double2), op.s, Vb, ne, Wv, ne, r,
+                           out);
     }
 
-    std::vector<double> norms(const double2 *Wv) {
+    // Squared column norms of an FP64 vector into n2d and its pinned copy; read_norms() after
+    // the stream has passed (so that several Krylov objects share one wait).
+    void enqueue_norms(const double2 *Wv) {
         TT_HIP(hipMemsetAsync(n2d.p, 0, sizeof(double2) * r, op.s));   // eager, never captured
         dots_dev(Wv, 1, Wv, n2d.p);
-        std::vector<double2> h(r);
-        TT_HIP(hipMemcpyAsync(h.data(), n2d.p, sizeof(double2) * r, hipMemcpyDeviceToHost, op.s));
-        TT_HIP(hipStreamSynchronize(op.s));
+        TT_HIP(hipMemcpyAsync(n2h, n2d.p, sizeof(double2) * r, hipMemcpyDeviceToHost, op.s));
+    }
+    std::vector<double> read_norms() const {
         std::vector<double> n(r);
-        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, h[k].x));
+        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, n2h[k].x));
         return n;
     }
-
-    // out = M(y) = y - (y .* G) A
-    // Timed through ev when given, else through the event pool when timed.
-    void apply_M(const double2 *y, double2 *out, bool fp32, const hipEvent_t *ev = nullptr, bool timed = true) {
-        op.apply(y, out, r, true, fp32, ev != nullptr || timed, y, ev);
+    std::vector<double> norms(const double2 *Wv) {
+        enqueue_norms(Wv);
+        TT_HIP(hipStreamSynchronize(op.s));
+        return read_norms();
     }
 
     // The work of Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
     // The product's launch time is sampled in the first step of a cycle only: timing events
     // are not free (a few microseconds of gap each).
-    void enqueue_step(int j, bool fp32, const hipEvent_t *ev, bool capturing = false) {
-        double2 *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
+    void enqueue_step(int j, const hipEvent_t *ev, bool capturing = false) {
+        VT *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
         op.zero_next = h1;   // h1, h2 and n2 accumulate by atomics: the step's first kernel zeroes them
         op.zero_next_n = slot_len;
-        // the new vector is formed in place, in column j + 1 of the basis
-        apply_M(vj, vj1, fp32, ev, !capturing && j == 0);
+        // the new vector w = M v_j is formed in place, in column j + 1 of the basis
+        op.apply(vj, vj1, r, true, fp32, ev != nullptr || (!capturing && j == 0), vj, ev);
         dots_dev(V.p, j + 1, vj1, h1);
-        hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h1, ne, r, -1.0);
+        hipLaunchKernelGGL((k_combo<VT, VT>), dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h1, ne, r,
+                           -1.0);
         dots_dev(V.p, j + 2, vj1, h2);   // h2[j + 1] = <w, w>
-        hipLaunchKernelGGL(k_combo_normalize, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h2, ne, r,
-                           n2);
+        hipLaunchKernelGGL(k_combo_normalize<VT>, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h2, ne,
+                           r, n2);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
     }
 
     // Launch Arnoldi step j (a replay of its graph, captured on first use) and record step_ev[j].
-    void step(int j, bool fp32) {
+    void step(int j) {
         if (!use_graphs) {
-            enqueue_step(j, fp32, nullptr);
+            enqueue_step(j, nullptr);
             TT_HIP(hipEventRecord(step_ev[j], op.s));
             return;
         }
-        StepGraph &g = graphs[fp32 ? 1 : 0][j];
+        StepGraph &gr = graphs[j];
         op.prepare(r, fp32);
-        if (!g.exec || g.gen != op.layout_gen) {
-            if (g.exec) TT_HIP(hipGraphExecDestroy(g.exec));
-            g.exec = nullptr;
-            for (auto &e : g.ev)
+        if (!gr.exec || gr.gen != op.layout_gen) {
+            if (gr.exec) TT_HIP(hipGraphExecDestroy(gr.exec));
+            gr.exec = nullptr;
+            for (auto &e : gr.ev)
                 if (!e) TT_HIP(hipEventCreate(&e));
             const int na = op.n_apply, na32 = op.n_apply32;  // capture runs nothing: count launches below
             TT_HIP(hipStreamBeginCapture(op.s, hipStreamCaptureModeThreadLocal));
@@ -1485,7 +1563,7 @@ struct Gmres {
             try {
                 // timing events (external event nodes, a few microseconds of gap each) only in
                 // the first step of a cycle: the product's launch time is sampled there
-                enqueue_step(j, fp32, j == 0 ? g.ev : nullptr, true);
+                enqueue_step(j, j == 0 ? gr.ev : nullptr, true);
             } catch (...) {
                 (void)hipStreamEndCapture(op.s, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1494,15 +1572,15 @@ struct Gmres {
             op.n_apply = na;
             op.n_apply32 = na32;
             TT_HIP(hipStreamEndCapture(op.s, &graph));
-            hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            hipError_t e = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) {
-                g.exec = nullptr;
+                gr.exec = nullptr;
                 fail(TT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
             }
-            g.gen = op.layout_gen;
+            gr.gen = op.layout_gen;
         }
-        TT_HIP(hipGraphLaunch(g.exec, op.s));
+        TT_HIP(hipGraphLaunch(gr.exec, op.s));
         TT_HIP(hipEventRecord(step_ev[j], op.s));
         ++op.n_apply;
         if (fp32) ++op.n_apply32;
@@ -1511,126 +1589,133 @@ struct Gmres {
     // Timing of a completed step (graph mode records the application of step 0 into the
     // graph's events as external event nodes); stops asking once the runtime reports no
     // timestamps for them.
-    bool graph_timing = true;
-    void read_step_timing(int j, bool fp32) {
+    void read_step_timing(int j) {
         if (use_graphs && graph_timing && j == 0)
-            graph_timing = op.add_timing(graphs[fp32 ? 1 : 0][j].ev, op.product_bytes(r, fp32), fp32);
+            graph_timing = op.add_timing(graphs[j].ev, op.product_bytes(r, fp32), fp32);
     }
 
-    // Solve M x = b (b device, ne); x device (in: initial guess, out: solution).  x_zero: the
-    // initial guess is zero (the first residual is b, no product).  verify: a cycle that
-    // converged by the GMRES residual estimate is confirmed by the true residual b - M x (one
-    // more product); without it the solve ends on the estimate (the inner solves of iterative
-    // refinement, whose caller forms the true residual in higher precision anyway).
-    // Returns iterations; res_out = final relative residual, max over systems (true residual
-    // with verify, else the estimate).
-    int solve(const double2 *b, double2 *x, double tol, int max_iter, bool fp32, double &res_out,
-              const std::vector<double> &bnorm, bool x_zero = false, bool verify = true) {
-        int it_total = 0;
-        std::vector<cdouble> H((size_t)(m + 1) * m * r), cs((size_t)m * r), sn((size_t)m * r), g((size_t)(m + 1) * r);
-        double res_max = 0.0;
-        bool first = true;
-        while (true) {
-            if (first && x_zero) {
-                TT_HIP(hipMemcpyAsync(R.p, b, sizeof(double2) * ne, hipMemcpyDeviceToDevice, op.s));
-            } else {
-                apply_M(x, R.p, fp32);  // R = M x, then R = b - M x
-                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, R.p, b, R.p, 1.0, -1.0, ne);
+    // Starts a cycle on the residual R (FP64, device) whose column norms beta are in n2d
+    // (enqueue_norms(R) ran last on this object): v_0 = R / beta.  Residual estimates are
+    // relative to ref; the cycle stops when all are <= tol (with a margin of 2) or after
+    // jmax_ steps.
+    void begin_cycle(const double2 *R, const std::vector<double> &beta, const std::vector<double> &ref_, double tol_,
+                     int jmax_) {
+        hipLaunchKernelGGL(k_scale_by_norm<VT>, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R, n2d.p, ne, r);
+        ref = ref_;
+        tol = tol_;
+        jmax = std::min(jmax_, m);
+        std::fill(g.begin(), g.end(), 0.0);
+        for (int k = 0; k < r; ++k) {
+            g[k] = beta[k];
+            prev_rel[k] = beta[k] / std::max(ref[k], 1e-300);
+        }
+        used = launched = 0;
+        hold = false;
+        conv = false;
+        in_cycle = jmax > 0;
+    }
+
+    // Enqueue step j if not yet launched, and step j + 1 ahead unless held back.
+    void advance(int j) {
+        if (launched <= j) step(launched++);
+        if (!hold && launched == j + 1 && launched < jmax) step(launched++);
+    }
+
+    // Hessenberg column j (the step has completed): Givens rotations and residual estimates.
+    void take_column(int j) {
+        read_step_timing(j);
+        const double2 *hj = Hh + (size_t)j * slot_len;
+        bool conv_all = true, hold_next = true;
+        est = 0.0;
+        for (int k = 0; k < r; ++k) {
+            auto Hk = [&](int i, int jj) -> cdouble & { return H[((size_t)i * m + jj) * r + k]; };
+            for (int i = 0; i <= j; ++i) {
+                const double2 a1 = hj[(size_t)i * r + k], a2 = hj[(size_t)(m + 1 + i) * r + k];
+                Hk(i, j) = cdouble(a1.x + a2.x, a1.y + a2.y);
             }
-            first = false;
-            std::vector<double> beta = norms(R.p);
-            res_max = 0.0;
-            bool all = true;
-            for (int k = 0; k < r; ++k) {
-                double rel = beta[k] / std::max(bnorm[k], 1e-300);
-                res_max = std::max(res_max, rel);
-                if (rel > tol) all = false;
+            Hk(j + 1, j) = std::sqrt(std::max(0.0, hj[(size_t)2 * (m + 1) * r + k].x));
+            for (int i = 0; i < j; ++i) {
+                cdouble c = cs[(size_t)i * r + k], sv = sn[(size_t)i * r + k];
+                cdouble t0 = std::conj(c) * Hk(i, j) + std::conj(sv) * Hk(i + 1, j);
+                cdouble t1 = -sv * Hk(i, j) + c * Hk(i + 1, j);
+                Hk(i, j) = t0;
+                Hk(i + 1, j) = t1;
             }
-            if (all || it_total >= max_iter) break;
-            hipLaunchKernelGGL(k_scale_by_norm, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R.p, n2d.p, ne, r);
-            std::fill(g.begin(), g.end(), 0.0);
-            for (int k = 0; k < r; ++k) g[k] = beta[k];
-            int used = 0;
-            bool converged = false;
-            const int jmax = std::min(m, max_iter - it_total);
-            // The device runs one step ahead of the host's Givens rotations, except where the
-            // residual history predicts that the step being waited for converges: that
-            // speculative step would be wasted, a wrong guess costs one host round trip.
-            int launched = 0;
-            bool hold = false;
-            std::vector<double> prev_rel(r);
-            for (int k = 0; k < r; ++k) prev_rel[k] = beta[k] / std::max(bnorm[k], 1e-300);
-            for (int j = 0; j < jmax; ++j) {
-                if (launched <= j) step(launched++, fp32);
-                if (!hold && launched == j + 1 && launched < jmax) step(launched++, fp32);
-                TT_HIP(hipEventSynchronize(step_ev[j]));
-                read_step_timing(j, fp32);
-                const double2 *hj = Hh + (size_t)j * slot_len;
-                bool conv_all = true, hold_next = true;
-                for (int k = 0; k < r; ++k) {
-                    auto Hk = [&](int i, int jj) -> cdouble & { return H[((size_t)i * m + jj) * r + k]; };
-                    for (int i = 0; i <= j; ++i) {
-                        const double2 a1 = hj[(size_t)i * r + k], a2 = hj[(size_t)(m + 1 + i) * r + k];
-                        Hk(i, j) = cdouble(a1.x + a2.x, a1.y + a2.y);
-                    }
-                    Hk(j + 1, j) = std::sqrt(std::max(0.0, hj[(size_t)2 * (m + 1) * r + k].x));
-                    for (int i = 0; i < j; ++i) {
-                        cdouble c = cs[(size_t)i * r + k], sv = sn[(size_t)i * r + k];
-                        cdouble t0 = std::conj(c) * Hk(i, j) + std::conj(sv) * Hk(i + 1, j);
-                        cdouble t1 = -sv * Hk(i, j) + c * Hk(i + 1, j);
-                        Hk(i, j) = t0;
-                        Hk(i + 1, j) = t1;
-                    }
-                    cdouble a = Hk(j, j), bb = Hk(j + 1, j);
-                    double den = std::sqrt(std::norm(a) + std::norm(bb));
-                    cdouble c = den > 0 ? a / den : 1.0, sv = den > 0 ? bb / den : 0.0;
-                    cs[(size_t)j * r + k] = c;
-                    sn[(size_t)j * r + k] = sv;
-                    Hk(j, j) = den;
-                    Hk(j + 1, j) = 0.0;
-                    cdouble gj = g[(size_t)j * r + k];
-                    g[(size_t)j * r + k] = std::conj(c) * gj;
-                    g[(size_t)(j + 1) * r + k] = -sv * gj;
-                    double rel = std::abs(g[(size_t)(j + 1) * r + k]) / std::max(bnorm[k], 1e-300);
-                    if (rel > 0.5 * tol) conv_all = false;
-                    // geometric extrapolation of this system's residual to the next step
-                    const double pred = prev_rel[k] > 0.0 ? rel * std::min(1.0, rel / prev_rel[k]) : 0.0;
-                    if (pred > 0.5 * tol) hold_next = false;
-                    prev_rel[k] = rel;
-                }
-                used = j + 1;
-                ++it_total;
-                if (conv_all) {
-                    converged = true;
-                    break;
-                }
-                hold = hold_next;
-            }
-            // x += V z from the triangular least-squares solution
-            std::vector<cdouble> z((size_t)(m + 1) * r, 0.0);
-            for (int k = 0; k < r; ++k) {
-                auto Hk = [&](int i, int jj) { return H[((size_t)i * m + jj) * r + k]; };
-                for (int i = used - 1; i >= 0; --i) {
-                    cdouble sacc = g[(size_t)i * r + k];
-                    for (int l = i + 1; l < used; ++l) sacc -= Hk(i, l) * z[(size_t)l * r + k];
-                    z[(size_t)i * r + k] = Hk(i, i) != cdouble(0.0) ? sacc / Hk(i, i) : 0.0;
-                }
-            }
-            TT_HIP(hipStreamSynchronize(op.s));  // speculative steps may still use zdev's neighbours
-            for (int j = used; j < launched; ++j) read_step_timing(j, fp32);
-            zdev.upload(reinterpret_cast<const double2 *>(z.data()), z.size(), op.s);
-            hipLaunchKernelGGL(k_combo, dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r, 1.0);
-            if (!verify && (converged || it_total >= max_iter)) {
-                res_max = 0.0;
-                for (int k = 0; k < r; ++k)
-                    res_max = std::max(res_max, std::abs(g[(size_t)used * r + k]) / std::max(bnorm[k], 1e-300));
-                break;
+            cdouble a = Hk(j, j), bb = Hk(j + 1, j);
+            double den = std::sqrt(std::norm(a) + std::norm(bb));
+            cdouble c = den > 0 ? a / den : 1.0, sv = den > 0 ? bb / den : 0.0;
+            cs[(size_t)j * r + k] = c;
+            sn[(size_t)j * r + k] = sv;
+            Hk(j, j) = den;
+            Hk(j + 1, j) = 0.0;
+            cdouble gj = g[(size_t)j * r + k];
+            g[(size_t)j * r + k] = std::conj(c) * gj;
+            g[(size_t)(j + 1) * r + k] = -sv * gj;
+            double rel = std::abs(g[(size_t)(j + 1) * r + k]) / std::max(ref[k], 1e-300);
+            est = std::max(est, rel);
+            if (rel > 0.5 * tol) conv_all = false;
+            // geometric extrapolation of this system's residual to the next step
+            const double pred = prev_rel[k] > 0.0 ? rel * std::min(1.0, rel / prev_rel[k]) : 0.0;
+            if (pred > 0.5 * tol) hold_next = false;
+            prev_rel[k] = rel;
+        }
+        used = j + 1;
+        hold = hold_next;
+        if (conv_all) conv = true;
+        if (conv_all || used >= jmax) in_cycle = false;
+    }
+
+    // x += V z from the triangular least-squares solution of the cycle (enqueued).  Steps
+    // launched ahead past the end of the cycle write basis vectors this does not read.
+    template <typename TX>
+    void end_cycle(TX *x) {
+        for (int k = 0; k < r; ++k) {
+            auto Hk = [&](int i, int jj) { return H[((size_t)i * m + jj) * r + k]; };
+            for (int i = used - 1; i >= 0; --i) {
+                cdouble sacc = g[(size_t)i * r + k];
+                for (int l = i + 1; l < used; ++l)
+                    sacc -= Hk(i, l) * cdouble(zh[(size_t)l * r + k].x, zh[(size_t)l * r + k].y);
+                const cdouble zi = Hk(i, i) != cdouble(0.0) ? sacc / Hk(i, i) : 0.0;
+                zh[(size_t)i * r + k] = make_double2(zi.real(), zi.imag());
             }
         }
-        res_out = res_max;
-        return it_total;
+        in_cycle = false;
+        if (used == 0) return;
+        // zh is rewritten by the next cycle only after that cycle's steps, which follow this copy
+        TT_HIP(hipMemcpyAsync(zdev.p, zh, sizeof(double2) * (size_t)used * r, hipMemcpyHostToDevice, op.s));
+        hipLaunchKernelGGL((k_combo<TX, VT>), dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r,
+                           1.0);
     }
 };
+
+// One lockstep cycle over the Krylov objects ks (each begun with begin_cycle): step j of every
+// object still in its cycle is enqueued (with one step of lookahead), one wait covers them all,
+// then the host takes each column; an object whose cycle ends adds its update to its x
+// (xs[i]) at once.  Returns when every cycle has ended.
+template <typename VT, typename TX>
+static void lockstep_cycle(const std::vector<Krylov<VT> *> &ks, const std::vector<TX *> &xs) {
+    int jmax = 0;
+    for (auto *k : ks) jmax = std::max(jmax, k->in_cycle ? k->jmax : 0);
+    for (int j = 0; j < jmax; ++j) {
+        bool any = false;
+        for (auto *k : ks)
+            if (k->in_cycle) {
+                k->advance(j);
+                any = true;
+            }
+        if (!any) break;
+        for (auto *k : ks)
+            if (k->in_cycle) TT_HIP(hipEventSynchronize(k->step_ev[j]));
+        for (size_t i = 0; i < ks.size(); ++i) {
+            Krylov<VT> *k = ks[i];
+            if (!k->in_cycle) continue;
+            k->take_column(j);
+            if (!k->in_cycle) k->end_cycle(xs[i]);
+        }
+    }
+    for (size_t i = 0; i < ks.size(); ++i)
+        if (ks[i]->in_cycle) ks[i]->end_cycle(xs[i]);
+}
 
 // ------------------------------------------------------------------ solve driver
 struct OnshellLayout {
@@ -1677,13 +1762,22 @@ static void gather_onshell(RowOperator &op, const double2 *rows, const OnshellLa
     for (int i = 0; i < n; ++i) out[i] = cdouble(h[i].x, h[i].y);
 }
 
-static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 *Bout) {
-    DevBuf<double2> E((size_t)op.dim * L.r);
-    DevBuf<int64_t> di;
-    TT_HIP(hipMemsetAsync(E.p, 0, sizeof(double2) * E.n, op.s));
-    di.upload(L.row_idx.data(), L.r, op.s);
-    hipLaunchKernelGGL(k_unit_rows, dim3(nblk(L.r, 64)), dim3(64), 0, op.s, E.p, di.p, L.r);
-    op.apply(E.p, Bout, L.r, false, false);
+// Bout = the on-shell rows e_k^T A (k = (d_row, j)); with scratch (dim * r elements) and idx the
+// unit rows are formed there instead of in buffers of their own.
+static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 *Bout, double2 *scratch = nullptr,
+                               DevBuf<int64_t> *idx = nullptr) {
+    DevBuf<double2> own;
+    DevBuf<int64_t> own_idx;
+    if (!scratch) {
+        own.alloc((size_t)op.dim * L.r);
+        scratch = own.p;
+    }
+    if (!idx) idx = &own_idx;
+    TT_HIP(hipMemsetAsync(scratch, 0, sizeof(double2) * (size_t)op.dim * L.r, op.s));
+    idx->upload(L.row_idx.data(), L.r, op.s);
+    hipLaunchKernelGGL(k_unit_rows, dim3(nblk(L.r, 64)), dim3(64), 0, op.s, scratch, idx->p, L.r);
+    op.apply(scratch, Bout, L.r, false, false);
+    if (own.p) TT_HIP(hipStreamSynchronize(op.s));   // own buffers are freed on return
 }
 
 static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
@@ -1748,16 +1842,20 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
 
 // G[j][idx] at the total energies E[0..nE) (host), written to device memory dG (nE*dim);
 // with partials, the three partial-derivative arrays of k_resolvent_partials (3*nE*dim).
+// With stage the inputs are uploaded into the caller's buffers, which outlive the kernel (no
+// wait); without it into buffers of this call, and the call waits for the kernel.
 static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt_swp_states &swp,
                                 const tt_pw_states &chn, const double *E, int nE, double2 *dG,
-                                bool partials = false) {
+                                bool partials = false, ResolventStaging *stage = nullptr) {
     std::vector<int> which, offset;
     swp_edge_offsets(chn, swp, rp, which, offset);
     const int Na = chn.Nalpha, Nq = swp.Nq_WP, Np = swp.Np_WP;
     const int64_t dim = (int64_t)Na * Nq * Np;
     const size_t nu = (size_t)swp.num_2N_unco_states * (Np + 1), nc = (size_t)swp.num_2N_coup_states * 2 * (Np + 1);
-    DevBuf<double> eu, ec, qe, dE;
-    DevBuf<int> dw, doff;
+    ResolventStaging own;
+    ResolventStaging &st = stage ? *stage : own;
+    DevBuf<double> &eu = st.eu, &ec = st.ec, &qe = st.qe, &dE = st.E;
+    DevBuf<int> &dw = st.which, &doff = st.offset;
     eu.upload(swp.e_SWP_unco, std::max<size_t>(nu, 1), s);
     if (nc) ec.upload(swp.e_SWP_coup, nc, s);
     qe.upload(swp.q_WP, Nq + 1, s);
@@ -1771,8 +1869,8 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
         hipLaunchKernelGGL(k_resolvent, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p, doff.p,
                            qe.p, dE.p, Na, Nq, Np, nE);
     TT_HIP(hipGetLastError());
-    // the staging buffers are freed on return: let the kernel finish with them first
-    TT_HIP(hipStreamSynchronize(s));
+    // the call's own staging buffers are freed on return: let the kernel finish with them first
+    if (!stage) TT_HIP(hipStreamSynchronize(s));
 }
 
 // ------------------------------------------------------------------ solve plans
@@ -1786,22 +1884,77 @@ struct SolvePlan {
     bool col = false;   // column operator (adjoint solves)
     uint64_t last_use = 0;
     RowOperator op;
-    std::unique_ptr<Gmres> gm;
-    DevBuf<double2> b, x, rres, d;
+    std::unique_ptr<Krylov<double2>> k64;   // FP64 GMRES
+    std::unique_ptr<Krylov<float2>> k32;    // inner solves of iterative refinement
+    DevBuf<double2> b, x, rres;
+    // on-shell index lists and the gathered U elements (pinned host copy)
+    DevBuf<int64_t> unit_idx, os_idx;
+    DevBuf<int> os_k;
+    DevBuf<double2> os_out;
+    double2 *os_out_h = nullptr;
+    size_t os_out_cap = 0;
+    uint64_t in_use = 0;   // mark of the lockstep solve holding this plan
 
-    Gmres &gmres(int r, int m, bool graphs) {
-        if (!gm || gm->r != r || gm->m != m || gm->use_graphs != graphs) {
-            gm.reset();
-            gm.reset(new Gmres(op, r, m, graphs));
+    ~SolvePlan() {
+        if (os_out_h) (void)hipHostFree(os_out_h);
+    }
+
+    template <typename VT>
+    Krylov<VT> &krylov(int r, int m, bool graphs) {
+        std::unique_ptr<Krylov<VT>> *k;
+        if constexpr (std::is_same<VT, double2>::value)
+            k = &k64;
+        else
+            k = &k32;
+        if (!*k || (*k)->r != r || (*k)->m != m || (*k)->use_graphs != graphs) {
+            k->reset();
+            k->reset(new Krylov<VT>(op, r, m, graphs));
         }
-        return *gm;
+        return **k;
+    }
+
+    // b, x and the residual vector hold dim * r elements
+    void fit(int r) {
+        const size_t ne = (size_t)op.dim * r;
+        for (DevBuf<double2> *v : {&b, &x, &rres})
+            if (v->n < ne) v->alloc(ne);
+    }
+
+    // Enqueue the on-shell elements of x (rows (d_row, j), columns (alpha_dc, q_j, 0)) into
+    // pinned host memory, read by read_onshell once the stream has passed:
+    // out[dr][dc][j] = x[(alpha_dc, q_j, 0)][dr * nE + j].
+    void enqueue_onshell(const double2 *xv, const OnshellLayout &L, const tt_channel_os &os) {
+        std::vector<int64_t> idx;
+        std::vector<int> kk;
+        for (int dr = 0; dr < L.nd; ++dr)
+            for (int dc = 0; dc < L.nd; ++dc)
+                for (int j = 0; j < L.nE; ++j) {
+                    idx.push_back(((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np);
+                    kk.push_back(dr * L.nE + j);
+                }
+        const int n = (int)idx.size();
+        if ((size_t)n > os_out_cap) {
+            if (os_out_h) TT_HIP(hipHostFree(os_out_h));
+            os_out_h = nullptr;
+            TT_HIP(hipHostMalloc(&os_out_h, sizeof(double2) * n, hipHostMallocDefault));
+            os_out_cap = n;
+            os_out.alloc(n);
+        }
+        os_idx.upload(idx.data(), n, op.s);
+        os_k.upload(kk.data(), n, op.s);
+        hipLaunchKernelGGL(k_gather, dim3(nblk(n, 64)), dim3(64), 0, op.s, xv, os_idx.p, os_k.p, os_out.p, n, L.r);
+        TT_HIP(hipMemcpyAsync(os_out_h, os_out.p, sizeof(double2) * n, hipMemcpyDeviceToHost, op.s));
+    }
+    void read_onshell(const OnshellLayout &L, cdouble *U) const {
+        const int n = L.nd * L.nd * L.nE;
+        for (int i = 0; i < n; ++i) U[i] = cdouble(os_out_h[i].x, os_out_h[i].y);
     }
 };
 
 struct PlanCache {
     std::vector<std::unique_ptr<SolvePlan>> plans;
     uint64_t tick = 0;
-    static constexpr size_t max_plans = 32;
+    static constexpr size_t max_plans = 64;
 };
 
 void free_plan_cache(PlanCache *c) { delete c; }
@@ -1865,12 +2018,14 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
             return *pl;
         }
     if (c.plans.size() >= PlanCache::max_plans) {
-        auto lru = std::min_element(c.plans.begin(), c.plans.end(),
-                                    [](const std::unique_ptr<SolvePlan> &a, const std::unique_ptr<SolvePlan> &b) {
-                                        return a->last_use < b->last_use;
-                                    });
-        TT_HIP(hipStreamSynchronize(ctx->stream));
-        c.plans.erase(lru);
+        // the least recently used plan that no running lockstep group holds
+        auto lru = c.plans.end();
+        for (auto it = c.plans.begin(); it != c.plans.end(); ++it)
+            if (!(*it)->in_use && (lru == c.plans.end() || (*it)->last_use < (*lru)->last_use)) lru = it;
+        if (lru != c.plans.end()) {
+            TT_HIP(hipStreamSynchronize(ctx->stream));
+            c.plans.erase(lru);
+        }
     }
     std::unique_ptr<SolvePlan> pl(new SolvePlan);
     pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format, col);
@@ -1884,57 +2039,127 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
     return *c.plans.back();
 }
 
-static void solve_krylov(SolvePlan &pl, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
-                         tt_solve_stats &st, bool mixed) {
-    RowOperator &op = pl.op;
-    OnshellLayout L = onshell_layout(os, op.Nq, op.Np);
-    op.set_gidx(L.row_g);
-    const int64_t ne = op.dim * L.r;
-    auto fit = [&](DevBuf<double2> &v) {
-        if (v.n < (size_t)ne) v.alloc(ne);
-    };
-    fit(pl.b);
-    fit(pl.x);
-    on_shell_rows_of_A(op, L, pl.b.p);
-    const int m = o.restart > 0 ? o.restart : 40;
-    Gmres &gm = pl.gmres(L.r, m, o.use_graphs != 0);
-    std::vector<double> bnorm = gm.norms(pl.b.p);
-    TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * ne, op.s));
+// One channel of a lockstep Krylov solve: its plan, on-shell layout and right-hand sides b
+// (the on-shell rows of its operator), solution x and residual in the plan's buffers.
+struct KMember {
+    SolvePlan *pl = nullptr;
+    const tt_channel_os *os = nullptr;
+    OnshellLayout L;
+    std::vector<double> bnorm;
     double res = 0.0;
     int iters = 0;
-    if (!mixed) {
-        iters = gm.solve(pl.b.p, pl.x.p, o.tol, o.max_iter, false, res, bnorm, true);
-    } else {
-        // iterative refinement: FP64 residual, FP32-product GMRES corrections
-        fit(pl.rres);
-        fit(pl.d);
-        for (int outer = 0; outer < 20; ++outer) {
-            if (outer == 0) {   // x = 0: the residual is b
-                TT_HIP(hipMemcpyAsync(pl.rres.p, pl.b.p, sizeof(double2) * ne, hipMemcpyDeviceToDevice, op.s));
-            } else {
-                gm.apply_M(pl.x.p, pl.rres.p, false);
-                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.rres.p, pl.b.p, pl.rres.p,
-                                   1.0, -1.0, ne);
+    bool done = false;
+};
+
+// GMRES (FP64) or iterative refinement (FP64 residual, FP32 inner GMRES) of every member, all
+// members advancing together on the one stream of their plans: every host wait (residual
+// norms, Arnoldi steps) serves all members.  G must be in each operator; the solutions stay in
+// the plans' x.
+static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, bool mixed) {
+    if (ms.empty()) return;
+    const int m = o.restart > 0 ? o.restart : 40;
+    const bool graphs = o.use_graphs != 0;
+    const size_t n = ms.size();
+    hipStream_t s = ms[0].pl->op.s;
+    std::vector<Krylov<double2> *> k64(n, nullptr);
+    std::vector<Krylov<float2> *> k32(n, nullptr);
+    for (size_t i = 0; i < n; ++i) {
+        KMember &mb = ms[i];
+        SolvePlan &pl = *mb.pl;
+        TT_REQUIRE(pl.op.s == s, "lockstep members share one stream");
+        mb.L = onshell_layout(*mb.os, pl.op.Nq, pl.op.Np);
+        const int r = mb.L.r;
+        pl.op.set_gidx(mb.L.row_g);
+        pl.fit(r);
+        on_shell_rows_of_A(pl.op, mb.L, pl.b.p, pl.rres.p, &pl.unit_idx);
+        TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * (size_t)pl.op.dim * r, s));
+        if (mixed) {
+            k32[i] = &pl.krylov<float2>(r, m, graphs);
+            k32[i]->enqueue_norms(pl.b.p);
+        } else {
+            k64[i] = &pl.krylov<double2>(r, m, graphs);
+            k64[i]->enqueue_norms(pl.b.p);
+        }
+        mb.res = 1.0;
+        mb.iters = 0;
+        mb.done = false;
+    }
+    TT_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) ms[i].bnorm = mixed ? k32[i]->read_norms() : k64[i]->read_norms();
+    auto rel_max = [](const std::vector<double> &a, const std::vector<double> &ref) {
+        double v = 0.0;
+        for (size_t k = 0; k < a.size(); ++k) v = std::max(v, a[k] / std::max(ref[k], 1e-300));
+        return v;
+    };
+    // R = b - M x for the members still solving, with the column norms of R enqueued
+    auto residuals = [&](const std::vector<size_t> &act) {
+        for (size_t i : act) {
+            SolvePlan &pl = *ms[i].pl;
+            const int64_t ne = pl.op.dim * ms[i].L.r;
+            pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);   // M x
+            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, s, pl.rres.p, pl.b.p, pl.rres.p, 1.0, -1.0,
+                               ne);
+            if (mixed)
+                k32[i]->enqueue_norms(pl.rres.p);
+            else
+                k64[i]->enqueue_norms(pl.rres.p);
+        }
+    };
+    for (int outer = 0;; ++outer) {
+        // members still solving; the first pass takes R = b (x = 0), whose norms are in hand
+        std::vector<size_t> act;
+        for (size_t i = 0; i < n; ++i)
+            if (!ms[i].done) act.push_back(i);
+        if (act.empty()) break;
+        std::vector<std::vector<double>> beta(n);
+        if (outer == 0) {
+            for (size_t i : act) beta[i] = ms[i].bnorm;
+        } else {
+            residuals(act);
+            TT_HIP(hipStreamSynchronize(s));
+            for (size_t i : act) beta[i] = mixed ? k32[i]->read_norms() : k64[i]->read_norms();
+        }
+        std::vector<size_t> go;
+        for (size_t i : act) {
+            KMember &mb = ms[i];
+            mb.res = rel_max(beta[i], mb.bnorm);
+            if (mb.res <= o.tol || mb.iters >= o.max_iter || (mixed && outer >= 20))
+                mb.done = true;
+            else
+                go.push_back(i);
+        }
+        if (go.empty()) break;
+        const double2 *R0 = nullptr;
+        if (!mixed) {
+            std::vector<Krylov<double2> *> ks;
+            std::vector<double2 *> xs;
+            for (size_t i : go) {
+                SolvePlan &pl = *ms[i].pl;
+                R0 = outer == 0 ? pl.b.p : pl.rres.p;
+                k64[i]->begin_cycle(R0, beta[i], ms[i].bnorm, o.tol, o.max_iter - ms[i].iters);
+                ks.push_back(k64[i]);
+                xs.push_back(pl.x.p);
             }
-            std::vector<double> rn = gm.norms(pl.rres.p);
-            res = 0.0;
-            for (int k = 0; k < L.r; ++k) res = std::max(res, rn[k] / std::max(bnorm[k], 1e-300));
-            if (res <= o.tol || iters >= o.max_iter) break;
-            TT_HIP(hipMemsetAsync(pl.d.p, 0, sizeof(double2) * ne, op.s));   // the solve adds its update to d
-            double inner_res = 0.0;
-            // inner tolerance relative to this residual
-            std::vector<double> rnorm = rn;
-            double inner_tol = std::max(1e-6, o.tol / std::max(res, 1e-300));
-            inner_tol = std::min(inner_tol, 1e-4);
-            iters += gm.solve(pl.rres.p, pl.d.p, inner_tol, o.max_iter - iters, true, inner_res, rnorm, true, false);
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, op.s, pl.x.p, pl.x.p, pl.d.p, 1.0, 1.0, ne);
+            lockstep_cycle(ks, xs);
+            for (size_t k = 0; k < go.size(); ++k) ms[go[k]].iters += ks[k]->used;
+        } else {
+            // inner FP32 GMRES on the residual, to a tolerance relative to it
+            std::vector<Krylov<float2> *> ks;
+            std::vector<double2 *> xs;
+            for (size_t i : go) {
+                SolvePlan &pl = *ms[i].pl;
+                R0 = outer == 0 ? pl.b.p : pl.rres.p;
+                const double inner_tol = std::min(1e-4, std::max(1e-6, o.tol / std::max(ms[i].res, 1e-300)));
+                k32[i]->begin_cycle(R0, beta[i], beta[i], inner_tol, o.max_iter - ms[i].iters);
+                ks.push_back(k32[i]);
+                xs.push_back(pl.x.p);
+            }
+            lockstep_cycle(ks, xs);
+            for (size_t k = 0; k < go.size(); ++k) ms[go[k]].iters += ks[k]->used;
         }
     }
-    std::vector<cdouble> out;
-    gather_onshell(op, pl.x.p, L, os, out);
-    for (size_t e = 0; e < out.size(); ++e) U[e] = out[e];
-    st.iterations = iters;
-    st.residual = res;
+    for (auto &mb : ms) mb.pl->enqueue_onshell(mb.pl->x.p, mb.L, *mb.os);
+    TT_HIP(hipStreamSynchronize(s));
 }
 
 // Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
@@ -2153,21 +2378,24 @@ tt_status tt_calculate_resolvent(tt_ctx *ctx, const tt_run_params *rp, const tt_
 }  // extern "C"
 
 namespace tt {
-// One channel's on-shell solve (the body of tt_solve_faddeev_equations; throws on error).
-static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G, const tt_csc *P123,
-                          const double *V_unco, const double *V_coup, const tt_swp_states *swp,
-                          const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o,
-                          tt_solve_stats *stats) {
-    TT_REQUIRE(ctx && rp && U && P123 && V_unco && swp && os && chn, "null argument");
+// The plan of one channel's solve (checked arguments).
+static SolvePlan &find_plan(tt_ctx *ctx, const tt_run_params *rp, const double *G, const tt_csc *P123,
+                            const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                            const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o) {
+    TT_REQUIRE(ctx && rp && P123 && V_unco && swp && os && chn, "null argument");
     TT_REQUIRE(G || o.resolvent_on_device, "G required unless opts->resolvent_on_device is set");
     TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
     TT_HIP(hipSetDevice(ctx->device));
-    auto t0 = std::chrono::steady_clock::now();
-    tt_solve_stats st;
-    std::memset(&st, 0, sizeof(st));
     const DeviceCSC *P = dev_of(P123);
     TT_REQUIRE(P->dim == (int64_t)chn->Nalpha * swp->Nq_WP * swp->Np_WP, "P123 dimension does not match Nalpha*Nq*Np");
-    SolvePlan &pl = get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format);
+    return get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format);
+}
+
+// G of the solve in place in the plan (computed on the device or uploaded) and the plan's
+// statistics counters cleared.
+static void load_plan(tt_ctx *ctx, SolvePlan &pl, const tt_run_params *rp, const double *G,
+                      const tt_swp_states *swp, const tt_channel_os *os, const tt_pw_states *chn,
+                      const tt_solve_opts &o) {
     RowOperator &op = pl.op;
     op.collect();
     op.ms_apply = op.ms_spmm = op.spmm_bytes = op.ms_spmm32 = op.spmm_bytes32 = 0.0;
@@ -2177,30 +2405,26 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
         TT_REQUIRE(os->E_com, "resolvent_on_device needs the on-shell energies (os->E_com)");
         std::vector<double> E(os->num_T_lab);
         for (int j = 0; j < os->num_T_lab; ++j) E[j] = os->E_com[j] + swp->E_bound;
-        resolvent_on_device(ctx->stream, *rp, *swp, *chn, E.data(), os->num_T_lab, op.G_buffer(os->num_T_lab));
+        resolvent_on_device(ctx->stream, *rp, *swp, *chn, E.data(), os->num_T_lab, op.G_buffer(os->num_T_lab), false,
+                            &op.rstage);
     } else {
         op.set_G(G, os->num_T_lab);
     }
-    std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
-    switch (o.method) {
-        case TT_SOLVE_PADE:
-            solve_pade(op, *os, o, Uc.data(), st);
-            break;
-        case TT_SOLVE_DENSE:
-            solve_dense(op, *os, Uc.data(), st);
-            break;
-        case TT_SOLVE_GMRES:
-            solve_krylov(pl, *os, o, Uc.data(), st, false);
-            break;
-        case TT_SOLVE_MIXED:
-            solve_krylov(pl, *os, o, Uc.data(), st, true);
-            break;
-        default:
-            fail(TT_ERR_ARG, "unknown solve method");
-    }
-    TT_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// find_plan + load_plan: the set-up shared by the single and the batched solves.
+static SolvePlan &channel_plan(tt_ctx *ctx, const tt_run_params *rp, const double *G, const tt_csc *P123,
+                               const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                               const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o) {
+    SolvePlan &pl = find_plan(ctx, rp, G, P123, V_unco, V_coup, swp, os, chn, o);
+    load_plan(ctx, pl, rp, G, swp, os, chn, o);
+    return pl;
+}
+
+// Statistics of the plan's operator since channel_plan (the stream has passed).
+static void channel_stats(SolvePlan &pl, tt_solve_stats &st) {
+    RowOperator &op = pl.op;
     op.collect();
-    std::memcpy(U, Uc.data(), sizeof(cdouble) * Uc.size());
     st.n_apply = op.n_apply;
     st.n_apply_fp32 = op.n_apply32;
     st.ms_apply = op.ms_apply;
@@ -2210,12 +2434,146 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
     st.n_spmm_fp32 = op.n_spmm32;
     st.ms_spmm_fp32 = op.ms_spmm32;
     st.spmm_bytes_fp32 = op.spmm_bytes32;
-    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (stats) *stats = st;
-    if ((o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED) && !(st.residual <= o.tol))
-        fail(TT_ERR_NOT_CONVERGED, "Krylov solve stopped at relative residual " + std::to_string(st.residual) +
+}
+
+static void require_converged(const tt_solve_opts &o, double res) {
+    if ((o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED) && !(res <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "Krylov solve stopped at relative residual " + std::to_string(res) +
                                        " above the requested tolerance (U holds the last iterate)");
 }
+
+// One channel's on-shell solve (the body of tt_solve_faddeev_equations; throws on error).
+static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const double *G, const tt_csc *P123,
+                          const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                          const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o,
+                          tt_solve_stats *stats) {
+    TT_REQUIRE(U, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    tt_solve_stats st;
+    std::memset(&st, 0, sizeof(st));
+    SolvePlan &pl = channel_plan(ctx, rp, G, P123, V_unco, V_coup, swp, os, chn, o);
+    RowOperator &op = pl.op;
+    std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
+    switch (o.method) {
+        case TT_SOLVE_PADE:
+            solve_pade(op, *os, o, Uc.data(), st);
+            break;
+        case TT_SOLVE_DENSE:
+            solve_dense(op, *os, Uc.data(), st);
+            break;
+        case TT_SOLVE_GMRES:
+        case TT_SOLVE_MIXED: {
+            std::vector<KMember> ms(1);
+            ms[0].pl = &pl;
+            ms[0].os = os;
+            solve_lockstep(ms, o, o.method == TT_SOLVE_MIXED);
+            pl.read_onshell(ms[0].L, Uc.data());
+            st.iterations = ms[0].iters;
+            st.residual = ms[0].res;
+            break;
+        }
+        default:
+            fail(TT_ERR_ARG, "unknown solve method");
+    }
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(U, Uc.data(), sizeof(cdouble) * Uc.size());
+    channel_stats(pl, st);
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    require_converged(o, st.residual);
+}
+// ------------------------------------------------------------------ batch
+// Every item is set up (plan, G on the device); the Krylov methods then solve all items'
+// channels in lockstep (solve_lockstep: one stream, every wait shared), at most
+// PlanCache::max_plans channels at a time; a plan in use by the running group is never evicted.
+// Pade and dense solves go item by item.  Every item is attempted and gets its own
+// status; the first failure is raised at the end.
+static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_unco, const double *V_coup,
+                        const tt_swp_states *swp, tt_batch_item *items, int n_items, const tt_solve_opts &o) {
+    tt_status first = TT_OK;
+    std::string first_msg;
+    auto record = [&](int i, tt_status code, const std::string &msg) {
+        items[i].status = code;
+        if (code != TT_OK && first == TT_OK) {
+            first = code;
+            first_msg = "batch item " + std::to_string(i) + ": " + msg;
+        }
+    };
+    for (int i = 0; i < n_items; ++i) items[i].status = TT_OK;
+    const bool krylov = o.method == TT_SOLVE_GMRES || o.method == TT_SOLVE_MIXED;
+    if (!krylov) {
+        for (int i = 0; i < n_items; ++i) {
+            tt_batch_item &it = items[i];
+            try {
+                solve_channel(ctx, rp, it.U, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o, it.stats);
+            } catch (const Error &e) {
+                record(i, e.code, e.msg);
+            }
+        }
+        if (first != TT_OK) fail(first, first_msg);
+        return;
+    }
+    const int group_max = (int)PlanCache::max_plans;
+    int i0 = 0;
+    while (i0 < n_items) {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<KMember> ms;
+        std::vector<int> idx;
+        struct Release {   // plans of the group may be evicted again afterwards
+            std::vector<KMember> *m;
+            ~Release() {
+                for (auto &mb : *m) mb.pl->in_use = 0;
+            }
+        } release{&ms};
+        int i = i0;
+        for (; i < n_items && (int)ms.size() < group_max; ++i) {
+            tt_batch_item &it = items[i];
+            try {
+                TT_REQUIRE(it.U, "null U");
+                SolvePlan &pl = find_plan(ctx, rp, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o);
+                if (pl.in_use) break;   // the same channel twice: it starts the next group
+                load_plan(ctx, pl, rp, nullptr, swp, it.os, it.chn, o);
+                pl.in_use = 1;
+                KMember mb;
+                mb.pl = &pl;
+                mb.os = it.os;
+                ms.push_back(std::move(mb));
+                idx.push_back(i);
+            } catch (const Error &e) {
+                record(i, e.code, e.msg);
+            }
+        }
+        i0 = i;
+        try {
+            solve_lockstep(ms, o, o.method == TT_SOLVE_MIXED);
+        } catch (const Error &e) {
+            for (int k : idx) record(k, e.code, e.msg);
+            continue;
+        }
+        const double ms_group = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (size_t k = 0; k < ms.size(); ++k) {
+            tt_batch_item &it = items[idx[k]];
+            SolvePlan &pl = *ms[k].pl;
+            std::vector<cdouble> Uc((size_t)ms[k].L.nd * ms[k].L.nd * ms[k].L.nE);
+            pl.read_onshell(ms[k].L, Uc.data());
+            std::memcpy(it.U, Uc.data(), sizeof(cdouble) * Uc.size());
+            tt_solve_stats st;
+            std::memset(&st, 0, sizeof(st));
+            channel_stats(pl, st);
+            st.iterations = ms[k].iters;
+            st.residual = ms[k].res;
+            st.ms_total = ms_group;   // the group's wall time (its channels are solved together)
+            if (it.stats) *it.stats = st;
+            try {
+                require_converged(o, st.residual);
+            } catch (const Error &e) {
+                record(idx[k], e.code, e.msg);
+            }
+        }
+    }
+    if (first != TT_OK) fail(first, first_msg);
+}
+
 // ------------------------------------------------------------------ gradient
 // U(theta) and dU/dtheta for one channel, V = sum_i theta_i V_i (no reference counterpart).
 //   forward:  V, SWP states, on-shell bins, G on the device, row GMRES y_k (1 - G_j A) = e_k^T A
@@ -2278,20 +2636,25 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     o.method = TT_SOLVE_GMRES;
     o.resolvent_on_device = 1;
     o.p123_format = TT_P123_CELLS;
-    tt_solve_stats st_f;
-    solve_channel(ctx, &rp, U, nullptr, P123, Vu.data(), Vc.data(), &swp, &os, &chn, o, &st_f);
     tt_grad_stats g;
     std::memset(&g, 0, sizeof(g));
-    g.ms_forward = ms_since();
-    g.n_apply_forward = st_f.n_apply;
-    g.residual_forward = st_f.residual;
     if (!dU) {
-        g.ms_total = g.ms_forward;
+        tt_solve_stats st_f;
+        solve_channel(ctx, &rp, U, nullptr, P123, Vu.data(), Vc.data(), &swp, &os, &chn, o, &st_f);
+        g.ms_forward = g.ms_total = ms_since();
+        g.n_apply_forward = st_f.n_apply;
+        g.residual_forward = st_f.residual;
         if (gs) *gs = g;
         return;
     }
+    // the forward row solve and the adjoint column solve are independent: they run in lockstep
     const DeviceCSC *P = dev_of(P123);
-    SolvePlan &rowp = get_plan(ctx, rp, P, Vu.data(), Vc.data(), swp, chn, TT_P123_CELLS, false);
+    SolvePlan &rowp = channel_plan(ctx, &rp, nullptr, P123, Vu.data(), Vc.data(), &swp, &os, &chn, o);
+    rowp.in_use = 1;   // not evicted by the lookup of the column plan
+    struct Release {
+        SolvePlan *p;
+        ~Release() { p->in_use = 0; }
+    } release{&rowp};
     SolvePlan &colp = get_plan(ctx, rp, P, Vu.data(), Vc.data(), swp, chn, TT_P123_CELLS, true);
     RowOperator &R = rowp.op, &Cl = colp.op;
     hipStream_t s = ctx->stream;
@@ -2301,25 +2664,28 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     // the column operator takes the same G and energy of each system
     double2 *Gc = Cl.G_buffer(nE);
     TT_HIP(hipMemcpyAsync(Gc, R.G.p, sizeof(double2) * (size_t)nE * dim, hipMemcpyDeviceToDevice, s));
-    Cl.set_gidx(L.row_g);
     Cl.collect();
-    Cl.n_apply = 0;
-    auto fit = [&](DevBuf<double2> &v) {
-        if (v.n < (size_t)ne) v.alloc(ne);
-    };
-    fit(colp.b);
-    fit(colp.x);
-    on_shell_rows_of_A(Cl, L, colp.b.p);   // A e_c
-    const int m = o.restart > 0 ? o.restart : 40;
-    Gmres &gc = colp.gmres(r, m, o.use_graphs != 0);
-    std::vector<double> bnorm = gc.norms(colp.b.p);
-    TT_HIP(hipMemsetAsync(colp.x.p, 0, sizeof(double2) * ne, s));
-    double res_a = 0.0;
-    gc.solve(colp.b.p, colp.x.p, o.tol, o.max_iter, false, res_a, bnorm, true);
-    g.residual_adjoint = res_a;
+    Cl.n_apply = Cl.n_apply32 = 0;
+    std::vector<KMember> ms(2);
+    ms[0].pl = &rowp;   // y_k (1 - G A) = e_k^T A
+    ms[0].os = &os;
+    ms[1].pl = &colp;   // (1 - A G) t_k = A e_k
+    ms[1].os = &os;
+    solve_lockstep(ms, o, false);
+    {
+        std::vector<cdouble> Uc((size_t)n);
+        rowp.read_onshell(ms[0].L, Uc.data());
+        std::memcpy(U, Uc.data(), sizeof(cdouble) * n);
+    }
+    g.ms_forward = ms_since();   // V, SWP states, G and both solves (run together)
+    g.n_apply_forward = R.n_apply;
+    g.residual_forward = ms[0].res;
+    g.residual_adjoint = ms[1].res;
     g.n_apply_adjoint = Cl.n_apply;
-    if (!(res_a <= o.tol))
-        fail(TT_ERR_NOT_CONVERGED, "adjoint column solve stopped at relative residual " + std::to_string(res_a));
+    if (!(ms[0].res <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "forward row solve stopped at relative residual " + std::to_string(ms[0].res));
+    if (!(ms[1].res <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "adjoint column solve stopped at relative residual " + std::to_string(ms[1].res));
     DevBuf<int64_t> unit;
     unit.upload(L.row_idx.data(), r, s);
     DevBuf<double2> z(ne), w(ne), Uw(ne), bz(ne), Dw(ne), Dz(ne);
@@ -2492,22 +2858,7 @@ tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_u
     tt_solve_opts_default(&o);
     if (opts) o = *opts;
     o.resolvent_on_device = 1;
-    tt_status first = TT_OK;
-    std::string first_msg;
-    for (int i = 0; i < n_items; ++i) {
-        tt_batch_item &it = items[i];
-        it.status = TT_OK;
-        try {
-            solve_channel(ctx, rp, it.U, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o, it.stats);
-        } catch (const Error &e) {
-            it.status = e.code;
-            if (first == TT_OK) {
-                first = e.code;
-                first_msg = "batch item " + std::to_string(i) + ": " + e.msg;
-            }
-        }
-    }
-    if (first != TT_OK) fail(first, first_msg);
+    tt::solve_batch(ctx, rp, V_unco, V_coup, swp, items, n_items, o);
     TT_GUARD_END
 }
 
