@@ -280,7 +280,20 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=20.0):
             "max_rel_vs_gpu_plain_fp64": diff}
 
 
-LEGS = ("plain", "single", "n40", "grad", "cpu")
+def copy_calibration(mib=256, reps=3):
+    """A plain device-to-device copy of known size (torch's copy kernel, mib MiB each way, reps
+    launches): the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to
+    true HBM bytes in the traffic summaries."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_each_way": mib << 20, "launches": reps}
+
+
+LEGS = ("plain", "single", "n40", "grad", "cpu", "copy")
 
 
 def main():
@@ -288,7 +301,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed batches (144 channel-solves each)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--legs", default=",".join(LEGS),
+    ap.add_argument("--legs", default=",".join(x for x in LEGS if x != "copy"),
                     help="comma-separated extra lines beside the headline batch: " + ", ".join(LEGS) + " (or none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -310,7 +323,7 @@ def main():
     t_64, U_64, st_64 = time_batches(ctx, wl, tt.SOLVE_GMRES, n64, 1)
     value = wl.n_solves * args.steps / t_mix
     fp64 = wl.n_solves * n64 / t_64
-    k32 = "k_cellmfma<float, MT, NT, 4>"
+    k32 = "k_cellmfma<float, MT, NT, UN>"
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -333,8 +346,9 @@ def main():
         "mixed_over_plain_fp64": None,
         "grad_over_fwd": None,
         "roofline": product_roofline(st_mix, k32, True, measured_traffic(k32)),
-        "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, MT, NT, 2>", False,
-                                          measured_traffic("k_cellmfma<double, MT, NT, 2>")),
+        "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, MT, NT, UN>", False,
+                                          measured_traffic("k_cellmfma<double, MT, NT, UN>")),
+        "cell_autotune": tt.cell_tuning(),
     }
     U_plain = None
     if "plain" in legs or "cpu" in legs:
@@ -363,6 +377,9 @@ def main():
         log("gradient")
         out["gradient"] = gradient_ratio(ctx)
         out["grad_over_fwd"] = out["gradient"]["grad_over_fwd"]
+    if "copy" in legs:
+        log("copy calibration")
+        out["copy_calibration"] = copy_calibration()
     if "cpu" in legs:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(wl, ctx, U_plain)

@@ -320,6 +320,11 @@ tt_status tt_solve_gradient(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_s
                             double *dU_dtheta, int capacity, int *n_out, const tt_solve_opts *opts,
                             tt_grad_stats *stats);
 
+/* The choices of the cell-product autotuner so far (no reference counterpart): one text line
+ * per tuned shape, "dev D row|column fp32|fp64 Nalpha N R2 R ncell C -> item_len L un U (T us)".
+ * Writes at most len bytes (NUL-terminated) into buf; returns the size the whole report needs. */
+int tt_cell_tuning(char *buf, int len);
+
 /* Raw kernel-matrix application for tests and profiling: Y = (X .* G_sel) A in
  * the reference's row form (CPP/solve_faddeev.cpp:956-1088), r rows.
  * X, Y: host, r*dim complex (row-major).  g_index[k] selects the energy of row k

@@ -15,10 +15,13 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 
 #include "tt_internal.h"
@@ -474,12 +477,20 @@ __global__ __launch_bounds__(256) void k_spmm_gather(const uint64_t *__restrict_
 // wave w takes batches w, w+4, ..., loading the next batch while the MFMAs of the current one
 // run.  The four waves' accumulators are added in LDS in a fixed order, the items of a ket by
 // k_cell_reduce, so the result does not depend on timing.  blockIdx.y selects the v-tile.
-// Measured at Nalpha = 27, R2 = 8 on 22k cells: 16-cell items, batches of 2 k-steps (FP64)
-// or 4 (FP32) per wave.
+// The item length and the batch depth UN are chosen per operator and shape by a timing
+// autotuner (RowOperator::tune); the defaults below are its starting point (measured at
+// Nalpha = 27, R2 = 8 on 22k cells).
 constexpr int CELL_ITEM = 16;
 template <typename T>
 constexpr int cell_un() {
     return sizeof(T) == 8 ? 2 : 4;
+}
+// candidate batch depths of the autotuner: the deeper batches keep more bytes in flight per
+// wave, which pays at large tiles (MT * NT), where the accumulators limit occupancy
+template <typename T>
+static const std::vector<int> &cell_un_candidates() {
+    static const std::vector<int> c = sizeof(T) == 8 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4, 8};
+    return c;
 }
 
 template <typename T>
@@ -664,12 +675,34 @@ struct CellItems {
     }
 };
 
+template <typename T, int MT, int NT, int UN>
+static void launch_cellmfma_un(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
+                               T *Upart, int R2) {
+    const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
+    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ci.items.p, P->partner, blk, S, Upart, P->Na,
+                       R2);
+}
+
+// un: batch depth (one of cell_un_candidates<T>(); 0 = the default)
 template <typename T, int MT, int NT>
 static void launch_cellmfma(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
-                            T *Upart, int R2) {
-    const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
-    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, cell_un<T>()>), grid, block, 0, s, ci.items.p, P->partner, blk, S, Upart,
-                       P->Na, R2);
+                            T *Upart, int R2, int un) {
+    if (un == 0) un = cell_un<T>();
+    if constexpr (sizeof(T) == 8) {
+        if (un == 1)
+            launch_cellmfma_un<T, MT, NT, 1>(s, P, ci, blk, S, Upart, R2);
+        else if (un == 4)
+            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2);
+        else
+            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2);
+    } else {
+        if (un == 2)
+            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2);
+        else if (un == 8)
+            launch_cellmfma_un<T, MT, NT, 8>(s, P, ci, blk, S, Upart, R2);
+        else
+            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2);
+    }
 }
 
 // N tiles per pass: up to 64 real components (32 complex right-hand sides) stream the blocks
@@ -684,19 +717,19 @@ static int cell_nt(int R2) {
 
 template <typename T, int MT>
 static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
-                               T *Upart, int R2) {
+                               T *Upart, int R2, int un) {
     switch (cell_nt(R2)) {
         case 1:
-            launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2, un);
             break;
         case 2:
-            launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2, un);
             break;
         case 3:
-            launch_cellmfma<T, MT, 3>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma<T, MT, 3>(s, P, ci, blk, S, Upart, R2, un);
             break;
         default:
-            launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2, un);
             break;
     }
 }
@@ -705,17 +738,17 @@ static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems
 // U = scale * (sum of each ket's item panels), the full product in cell-major order.
 template <typename T>
 static void launch_cellmm(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S, T *U,
-                          T *Upart, int R2, T scale) {
+                          T *Upart, int R2, T scale, int un = 0) {
     TT_REQUIRE(P->Na >= 1 && P->Na <= 64, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
     if (ci.n > 0) {
         if (P->Na <= 16)
-            launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2, un);
         else if (P->Na <= 32)
-            launch_cellmfma_nt<T, 2>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_nt<T, 2>(s, P, ci, blk, S, Upart, R2, un);
         else if (P->Na <= 48)
-            launch_cellmfma_nt<T, 3>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_nt<T, 3>(s, P, ci, blk, S, Upart, R2, un);
         else
-            launch_cellmfma_nt<T, 4>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_nt<T, 4>(s, P, ci, blk, S, Upart, R2, un);
     }
     if (U) {
         const int64_t nout = (int64_t)P->NqNp * P->Na * R2;
@@ -1018,6 +1051,29 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 
 // ------------------------------------------------------------------ the operator
 // Matrix-free row-form application Y = (X .* G_sel) A for r rows at once.
+// Process-wide table of the cell-product autotuner: the chosen item length and batch depth per
+// device, orientation (row / column operator), precision, Nalpha, R2 and cell count, with the
+// winning launch time.
+struct TuneKey {
+    int dev, col, fp32, Na, R2;
+    int64_t ncell;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(dev, col, fp32, Na, R2, ncell) < std::tie(o.dev, o.col, o.fp32, o.Na, o.R2, o.ncell);
+    }
+};
+struct TuneEntry {
+    int len, un;
+    float ms;
+};
+struct TuneTable {
+    std::mutex mu;
+    std::map<TuneKey, TuneEntry> map;
+};
+static TuneTable &tune_table() {
+    static TuneTable t;
+    return t;
+}
+
 // Device copies of the resolvent's inputs (SWP edges, q edges, energies, per-alpha edge
 // offsets), kept by an operator so that computing its G allocates nothing and needs no wait.
 struct ResolventStaging {
@@ -1053,7 +1109,13 @@ struct RowOperator {
     // through the transposed cells) instead of the row form y -> y A
     bool col = false;
     CellView cv;
-    CellItems items;
+    // work items by item length; the autotuner picks the length and batch depth per
+    // (precision, R2), and every application of that shape uses its choice
+    std::map<int, std::unique_ptr<CellItems>> items_by_len;
+    struct CellTune {
+        int len = CELL_ITEM, un = 0;
+    };
+    std::map<int, CellTune> tuned;   // key tune_key(fp32, R2)
     DevBuf<double> upart;  // partial panels of the work items (FP64 and FP32 share it)
     // bumped whenever a buffer the kernels address is (re)allocated: captured graphs hold
     // these addresses and are rebuilt when it changes
@@ -1110,7 +1172,7 @@ struct RowOperator {
         if (cells) {
             if (col) build_transposed_cells(*const_cast<DeviceCSC *>(csc), s);
             cv = cells_of(*csc, col);
-            items.build(cv, s);
+            items_of_len(CELL_ITEM);
         }
         set_blocks(rp, V_unco, V_coup, swp, chn);
     }
@@ -1192,6 +1254,7 @@ struct RowOperator {
         G.upload(reinterpret_cast<const double2 *>(Gh), n, s);
     }
 
+    // (the panels start zeroed: the autotuner times products over them before any data)
     void reserve(int r, bool fp32) {
         if (r > r_cap) {
             S.alloc((size_t)dim * r);
@@ -1200,11 +1263,13 @@ struct RowOperator {
             r_cap = r;
             S32.release();
             U32.release();
+            TT_HIP(hipMemsetAsync(S.p, 0, sizeof(double2) * S.n, s));
             ++layout_gen;
         }
         if (fp32 && !S32.p) {
             S32.alloc((size_t)dim * r_cap);
             U32.alloc((size_t)dim * r_cap);
+            TT_HIP(hipMemsetAsync(S32.p, 0, sizeof(float2) * S32.n, s));
             ++layout_gen;
         }
     }
@@ -1214,11 +1279,107 @@ struct RowOperator {
     void prepare(int r, bool fp32) {
         reserve(r, fp32);
         if (fp32) ensure_f32();
-        const size_t need = (size_t)std::max(items.n, 1) * Na * 2 * r;
-        if (cells && upart.n < need) {
+        if (cells) {
+            tune(r, fp32);
+            const size_t need = (size_t)std::max(items_for(fp32, 2 * r).n, 1) * Na * 2 * r;
+            if (upart.n < need) {
+                upart.alloc(need);
+                ++layout_gen;
+            }
+        }
+    }
+
+    static int tune_key(bool fp32, int R2) { return (fp32 ? 1 << 20 : 0) + R2; }
+
+    CellItems &items_of_len(int len) {
+        auto it = items_by_len.find(len);
+        if (it != items_by_len.end()) return *it->second;
+        std::unique_ptr<CellItems> ci(new CellItems);
+        ci->build(cv, s, len);
+        CellItems &ref = *ci;
+        items_by_len[len] = std::move(ci);
+        return ref;
+    }
+    const CellTune &choice(bool fp32, int R2) const {
+        static const CellTune dflt;
+        auto it = tuned.find(tune_key(fp32, R2));
+        return it == tuned.end() ? dflt : it->second;
+    }
+    CellItems &items_for(bool fp32, int R2) { return items_of_len(choice(fp32, R2).len); }
+
+    // Autotuner of the cell-block product: on the first application of a shape (precision, r),
+    // each candidate item length and batch depth is timed on this operator's own blocks (a
+    // warm-up launch, then the mean of three launches between events, the stream otherwise
+    // idle) and the fastest kept.  Choices are shared through a process-wide table keyed by
+    // device, orientation, precision, Nalpha, R2 and the cell count, so channels of equal shape
+    // tune once.  Runs eagerly from prepare(), never inside a graph capture.
+    void tune(int r, bool fp32) {
+        const int R2 = 2 * r;
+        const int key = tune_key(fp32, R2);
+        if (tuned.count(key)) return;
+        int dev = 0;
+        TT_HIP(hipGetDevice(&dev));
+        TuneTable &tt_ = tune_table();
+        const TuneKey gk{dev, col ? 1 : 0, fp32 ? 1 : 0, Na, R2, cv.ncell};
+        {
+            std::lock_guard<std::mutex> lock(tt_.mu);
+            auto it = tt_.map.find(gk);
+            if (it != tt_.map.end()) {
+                tuned[key] = {it->second.len, it->second.un};
+                ++layout_gen;
+                return;
+            }
+        }
+        const int lens[3] = {8, 16, 32};
+        size_t need = 0;
+        for (int len : lens) need = std::max(need, (size_t)std::max(items_of_len(len).n, 1) * Na * R2);
+        if (upart.n < need) {
             upart.alloc(need);
             ++layout_gen;
         }
+        hipEvent_t e0, e1;
+        TT_HIP(hipEventCreate(&e0));
+        TT_HIP(hipEventCreate(&e1));
+        CellTune best;
+        float best_ms = 1e30f;
+        try {
+            for (int len : lens) {
+                const CellItems &ci = items_of_len(len);
+                const std::vector<int> &uns = fp32 ? cell_un_candidates<float>() : cell_un_candidates<double>();
+                for (int un : uns) {
+                    auto launch = [&]() {
+                        if (fp32)
+                            launch_cellmm<float>(s, &cv, ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                                                 reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
+                        else
+                            launch_cellmm<double>(s, &cv, ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
+                                                  upart.p, R2, 2.0, un);
+                    };
+                    launch();
+                    TT_HIP(hipEventRecord(e0, s));
+                    for (int rep = 0; rep < 3; ++rep) launch();
+                    TT_HIP(hipEventRecord(e1, s));
+                    TT_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.len = len;
+                        best.un = un;
+                    }
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tuned[key] = best;
+        ++layout_gen;
+        std::lock_guard<std::mutex> lock(tt_.mu);
+        tt_.map[gk] = {best.len, best.un, best_ms / 3.0f};
     }
 
     void set_gidx(const std::vector<int> &g) {
@@ -1290,7 +1451,9 @@ struct RowOperator {
         const int cm = cells ? 1 : 0;
         // cell path: S -> work-item partial panels; the item sum and the factor 2 are folded
         // into the input staging of the V.C block op
-        const int *kip = cells ? items.ket_item_ptr.p : nullptr;
+        CellItems *ci = cells ? &items_for(fp32, R2) : nullptr;
+        const int un = cells ? choice(fp32, R2).un : 0;
+        const int *kip = cells ? ci->ket_item_ptr.p : nullptr;
         const double in_scale = cells ? 2.0 : 1.0;
         double2 *zb = zero_next;
         const int zn = zero_next_n;
@@ -1303,8 +1466,8 @@ struct RowOperator {
                                                1.0, zb, zn);
                 if (timed) record(e1);
                 if (cells)
-                    launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
-                                          upart.p, R2, 2.0);
+                    launch_cellmm<double>(s, &cv, *ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
+                                          upart.p, R2, 2.0, un);
                 else
                     launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
                                         reinterpret_cast<double *>(U.p), dim, R2, 2.0);
@@ -1321,8 +1484,8 @@ struct RowOperator {
                                           zb, zn);
             if (timed) record(e1);
             if (cells)
-                launch_cellmm<float>(s, &cv, items, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
-                                     reinterpret_cast<float *>(upart.p), R2, 2.0f);
+                launch_cellmm<float>(s, &cv, *ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                                     reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
             else
                 launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
                                    reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
@@ -1349,8 +1512,8 @@ struct RowOperator {
         prepare(r, false);
         launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, nullptr, gidx.p, Na,
                                        Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);
-        launch_cellmm<double>(s, &cv, items, cv.blk, reinterpret_cast<const double *>(S.p),
-                              reinterpret_cast<double *>(Ucell), upart.p, 2 * r, 2.0);
+        launch_cellmm<double>(s, &cv, items_for(false, 2 * r), cv.blk, reinterpret_cast<const double *>(S.p),
+                              reinterpret_cast<double *>(Ucell), upart.p, 2 * r, 2.0, choice(false, 2 * r).un);
     }
 
     // Adds the timing of one application recorded into caller events (after they completed).
@@ -1433,12 +1596,24 @@ static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
 // objects (the channels of a batch) advance in lockstep on one stream (lockstep_cycle), so one
 // host wait serves all of them and the stream always holds every channel's next step.
 // With use_graphs each step (j) is captured once into a HIP graph and replayed afterwards.
+// The part of a Krylov object the lockstep driver sees.
+struct KrylovBase {
+    virtual ~KrylovBase() = default;
+    bool in_cycle = false;
+    int jmax = 0, used = 0;
+    std::vector<hipEvent_t> step_ev;
+    virtual void advance(int j) = 0;       // enqueue step j (and one ahead)
+    virtual void take_column(int j) = 0;   // after step_ev[j]: Givens rotations, convergence
+    virtual void finish() = 0;             // enqueue x += V z (the x of begin_cycle)
+};
+
 template <typename VT>
-struct Krylov {
+struct Krylov : KrylovBase {
     RowOperator &op;
     int r, m;
     int64_t ne;
     bool use_graphs;
+    double2 *x_out = nullptr;            // solution the cycle's update is added to
     int nb_dots = 1;                    // row slices of k_dots per vector (at most 64)
     int slot_len;                       // (2(m+1)+1)*r double2 per Arnoldi step
     DevBuf<VT> V;                       // (m+1) basis vectors of ne elements
@@ -1446,7 +1621,6 @@ struct Krylov {
     double2 *Hh = nullptr;              // pinned mirror of Hd
     double2 *zh = nullptr;              // pinned staging of the update coefficients
     double2 *n2h = nullptr;             // pinned copy of n2d (squared norms)
-    std::vector<hipEvent_t> step_ev;
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1459,8 +1633,8 @@ struct Krylov {
     std::vector<cdouble> H, cs, sn, g;
     std::vector<double> ref, prev_rel;  // residual norms are relative to ref[k]
     double tol = 0.0;
-    int jmax = 0, used = 0, launched = 0;
-    bool hold = false, in_cycle = false, conv = false;
+    int launched = 0;
+    bool hold = false, conv = false;
     double est = 0.0;                   // largest relative residual estimate at the end of the cycle
     bool graph_timing = true;
 
@@ -1534,6 +1708,8 @@ struct Krylov {
         op.zero_next_n = slot_len;
         // the new vector w = M v_j is formed in place, in column j + 1 of the basis
         op.apply(vj, vj1, r, true, fp32, ev != nullptr || (!capturing && j == 0), vj, ev);
+        // two classical Gram-Schmidt passes (one pass, with the Pythagorean norm, loses the
+        // normalisation of the basis: the FP32 inner solves then need several times the steps)
         dots_dev(V.p, j + 1, vj1, h1);
         hipLaunchKernelGGL((k_combo<VT, VT>), dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h1, ne, r,
                            -1.0);
@@ -1599,7 +1775,8 @@ struct Krylov {
     // relative to ref; the cycle stops when all are <= tol (with a margin of 2) or after
     // jmax_ steps.
     void begin_cycle(const double2 *R, const std::vector<double> &beta, const std::vector<double> &ref_, double tol_,
-                     int jmax_) {
+                     int jmax_, double2 *x) {
+        x_out = x;
         hipLaunchKernelGGL(k_scale_by_norm<VT>, dim3(nblk(ne, 256)), dim3(256), 0, op.s, V.p, R, n2d.p, ne, r);
         ref = ref_;
         tol = tol_;
@@ -1616,13 +1793,13 @@ struct Krylov {
     }
 
     // Enqueue step j if not yet launched, and step j + 1 ahead unless held back.
-    void advance(int j) {
+    void advance(int j) override {
         if (launched <= j) step(launched++);
         if (!hold && launched == j + 1 && launched < jmax) step(launched++);
     }
 
     // Hessenberg column j (the step has completed): Givens rotations and residual estimates.
-    void take_column(int j) {
+    void take_column(int j) override {
         read_step_timing(j);
         const double2 *hj = Hh + (size_t)j * slot_len;
         bool conv_all = true, hold_next = true;
@@ -1667,8 +1844,7 @@ struct Krylov {
 
     // x += V z from the triangular least-squares solution of the cycle (enqueued).  Steps
     // launched ahead past the end of the cycle write basis vectors this does not read.
-    template <typename TX>
-    void end_cycle(TX *x) {
+    void finish() override {
         for (int k = 0; k < r; ++k) {
             auto Hk = [&](int i, int jj) { return H[((size_t)i * m + jj) * r + k]; };
             for (int i = used - 1; i >= 0; --i) {
@@ -1683,17 +1859,16 @@ struct Krylov {
         if (used == 0) return;
         // zh is rewritten by the next cycle only after that cycle's steps, which follow this copy
         TT_HIP(hipMemcpyAsync(zdev.p, zh, sizeof(double2) * (size_t)used * r, hipMemcpyHostToDevice, op.s));
-        hipLaunchKernelGGL((k_combo<TX, VT>), dim3(nblk(ne, 256)), dim3(256), 0, op.s, x, V.p, ne, used, zdev.p, ne, r,
-                           1.0);
+        hipLaunchKernelGGL((k_combo<double2, VT>), dim3(nblk(ne, 256)), dim3(256), 0, op.s, x_out, V.p, ne, used,
+                           zdev.p, ne, r, 1.0);
     }
 };
 
 // One lockstep cycle over the Krylov objects ks (each begun with begin_cycle): step j of every
 // object still in its cycle is enqueued (with one step of lookahead), one wait covers them all,
-// then the host takes each column; an object whose cycle ends adds its update to its x
-// (xs[i]) at once.  Returns when every cycle has ended.
-template <typename VT, typename TX>
-static void lockstep_cycle(const std::vector<Krylov<VT> *> &ks, const std::vector<TX *> &xs) {
+// then the host takes each column; an object whose cycle ends adds its update to its x at once.
+// Returns when every cycle has ended.
+static void lockstep_cycle(const std::vector<KrylovBase *> &ks) {
     int jmax = 0;
     for (auto *k : ks) jmax = std::max(jmax, k->in_cycle ? k->jmax : 0);
     for (int j = 0; j < jmax; ++j) {
@@ -1706,15 +1881,14 @@ static void lockstep_cycle(const std::vector<Krylov<VT> *> &ks, const std::vecto
         if (!any) break;
         for (auto *k : ks)
             if (k->in_cycle) TT_HIP(hipEventSynchronize(k->step_ev[j]));
-        for (size_t i = 0; i < ks.size(); ++i) {
-            Krylov<VT> *k = ks[i];
+        for (auto *k : ks) {
             if (!k->in_cycle) continue;
             k->take_column(j);
-            if (!k->in_cycle) k->end_cycle(xs[i]);
+            if (!k->in_cycle) k->finish();
         }
     }
-    for (size_t i = 0; i < ks.size(); ++i)
-        if (ks[i]->in_cycle) ks[i]->end_cycle(xs[i]);
+    for (auto *k : ks)
+        if (k->in_cycle) k->finish();
 }
 
 // ------------------------------------------------------------------ solve driver
@@ -1894,6 +2068,15 @@ struct SolvePlan {
     double2 *os_out_h = nullptr;
     size_t os_out_cap = 0;
     uint64_t in_use = 0;   // mark of the lockstep solve holding this plan
+    // work buffers of the gradient assembly (row plans of tt_solve_gradient)
+    struct GradWork {
+        DevBuf<int64_t> unit;
+        DevBuf<double2> z, w, Uw, bz, Dw, Dz, part, dG, Gp, tri;
+        DevBuf<int4> pairs;
+        DevBuf<int> which, offset, ints;
+        DevBuf<double> dbls;
+    };
+    std::unique_ptr<GradWork> gw;
 
     ~SolvePlan() {
         if (os_out_h) (void)hipHostFree(os_out_h);
@@ -2041,9 +2224,11 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
 
 // One channel of a lockstep Krylov solve: its plan, on-shell layout and right-hand sides b
 // (the on-shell rows of its operator), solution x and residual in the plan's buffers.
+// mixed: iterative refinement (FP64 residual, FP32 inner GMRES) instead of FP64 GMRES.
 struct KMember {
     SolvePlan *pl = nullptr;
     const tt_channel_os *os = nullptr;
+    bool mixed = false;
     OnshellLayout L;
     std::vector<double> bnorm;
     double res = 0.0;
@@ -2051,11 +2236,12 @@ struct KMember {
     bool done = false;
 };
 
-// GMRES (FP64) or iterative refinement (FP64 residual, FP32 inner GMRES) of every member, all
-// members advancing together on the one stream of their plans: every host wait (residual
-// norms, Arnoldi steps) serves all members.  G must be in each operator; the solutions stay in
-// the plans' x.
-static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, bool mixed) {
+// Every member solved to o.tol, all members advancing together on the one stream of their
+// plans: every host wait (residual norms, Arnoldi steps) serves all members.  An outer pass is
+// one cycle per member (a restart cycle of FP64 GMRES, or the inner FP32 solve of a refinement
+// step) followed by the true FP64 residual.  G must be in each operator; the solutions stay in
+// the plans' x and their on-shell elements are fetched (read_onshell) before returning.
+static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
     if (ms.empty()) return;
     const int m = o.restart > 0 ? o.restart : 40;
     const bool graphs = o.use_graphs != 0;
@@ -2063,6 +2249,13 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, boo
     hipStream_t s = ms[0].pl->op.s;
     std::vector<Krylov<double2> *> k64(n, nullptr);
     std::vector<Krylov<float2> *> k32(n, nullptr);
+    auto enqueue_norms = [&](size_t i, const double2 *v) {
+        if (ms[i].mixed)
+            k32[i]->enqueue_norms(v);
+        else
+            k64[i]->enqueue_norms(v);
+    };
+    auto read_norms = [&](size_t i) { return ms[i].mixed ? k32[i]->read_norms() : k64[i]->read_norms(); };
     for (size_t i = 0; i < n; ++i) {
         KMember &mb = ms[i];
         SolvePlan &pl = *mb.pl;
@@ -2073,90 +2266,66 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, boo
         pl.fit(r);
         on_shell_rows_of_A(pl.op, mb.L, pl.b.p, pl.rres.p, &pl.unit_idx);
         TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * (size_t)pl.op.dim * r, s));
-        if (mixed) {
+        if (mb.mixed)
             k32[i] = &pl.krylov<float2>(r, m, graphs);
-            k32[i]->enqueue_norms(pl.b.p);
-        } else {
+        else
             k64[i] = &pl.krylov<double2>(r, m, graphs);
-            k64[i]->enqueue_norms(pl.b.p);
-        }
+        enqueue_norms(i, pl.b.p);
         mb.res = 1.0;
         mb.iters = 0;
         mb.done = false;
     }
     TT_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < n; ++i) ms[i].bnorm = mixed ? k32[i]->read_norms() : k64[i]->read_norms();
+    for (size_t i = 0; i < n; ++i) ms[i].bnorm = read_norms(i);
     auto rel_max = [](const std::vector<double> &a, const std::vector<double> &ref) {
         double v = 0.0;
         for (size_t k = 0; k < a.size(); ++k) v = std::max(v, a[k] / std::max(ref[k], 1e-300));
         return v;
     };
-    // R = b - M x for the members still solving, with the column norms of R enqueued
-    auto residuals = [&](const std::vector<size_t> &act) {
-        for (size_t i : act) {
-            SolvePlan &pl = *ms[i].pl;
-            const int64_t ne = pl.op.dim * ms[i].L.r;
-            pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);   // M x
-            hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, s, pl.rres.p, pl.b.p, pl.rres.p, 1.0, -1.0,
-                               ne);
-            if (mixed)
-                k32[i]->enqueue_norms(pl.rres.p);
-            else
-                k64[i]->enqueue_norms(pl.rres.p);
-        }
-    };
+    std::vector<std::vector<double>> beta(n);
     for (int outer = 0;; ++outer) {
         // members still solving; the first pass takes R = b (x = 0), whose norms are in hand
         std::vector<size_t> act;
         for (size_t i = 0; i < n; ++i)
             if (!ms[i].done) act.push_back(i);
         if (act.empty()) break;
-        std::vector<std::vector<double>> beta(n);
         if (outer == 0) {
             for (size_t i : act) beta[i] = ms[i].bnorm;
         } else {
-            residuals(act);
+            for (size_t i : act) {   // R = b - M x (FP64) and its column norms
+                SolvePlan &pl = *ms[i].pl;
+                const int64_t ne = pl.op.dim * ms[i].L.r;
+                pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);
+                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, s, pl.rres.p, pl.b.p, pl.rres.p, 1.0,
+                                   -1.0, ne);
+                enqueue_norms(i, pl.rres.p);
+            }
             TT_HIP(hipStreamSynchronize(s));
-            for (size_t i : act) beta[i] = mixed ? k32[i]->read_norms() : k64[i]->read_norms();
+            for (size_t i : act) beta[i] = read_norms(i);
         }
-        std::vector<size_t> go;
+        std::vector<KrylovBase *> ks;
         for (size_t i : act) {
             KMember &mb = ms[i];
             mb.res = rel_max(beta[i], mb.bnorm);
-            if (mb.res <= o.tol || mb.iters >= o.max_iter || (mixed && outer >= 20))
+            if (mb.res <= o.tol || mb.iters >= o.max_iter || (mb.mixed && outer >= 20)) {
                 mb.done = true;
-            else
-                go.push_back(i);
-        }
-        if (go.empty()) break;
-        const double2 *R0 = nullptr;
-        if (!mixed) {
-            std::vector<Krylov<double2> *> ks;
-            std::vector<double2 *> xs;
-            for (size_t i : go) {
-                SolvePlan &pl = *ms[i].pl;
-                R0 = outer == 0 ? pl.b.p : pl.rres.p;
-                k64[i]->begin_cycle(R0, beta[i], ms[i].bnorm, o.tol, o.max_iter - ms[i].iters);
+                continue;
+            }
+            SolvePlan &pl = *mb.pl;
+            const double2 *R0 = outer == 0 ? pl.b.p : pl.rres.p;
+            if (!mb.mixed) {
+                k64[i]->begin_cycle(R0, beta[i], mb.bnorm, o.tol, o.max_iter - mb.iters, pl.x.p);
                 ks.push_back(k64[i]);
-                xs.push_back(pl.x.p);
-            }
-            lockstep_cycle(ks, xs);
-            for (size_t k = 0; k < go.size(); ++k) ms[go[k]].iters += ks[k]->used;
-        } else {
-            // inner FP32 GMRES on the residual, to a tolerance relative to it
-            std::vector<Krylov<float2> *> ks;
-            std::vector<double2 *> xs;
-            for (size_t i : go) {
-                SolvePlan &pl = *ms[i].pl;
-                R0 = outer == 0 ? pl.b.p : pl.rres.p;
-                const double inner_tol = std::min(1e-4, std::max(1e-6, o.tol / std::max(ms[i].res, 1e-300)));
-                k32[i]->begin_cycle(R0, beta[i], beta[i], inner_tol, o.max_iter - ms[i].iters);
+            } else {   // inner FP32 GMRES on the residual, to a tolerance relative to it
+                const double inner_tol = std::min(1e-4, std::max(1e-6, o.tol / std::max(mb.res, 1e-300)));
+                k32[i]->begin_cycle(R0, beta[i], beta[i], inner_tol, o.max_iter - mb.iters, pl.x.p);
                 ks.push_back(k32[i]);
-                xs.push_back(pl.x.p);
             }
-            lockstep_cycle(ks, xs);
-            for (size_t k = 0; k < go.size(); ++k) ms[go[k]].iters += ks[k]->used;
         }
+        if (ks.empty()) break;
+        lockstep_cycle(ks);
+        for (size_t i : act)
+            if (!ms[i].done) ms[i].iters += ms[i].mixed ? k32[i]->used : k64[i]->used;
     }
     for (auto &mb : ms) mb.pl->enqueue_onshell(mb.pl->x.p, mb.L, *mb.os);
     TT_HIP(hipStreamSynchronize(s));
@@ -2466,7 +2635,8 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
             std::vector<KMember> ms(1);
             ms[0].pl = &pl;
             ms[0].os = os;
-            solve_lockstep(ms, o, o.method == TT_SOLVE_MIXED);
+            ms[0].mixed = o.method == TT_SOLVE_MIXED;
+            solve_lockstep(ms, o);
             pl.read_onshell(ms[0].L, Uc.data());
             st.iterations = ms[0].iters;
             st.residual = ms[0].res;
@@ -2537,6 +2707,7 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
                 KMember mb;
                 mb.pl = &pl;
                 mb.os = it.os;
+                mb.mixed = o.method == TT_SOLVE_MIXED;
                 ms.push_back(std::move(mb));
                 idx.push_back(i);
             } catch (const Error &e) {
@@ -2545,7 +2716,7 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
         }
         i0 = i;
         try {
-            solve_lockstep(ms, o, o.method == TT_SOLVE_MIXED);
+            solve_lockstep(ms, o);
         } catch (const Error &e) {
             for (int k : idx) record(k, e.code, e.msg);
             continue;
@@ -2647,7 +2818,8 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
         if (gs) *gs = g;
         return;
     }
-    // the forward row solve and the adjoint column solve are independent: they run in lockstep
+    // The forward row solve and the adjoint column solve are independent: they run in lockstep
+    // on the device while host threads compute the SWP perturbation of every theta.
     const DeviceCSC *P = dev_of(P123);
     SolvePlan &rowp = channel_plan(ctx, &rp, nullptr, P123, Vu.data(), Vc.data(), &swp, &os, &chn, o);
     rowp.in_use = 1;   // not evicted by the lookup of the column plan
@@ -2661,49 +2833,6 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     const OnshellLayout L = onshell_layout(os, Nq, Np);
     const int r = L.r, Na = chn.Nalpha;
     const int64_t dim = R.dim, ne = dim * r;
-    // the column operator takes the same G and energy of each system
-    double2 *Gc = Cl.G_buffer(nE);
-    TT_HIP(hipMemcpyAsync(Gc, R.G.p, sizeof(double2) * (size_t)nE * dim, hipMemcpyDeviceToDevice, s));
-    Cl.collect();
-    Cl.n_apply = Cl.n_apply32 = 0;
-    std::vector<KMember> ms(2);
-    ms[0].pl = &rowp;   // y_k (1 - G A) = e_k^T A
-    ms[0].os = &os;
-    ms[1].pl = &colp;   // (1 - A G) t_k = A e_k
-    ms[1].os = &os;
-    solve_lockstep(ms, o, false);
-    {
-        std::vector<cdouble> Uc((size_t)n);
-        rowp.read_onshell(ms[0].L, Uc.data());
-        std::memcpy(U, Uc.data(), sizeof(cdouble) * n);
-    }
-    g.ms_forward = ms_since();   // V, SWP states, G and both solves (run together)
-    g.n_apply_forward = R.n_apply;
-    g.residual_forward = ms[0].res;
-    g.residual_adjoint = ms[1].res;
-    g.n_apply_adjoint = Cl.n_apply;
-    if (!(ms[0].res <= o.tol))
-        fail(TT_ERR_NOT_CONVERGED, "forward row solve stopped at relative residual " + std::to_string(ms[0].res));
-    if (!(ms[1].res <= o.tol))
-        fail(TT_ERR_NOT_CONVERGED, "adjoint column solve stopped at relative residual " + std::to_string(ms[1].res));
-    DevBuf<int64_t> unit;
-    unit.upload(L.row_idx.data(), r, s);
-    DevBuf<double2> z(ne), w(ne), Uw(ne), bz(ne), Dw(ne), Dz(ne);
-    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, colp.x.p, z.p, R.G.p, R.gidx.p, unit.p,
-                       dim, r);
-    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, rowp.x.p, w.p, R.G.p, R.gidx.p, unit.p,
-                       dim, r);
-    R.stage_product(w.p, Uw.p, r);    // 2 P^T C_op w   (cell-major)
-    Cl.stage_product(z.p, bz.p, r);   // 2 P VC_op z    (cell-major)
-    std::vector<int4> pairs;
-    for (int dr = 0; dr < nd; ++dr)
-        for (int dc = 0; dc < nd; ++dc)
-            for (int j = 0; j < nE; ++j) pairs.push_back(make_int4(dr * nE + j, dc * nE + j, j, 0));
-    const int npairs = (int)pairs.size();
-    DevBuf<int4> dpairs;
-    dpairs.upload(pairs.data(), npairs, s);
-    const int nbp = (int)std::min<int64_t>(256, nblk(dim, 256));
-    DevBuf<double2> part((size_t)npairs * nbp), dG((size_t)nE * dim);
     std::vector<double> E(nE), fE(nE);
     const double Md = kMn + kMp + swp.E_bound;
     for (int j = 0; j < nE; ++j) {
@@ -2711,13 +2840,6 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
         // E_com = (q_l^2 + q_u^2) / (4 mu1(E_bound)) at the bin midpoint: dE/dE_bound
         fE[j] = 1.0 - os.E_com[j] * kMn / (Md * (kMn + Md));
     }
-    DevBuf<double2> Gp((size_t)3 * nE * dim);   // dG/d(lower edge), dG/d(upper edge), dG/dE
-    resolvent_on_device(s, rp, swp, chn, E.data(), nE, Gp.p, true);
-    std::vector<int> which, offset;
-    swp_edge_offsets(chn, swp, rp, which, offset);
-    DevBuf<int> dwhich, doffset;
-    dwhich.upload(which.data(), Na, s);
-    doffset.upload(offset.data(), Na, s);
     // host phase, one theta per task on a few threads: SWP perturbation, dVC = dV C + V dC,
     // and the partner lists of dC_op and dVC_op
     struct ThetaWork {
@@ -2754,72 +2876,133 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
         t.b2.push_back(0.0);
         t.de_u = d.de_unco;
         t.de_c = d.de_coup;
+        t.de_u.push_back(0.0);
         t.de_c.push_back(0.0);
         t.dE.resize(nE);
         for (int j = 0; j < nE; ++j) t.dE[j] = d.dE_bound * fE[j];
     };
-    {
-        const int nthr = (int)std::max(1u, std::min<unsigned>((unsigned)n_theta, std::min(8u, std::thread::hardware_concurrency())));
-        std::vector<std::exception_ptr> errs(nthr);
+    const int nthr = (int)std::max(1u, std::min<unsigned>((unsigned)n_theta, std::min(8u, std::thread::hardware_concurrency())));
+    std::vector<std::exception_ptr> errs(nthr);   // declared before the workers: outlives them
+    struct Workers {   // joined on every exit path
         std::vector<std::thread> th;
-        for (int t = 0; t < nthr; ++t)
-            th.emplace_back([&, t]() {
-                try {
-                    for (int i = t; i < n_theta; i += nthr) work(i);
-                } catch (...) {
-                    errs[t] = std::current_exception();
-                }
-            });
-        for (auto &x : th) x.join();
-        for (auto &e : errs)
-            if (e) std::rethrow_exception(e);
+        ~Workers() {
+            for (auto &x : th)
+                if (x.joinable()) x.join();
+        }
+    } workers;
+    for (int t = 0; t < nthr; ++t)
+        workers.th.emplace_back([&, t]() {
+            try {
+                for (int i = t; i < n_theta; i += nthr) work(i);
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        });
+    // the column operator takes the same G and energy of each system
+    double2 *Gc = Cl.G_buffer(nE);
+    TT_HIP(hipMemcpyAsync(Gc, R.G.p, sizeof(double2) * (size_t)nE * dim, hipMemcpyDeviceToDevice, s));
+    Cl.collect();
+    Cl.n_apply = Cl.n_apply32 = 0;
+    std::vector<KMember> ms(2);
+    ms[0].pl = &rowp;   // y_k (1 - G A) = e_k^T A
+    ms[0].os = &os;
+    ms[1].pl = &colp;   // (1 - A G) t_k = A e_k
+    ms[1].os = &os;
+    solve_lockstep(ms, o);
+    {
+        std::vector<cdouble> Uc((size_t)n);
+        rowp.read_onshell(ms[0].L, Uc.data());
+        std::memcpy(U, Uc.data(), sizeof(cdouble) * n);
     }
-    // device phase: buffers sized for every theta first (no reallocation behind pending work),
-    // then all thetas back to back and one read-back of the 3 * npairs terms per theta
-    size_t m_p = 1, m_a1 = 1, m_b1 = 1, m_a2 = 1, m_b2 = 1, m_u = 1, m_c = 1;
-    for (const ThetaWork &t : tw) {
-        m_p = std::max(m_p, t.p1.size());
-        m_a1 = std::max(m_a1, t.a1.size());
-        m_b1 = std::max(m_b1, t.b1.size());
-        m_a2 = std::max(m_a2, t.a2.size());
-        m_b2 = std::max(m_b2, t.b2.size());
-        m_u = std::max(m_u, t.de_u.size());
-        m_c = std::max(m_c, t.de_c.size());
-    }
-    DevBuf<int> bp1(m_p), ba1(m_a1), bp2(m_p), ba2(m_a2);
-    DevBuf<double> bb1(m_b1), bb2(m_b2), dde_u(m_u), dde_c(m_c), ddE(nE);
-    DevBuf<double2> tri_all((size_t)3 * npairs * n_theta);
-    std::vector<double2> tri_h((size_t)3 * npairs * n_theta);
+    g.ms_forward = ms_since();   // V, SWP states, G and both solves (run together)
+    g.n_apply_forward = R.n_apply;
+    g.residual_forward = ms[0].res;
+    g.residual_adjoint = ms[1].res;
+    g.n_apply_adjoint = Cl.n_apply;
+    if (!(ms[0].res <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "forward row solve stopped at relative residual " + std::to_string(ms[0].res));
+    if (!(ms[1].res <= o.tol))
+        fail(TT_ERR_NOT_CONVERGED, "adjoint column solve stopped at relative residual " + std::to_string(ms[1].res));
+    // device phase in the row plan's gradient buffers (allocated on first use, then reused)
+    if (!rowp.gw) rowp.gw.reset(new SolvePlan::GradWork);
+    SolvePlan::GradWork &gw = *rowp.gw;
+    for (DevBuf<double2> *v : {&gw.z, &gw.w, &gw.Uw, &gw.bz, &gw.Dw, &gw.Dz})
+        if (v->n < (size_t)ne) v->alloc(ne);
+    gw.unit.upload(L.row_idx.data(), r, s);
+    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, colp.x.p, gw.z.p, R.G.p, R.gidx.p,
+                       gw.unit.p, dim, r);
+    hipLaunchKernelGGL(k_unit_plus_G, dim3(nblk(ne, 256)), dim3(256), 0, s, rowp.x.p, gw.w.p, R.G.p, R.gidx.p,
+                       gw.unit.p, dim, r);
+    R.stage_product(gw.w.p, gw.Uw.p, r);    // 2 P^T C_op w   (cell-major)
+    Cl.stage_product(gw.z.p, gw.bz.p, r);   // 2 P VC_op z    (cell-major)
+    std::vector<int4> pairs;
+    for (int dr = 0; dr < nd; ++dr)
+        for (int dc = 0; dc < nd; ++dc)
+            for (int j = 0; j < nE; ++j) pairs.push_back(make_int4(dr * nE + j, dc * nE + j, j, 0));
+    const int npairs = (int)pairs.size();
+    gw.pairs.upload(pairs.data(), npairs, s);
+    const int nbp = (int)std::min<int64_t>(256, nblk(dim, 256));
+    if (gw.part.n < (size_t)npairs * nbp) gw.part.alloc((size_t)npairs * nbp);
+    if (gw.dG.n < (size_t)nE * dim) gw.dG.alloc((size_t)nE * dim);
+    if (gw.Gp.n < (size_t)3 * nE * dim) gw.Gp.alloc((size_t)3 * nE * dim);   // dG/d(lower edge), d(upper), dG/dE
+    resolvent_on_device(s, rp, swp, chn, E.data(), nE, gw.Gp.p, true, &R.rstage);
+    std::vector<int> which, offset;
+    swp_edge_offsets(chn, swp, rp, which, offset);
+    gw.which.upload(which.data(), Na, s);
+    gw.offset.upload(offset.data(), Na, s);
+    for (auto &x : workers.th) x.join();
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+    // every theta's lists packed into one int and one double array (one upload each)
+    struct Off {
+        size_t p1, a1, p2, a2, b1, b2, du, dc, dE;
+    };
+    std::vector<Off> off(n_theta);
+    std::vector<int> ints;
+    std::vector<double> dbls;
+    auto put_i = [&](const std::vector<int> &v) {
+        const size_t o1 = ints.size();
+        ints.insert(ints.end(), v.begin(), v.end());
+        return o1;
+    };
+    auto put_d = [&](const std::vector<double> &v) {
+        const size_t o1 = dbls.size();
+        dbls.insert(dbls.end(), v.begin(), v.end());
+        return o1;
+    };
     for (int i = 0; i < n_theta; ++i) {
         const ThetaWork &t = tw[i];
-        bp1.upload(t.p1.data(), t.p1.size(), s);
-        ba1.upload(t.a1.data(), t.a1.size(), s);
-        bb1.upload(t.b1.data(), t.b1.size(), s);
-        bp2.upload(t.p2.data(), t.p2.size(), s);
-        ba2.upload(t.a2.data(), t.a2.size(), s);
-        bb2.upload(t.b2.data(), t.b2.size(), s);
-        dde_u.upload(t.de_u.data(), std::max<size_t>(t.de_u.size(), 1), s);
-        dde_c.upload(t.de_c.data(), t.de_c.size(), s);
-        ddE.upload(t.dE.data(), nE, s);
-        launch_block<double2, double2>(s, w.p, Dw.p, bp1.p, ba1.p, bb1.p, t.ne1, nullptr, R.gidx.p, Na, Nq, Np, r,
-                                       nullptr, 0, 1, nullptr, 1.0);   // dC_op w
-        launch_block<double2, double2>(s, z.p, Dz.p, bp2.p, ba2.p, bb2.p, t.ne2, nullptr, R.gidx.p, Na, Nq, Np, r,
-                                       nullptr, 0, 1, nullptr, 1.0);   // dVC_op z
-        hipLaunchKernelGGL(k_resolvent_change, dim3(nblk((int64_t)nE * dim, 256)), dim3(256), 0, s, dG.p, Gp.p,
-                           dde_u.p, dde_c.p, dwhich.p, doffset.p, ddE.p, Na, Nq, Np, nE);
-        double2 *tri_i = tri_all.p + (size_t)3 * npairs * i;
-        const dim3 pg((unsigned)nbp, (unsigned)npairs);
-        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Dw.p, 1, bz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
-                           part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i);
-        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, Uw.p, 1, Dz.p, 1, nullptr, dpairs.p, Na, Nq * Np, r,
-                           part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i + npairs);
-        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, rowp.x.p, 0, colp.x.p, 0, dG.p, dpairs.p, Na, Nq * Np,
-                           r, part.p);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, part.p, nbp, 1, tri_i + 2 * npairs);
+        off[i] = {put_i(t.p1), put_i(t.a1), put_i(t.p2), put_i(t.a2), put_d(t.b1), put_d(t.b2), put_d(t.de_u),
+                  put_d(t.de_c), put_d(t.dE)};
     }
-    TT_HIP(hipMemcpyAsync(tri_h.data(), tri_all.p, sizeof(double2) * tri_h.size(), hipMemcpyDeviceToHost, s));
+    gw.ints.upload(ints.data(), ints.size(), s);
+    gw.dbls.upload(dbls.data(), dbls.size(), s);
+    if (gw.tri.n < (size_t)3 * npairs * n_theta) gw.tri.alloc((size_t)3 * npairs * n_theta);
+    const int *I = gw.ints.p;
+    const double *D = gw.dbls.p;
+    for (int i = 0; i < n_theta; ++i) {
+        const ThetaWork &t = tw[i];
+        const Off &f = off[i];
+        launch_block<double2, double2>(s, gw.w.p, gw.Dw.p, I + f.p1, I + f.a1, D + f.b1, t.ne1, nullptr, R.gidx.p,
+                                       Na, Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);   // dC_op w
+        launch_block<double2, double2>(s, gw.z.p, gw.Dz.p, I + f.p2, I + f.a2, D + f.b2, t.ne2, nullptr, R.gidx.p,
+                                       Na, Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);   // dVC_op z
+        hipLaunchKernelGGL(k_resolvent_change, dim3(nblk((int64_t)nE * dim, 256)), dim3(256), 0, s, gw.dG.p, gw.Gp.p,
+                           D + f.du, D + f.dc, gw.which.p, gw.offset.p, D + f.dE, Na, Nq, Np, nE);
+        double2 *tri_i = gw.tri.p + (size_t)3 * npairs * i;
+        const dim3 pg((unsigned)nbp, (unsigned)npairs);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, gw.Dw.p, 1, gw.bz.p, 1, nullptr, gw.pairs.p, Na, Nq * Np,
+                           r, gw.part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, gw.part.p, nbp, 1, tri_i);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, gw.Uw.p, 1, gw.Dz.p, 1, nullptr, gw.pairs.p, Na, Nq * Np,
+                           r, gw.part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, gw.part.p, nbp, 1, tri_i + npairs);
+        hipLaunchKernelGGL(k_pair_dots, pg, dim3(256), 0, s, rowp.x.p, 0, colp.x.p, 0, gw.dG.p, gw.pairs.p, Na,
+                           Nq * Np, r, gw.part.p);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(npairs), dim3(256), 0, s, gw.part.p, nbp, 1, tri_i + 2 * npairs);
+    }
+    std::vector<double2> tri_h((size_t)3 * npairs * n_theta);
+    TT_HIP(hipMemcpyAsync(tri_h.data(), gw.tri.p, sizeof(double2) * tri_h.size(), hipMemcpyDeviceToHost, s));
     TT_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < n_theta; ++i) {
         const double2 *tr = tri_h.data() + (size_t)3 * npairs * i;
@@ -2877,6 +3060,28 @@ tt_status tt_solve_gradient(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_s
     solve_gradient(ctx, *rp, *fwp, *pw_full, channel, P123, Vb_unco, Vb_coup, n_theta, theta, T_lab, n_T_lab, U,
                    dU_dtheta, capacity, n_out, o, stats);
     TT_GUARD_END
+}
+
+int tt_cell_tuning(char *buf, int len) {
+    tt::TuneTable &t = tt::tune_table();
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lock(t.mu);
+        for (const auto &kv : t.map) {
+            char line[160];
+            std::snprintf(line, sizeof(line), "dev %d %s %s Nalpha %d R2 %d ncell %lld -> item_len %d un %d (%.1f us)\n",
+                          kv.first.dev, kv.first.col ? "column" : "row", kv.first.fp32 ? "fp32" : "fp64",
+                          kv.first.Na, kv.first.R2, (long long)kv.first.ncell, kv.second.len, kv.second.un,
+                          1e3 * kv.second.ms);
+            out += line;
+        }
+    }
+    if (buf && len > 0) {
+        const int n = std::min<int>(len - 1, (int)out.size());
+        std::memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
+    return (int)out.size() + 1;
 }
 
 tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123, const double *V_unco,

@@ -130,7 +130,16 @@ PROTOTYPES = {
                                     C.c_int, _ip, C.POINTER(SolveOpts), C.POINTER(GradStats)]),
     "tt_apply_rows": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(CSC), _dp, _dp, C.POINTER(SwpStates),
                                 C.POINTER(PwStates), _dp, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int]),
+    "tt_cell_tuning": (C.c_int, [C.c_char_p, C.c_int]),
 }
+
+
+def cell_tuning():
+    """The cell-product autotuner's choices so far (tt_cell_tuning), one line per shape."""
+    n = lib().tt_cell_tuning(None, 0)
+    buf = C.create_string_buffer(n)
+    lib().tt_cell_tuning(buf, n)
+    return [ln for ln in buf.value.decode().splitlines() if ln]
 
 _lib = None
 
