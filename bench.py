@@ -60,8 +60,10 @@ class BatchWorkload:
         self.plain = [[self.prob.channel(c, [T]) for T in BATCH_T] for c in range(self.prob.n_channels)]
         self.n_solves = len(self.chans) * len(BATCH_T)
 
+    graphs = False
+
     def solve(self, ctx, method):
-        return self.prob.solve_batch(ctx, self.chans, self.Ps, method=method, tol=SOLVE_TOL)
+        return self.prob.solve_batch(ctx, self.chans, self.Ps, method=method, tol=SOLVE_TOL, graphs=self.graphs)
 
     def solve_plain(self, ctx, method=tt.SOLVE_GMRES):
         return [[ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)[0] for ch in row]
@@ -281,16 +283,28 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=20.0):
 
 
 def copy_calibration(mib=256, reps=3):
-    """A plain device-to-device copy of known size (torch's copy kernel, mib MiB each way, reps
-    launches): the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to
-    true HBM bytes in the traffic summaries."""
-    import torch
-    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(reps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    return {"bytes_each_way": mib << 20, "launches": reps}
+    """A plain device-to-device copy of known size (the HIP runtime's copy kernel behind
+    hipMemcpy, mib MiB each way, reps launches): the reference point that scales rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to true HBM bytes in the traffic summaries."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(dst, src, C.c_size_t(n), 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return {"bytes_each_way": n, "launches": reps}
 
 
 LEGS = ("plain", "single", "n40", "grad", "cpu", "copy")
@@ -303,6 +317,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--legs", default=",".join(x for x in LEGS if x != "copy"),
                     help="comma-separated extra lines beside the headline batch: " + ", ".join(LEGS) + " (or none)")
+    ap.add_argument("--graphs", action="store_true", help="replay captured HIP graphs of the Arnoldi steps in the "
+                    "batched solves (default: plain stream launches)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
@@ -316,6 +332,7 @@ def main():
     ctx = tt.Context(0)
     log("building P123 of the 18 channels")
     wl = BatchWorkload(ctx)
+    wl.graphs = args.graphs
     log("P123 built in %.1f s; timing the batched mixed solve" % wl.build_s)
     t_mix, U_mix, st_mix = time_batches(ctx, wl, tt.SOLVE_MIXED, args.steps, args.warmup)
     log("batched FP64")

@@ -299,6 +299,27 @@ __global__ void k_resolvent_change(double2 *__restrict__ dG, const double2 *__re
     dG[t] = make_double2(gl.x * dl + gu.x * du + gE.x * dj, gl.y * dl + gu.y * du + gE.y * dj);
 }
 
+// ------------------------------------------------------------------ MFMA 16x16x4 (FP64, FP32)
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of the 16x16 result held in accumulator register r of lane l (column = l % 16)
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
 // ------------------------------------------------------------------ block ops
 // Y[(a,q,p)][k] = sum_{e in part(a)} sum_p' B_e[p][p'] * Xg[(alpha_e,q,p')][k]   (or Z - that sum)
 // with Xg = X .* G_{gidx[k]} when G != nullptr.  One workgroup per output tile (a, q, k-chunk
@@ -493,26 +514,6 @@ static const std::vector<int> &cell_un_candidates() {
     return c;
 }
 
-template <typename T>
-struct Mfma16;
-template <>
-struct Mfma16<double> {
-    typedef double acc_t __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // row of the 16x16 result held in accumulator register r of lane l (column = l % 16)
-    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
-};
-template <>
-struct Mfma16<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
-};
-
 template <typename T, int N>
 struct Run {
     T v[N];
@@ -526,7 +527,11 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
     __shared__ acc_t red[3][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
-    const int item = blockIdx.x;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so workgroup b runs on
+    // XCD b % 8; it takes item start(b % 8) + b / 8, giving each XCD one contiguous range of
+    // items (neighbouring kets, which read largely the same bra rows of S from that XCD's L2)
+    const int nit = gridDim.x, xq = nit / 8, xr = nit % 8, xcd = blockIdx.x % 8;
+    const int item = xcd * xq + min(xcd, xr) + blockIdx.x / 8;
     const int v0 = blockIdx.y * 16 * NT;
     const int c_beg = items[item].x, ncell = items[item].y - c_beg;
     const int K = ncell * Na;
@@ -1308,7 +1313,8 @@ struct RowOperator {
     CellItems &items_for(bool fp32, int R2) { return items_of_len(choice(fp32, R2).len); }
 
     // Autotuner of the cell-block product: on the first application of a shape (precision, r),
-    // each candidate item length and batch depth is timed on this operator's own blocks (a
+    // each candidate item length and batch depth is timed, together with the block op that
+    // sums the work items' partial panels, on this operator's own blocks (a
     // warm-up launch, then the mean of three launches between events, the stream otherwise
     // idle) and the fastest kept.  Choices are shared through a process-wide table keyed by
     // device, orientation, precision, Nalpha, R2 and the cell count, so channels of equal shape
@@ -1330,7 +1336,7 @@ struct RowOperator {
                 return;
             }
         }
-        const int lens[3] = {8, 16, 32};
+        const int lens[4] = {8, 16, 32, 64};
         size_t need = 0;
         for (int len : lens) need = std::max(need, (size_t)std::max(items_of_len(len).n, 1) * Na * R2);
         if (upart.n < need) {
@@ -1347,13 +1353,22 @@ struct RowOperator {
                 const CellItems &ci = items_of_len(len);
                 const std::vector<int> &uns = fp32 ? cell_un_candidates<float>() : cell_un_candidates<double>();
                 for (int un : uns) {
+                    // the product and the block op that sums its item panels (the item length
+                    // trades the product's parallelism against that sum)
                     auto launch = [&]() {
-                        if (fp32)
+                        if (fp32) {
                             launch_cellmm<float>(s, &cv, ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                                  reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
-                        else
+                            launch_block<float2, float2>(s, reinterpret_cast<const float2 *>(upart.p), U32.p, vc_ptr.p,
+                                                         vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p, Na, Nq,
+                                                         Np, r, nullptr, 1, 0, ci.ket_item_ptr.p, 2.0);
+                        } else {
                             launch_cellmm<double>(s, &cv, ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
                                                   upart.p, R2, 2.0, un);
+                            launch_block<double2, double2>(s, reinterpret_cast<const double2 *>(upart.p), U.p, vc_ptr.p,
+                                                           vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p, Na,
+                                                           Nq, Np, r, nullptr, 1, 0, ci.ket_item_ptr.p, 2.0);
+                        }
                     };
                     launch();
                     TT_HIP(hipEventRecord(e0, s));
