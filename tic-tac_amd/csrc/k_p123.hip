@@ -110,8 +110,12 @@ __global__ __launch_bounds__(256) void k_angular(double *__restrict__ F, int rb0
     }
     __syncthreads();
     const int kmax = min(KCHUNK, K - k0);
-    for (int e = threadIdx.x; e < kmax * T; e += blockDim.x) {
-        const int kk = e / T, t = e % T;
+    // one tuple per wave at a time, its lanes on the points: every lane of a wave runs the
+    // same term list (no divergence on the tuples' different term counts) and the term data
+    // (coefficients and table indexes) are wave-uniform scalar loads
+    for (int e = threadIdx.x; e < KCHUNK * T; e += blockDim.x) {
+        const int t = __builtin_amdgcn_readfirstlane(e / KCHUNK), kk = e % KCHUNK;
+        if (kk >= kmax) continue;
         const int ix = (k0 + kk) % Nx;
         const double *y1 = Y1tab + (int64_t)ix * n1;
         const double *y2 = Y2 + kk * n2;

@@ -329,10 +329,24 @@ struct Mfma16<float> {
 // product for the input / output instead of the reference order [(alpha, q, p)][k].  With
 // ket_item_ptr the input is the array of work-item partial panels of the cell-block product
 // (cell-major per item) and Xg = in_scale * (sum of the ket's item panels, in item order).
-template <typename TI, typename TO>
+// CT: the type the tile is staged and summed in (double; float for the FP32 vectors of the inner
+// solves, whose blocks are then FP32 copies: half the LDS, so more workgroups per CU hide the
+// op's load latency).
+template <typename CT>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    typedef double2 type;
+};
+template <>
+struct Cplx<float> {
+    typedef float2 type;
+};
+
+template <typename TI, typename TO, typename CT = double>
 __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
                                                const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
-                                               const double *__restrict__ blocks, const double2 *__restrict__ G,
+                                               const CT *__restrict__ blocks, const double2 *__restrict__ G,
                                                const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
                                                int ne_max, const TO *__restrict__ Zsub, int in_cell,
                                                int out_cell, const int *__restrict__ ket_item_ptr,
@@ -345,9 +359,10 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     const int k0 = blockIdx.y * kc, nk = min(kc, r - k0);
     const int e0 = part_ptr[a], ne = part_ptr[a + 1] - e0;
     const int NpNp = Np * Np;
-    double *Bs = smem;                                                        // ne * Np * Np
-    double2 *xs = reinterpret_cast<double2 *>(smem + ((ne_max * NpNp + 1) & ~1));  // ne * Np * nk
-    const double *Bg = blocks + (size_t)e0 * NpNp;
+    typedef typename Cplx<CT>::type CT2;
+    CT *Bs = reinterpret_cast<CT *>(smem);                                    // ne * Np * Np
+    CT2 *xs = reinterpret_cast<CT2 *>(Bs + ((ne_max * NpNp + 1) & ~1));       // ne * Np * nk
+    const CT *Bg = blocks + (size_t)e0 * NpNp;
     for (int i = threadIdx.x; i < ne * NpNp; i += blockDim.x) Bs[i] = Bg[i];
     const int tile = Np * nk;
     for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
@@ -381,27 +396,28 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
                 xr = tr;
             }
         }
-        xs[idx] = make_double2(xr, xi);
+        xs[idx].x = (CT)xr;
+        xs[idx].y = (CT)xi;
     }
     __syncthreads();
     for (int o = threadIdx.x; o < tile; o += blockDim.x) {
         const int p = o / nk, kk = o - p * nk, k = k0 + kk;
-        double sr = 0.0, si = 0.0;
+        CT sr = 0, si = 0;
         for (int e = 0; e < ne; ++e) {
-            const double *B = Bs + (e * Np + p) * Np;
-            const double2 *xe = xs + e * tile + kk;
+            const CT *B = Bs + (e * Np + p) * Np;
+            const CT2 *xe = xs + e * tile + kk;
 #pragma unroll 6
             for (int pp = 0; pp < Np; ++pp) {
-                const double b = B[pp];
-                const double2 x = xe[pp * nk];
+                const CT b = B[pp];
+                const CT2 x = xe[pp * nk];
                 sr += b * x.x;
                 si += b * x.y;
             }
         }
         const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
         if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
-            sr = (double)Zsub[dst].x - sr;
-            si = (double)Zsub[dst].y - si;
+            sr = (CT)Zsub[dst].x - sr;
+            si = (CT)Zsub[dst].y - si;
         }
         Y[dst].x = sr;
         Y[dst].y = si;
@@ -410,18 +426,19 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
 // (every chunk stages all partner blocks, so a narrow remainder chunk costs a full one).
-template <typename TI, typename TO>
+template <typename TI, typename TO, typename CT = double>
 static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
-                         const double *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
+                         const CT *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
                          int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
                          double in_scale, double2 *zero_buf = nullptr, int zero_n = 0) {
     const int nch = (r + 23) / 24;
     const int kc = (r + nch - 1) / nch;
-    const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(double) + (size_t)ne_max * Np * kc * sizeof(double2);
+    const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(CT) +
+                       (size_t)ne_max * Np * kc * sizeof(typename Cplx<CT>::type);
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
-    hipLaunchKernelGGL((k_block<TI, TO>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na, Nq,
-                       Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n);
+    hipLaunchKernelGGL((k_block<TI, TO, CT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
+                       Nq, Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n);
 }
 
 // ------------------------------------------------------------------ gather SpMM
@@ -1094,6 +1111,7 @@ struct RowOperator {
     const DeviceCSC *P;
     DevBuf<int> c_ptr, c_alpha, vc_ptr, vc_alpha;
     DevBuf<double> c_blocks, vc_blocks;
+    DevBuf<float> c_blocks32, vc_blocks32;   // FP32 copies for the block ops on FP32 vectors
     DevBuf<double2> G;
     DevBuf<double2> S, U;      // FP64 scratch (dim * r)
     DevBuf<float2> S32, U32;   // FP32 scratch
@@ -1237,6 +1255,9 @@ struct RowOperator {
         up(vc_ptr, p2);
         up(vc_alpha, a2);
         up(vc_blocks, b2);
+        std::vector<float> f1(b1.begin(), b1.end()), f2(b2.begin(), b2.end());
+        up(c_blocks32, f1);
+        up(vc_blocks32, f2);
     }
 
     // Device buffer for the resolvent of nE energies (contents written by the caller).
@@ -1359,9 +1380,10 @@ struct RowOperator {
                         if (fp32) {
                             launch_cellmm<float>(s, &cv, ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                                  reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
-                            launch_block<float2, float2>(s, reinterpret_cast<const float2 *>(upart.p), U32.p, vc_ptr.p,
-                                                         vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p, Na, Nq,
-                                                         Np, r, nullptr, 1, 0, ci.ket_item_ptr.p, 2.0);
+                            launch_block<float2, float2, float>(s, reinterpret_cast<const float2 *>(upart.p), U32.p,
+                                                                vc_ptr.p, vc_alpha.p, vc_blocks32.p, vc_npart_max,
+                                                                nullptr, gidx.p, Na, Nq, Np, r, nullptr, 1, 0,
+                                                                ci.ket_item_ptr.p, 2.0);
                         } else {
                             launch_cellmm<double>(s, &cv, ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
                                                   upart.p, R2, 2.0, un);
@@ -1494,9 +1516,16 @@ struct RowOperator {
             }
         } else {
             ensure_f32();
-            launch_block<VT, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
-                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
-                                          zb, zn);
+            // FP32 vectors stage and sum the block ops in FP32 too (FP32 block copies); FP64
+            // vectors keep FP64 block ops around the FP32 product
+            if constexpr (v64)
+                launch_block<VT, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
+                                         use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
+                                         zb, zn);
+            else
+                launch_block<VT, float2, float>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks32.p, c_npart_max,
+                                                use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
+                                                1.0, zb, zn);
             if (timed) record(e1);
             if (cells)
                 launch_cellmm<float>(s, &cv, *ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
@@ -1506,8 +1535,12 @@ struct RowOperator {
                                    reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
             if (timed) record(e2);
             const float2 *Uin = cells ? reinterpret_cast<const float2 *>(upart.p) : U32.p;
-            launch_block<float2, VT>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
-                                     Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
+            if constexpr (v64)
+                launch_block<float2, VT>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
+                                         Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
+            else
+                launch_block<float2, VT, float>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks32.p, vc_npart_max, nullptr,
+                                                gidx.p, Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
             ++n_apply;
             ++n_apply32;
         }
