@@ -27,10 +27,10 @@ class Case:
     pass
 
 
-def _setup(ctx, rp, pot, T, oracle_p123=None):
+def _setup(ctx, rp, pot, T, oracle_p123=None, channel=0):
     c = Case()
     c.prob = tt.Problem(rp, pot)
-    c.ch = c.prob.channel(0, T)
+    c.ch = c.prob.channel(channel, T)
     c.P = c.ch.build_p123(ctx)
     c.G = c.ch.resolvent(ctx)
     c.och = oracle_pw(c.ch)
@@ -211,6 +211,41 @@ def test_apply_c2(ctx, c2):
     assert np.abs(Ycsc - Yo).max() <= 1e-13 * np.abs(Yo).max()
     Y32 = c2.ch.apply_rows(ctx, c2.P, X, G=c2.G, g_index=g, fp32=True)
     assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
+
+
+@pytest.fixture(scope="module")
+def c3(ctx):
+    """A J^Pi = 3/2 channel with N_alpha = 47 at Np = Nq = 12: the 3-tile (MT = 3) MFMA path."""
+    rp, pot, T = synthetic_config(N=12, two_J_3N_max=3, T_lab=(10.0, 50.0))
+    prob = tt.Problem(rp, pot)
+    big = max(range(prob.n_channels), key=lambda i: prob.channel(i, T).Nalpha)
+    c = _setup(ctx, rp, pot, T, channel=big)
+    assert c.ch.Nalpha > 32
+    return c
+
+
+def test_apply_large_channel(ctx, c3):
+    """The cell-block product at N_alpha = 47 (3 M tiles) with 18 right-hand sides (48-wide N
+    tiles) against the oracle's row application: 1e-13 in FP64, 1e-5 with FP32 products."""
+    rng = np.random.default_rng(5)
+    r = 18
+    X = rng.normal(size=(r, c3.ch.dim)) + 1j * rng.normal(size=(r, c3.ch.dim))
+    g = [k % c3.ch.nE for k in range(r)]
+    Gsel = np.stack([c3.G[k] for k in g])
+    Yo = c3.K.row_apply(X * Gsel)
+    Y = c3.ch.apply_rows(ctx, c3.P, X, G=c3.G, g_index=g)
+    assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max()
+    Y32 = c3.ch.apply_rows(ctx, c3.P, X, G=c3.G, g_index=g, fp32=True)
+    assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max()
+
+
+def test_mixed_solve_large_channel(ctx, c3):
+    """Iterative refinement (FP32 products and basis) on the N_alpha = 47 channel reaches the
+    FP64 GMRES answer to 1e-10."""
+    U64, _ = c3.ch.solve(ctx, c3.P, c3.G, method=tt.SOLVE_GMRES)
+    Umx, st = c3.ch.solve(ctx, c3.P, c3.G, method=tt.SOLVE_MIXED)
+    assert st.n_apply_fp32 > 0
+    assert np.abs(Umx - U64).max() <= 1e-10 * np.abs(U64).max()
 
 
 def test_solves_c2_against_oracle_gmres(ctx, c2):
