@@ -1336,7 +1336,7 @@ struct RowOperator {
     // Autotuner of the cell-block product: on the first application of a shape (precision, r),
     // each candidate item length and batch depth is timed, together with the block op that
     // sums the work items' partial panels, on this operator's own blocks (a
-    // warm-up launch, then the mean of three launches between events, the stream otherwise
+    // warm-up launch, then the mean of three launches between events, the device otherwise
     // idle) and the fastest kept.  Choices are shared through a process-wide table keyed by
     // device, orientation, precision, Nalpha, R2 and the cell count, so channels of equal shape
     // tune once.  Runs eagerly from prepare(), never inside a graph capture.
@@ -1364,6 +1364,8 @@ struct RowOperator {
             upart.alloc(need);
             ++layout_gen;
         }
+        // the trials run alone on the device (other streams of a lockstep group would share it)
+        TT_HIP(hipDeviceSynchronize());
         hipEvent_t e0, e1;
         TT_HIP(hipEventCreate(&e0));
         TT_HIP(hipEventCreate(&e1));
@@ -2294,7 +2296,13 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
     const int m = o.restart > 0 ? o.restart : 40;
     const bool graphs = o.use_graphs != 0;
     const size_t n = ms.size();
-    hipStream_t s = ms[0].pl->op.s;
+    // the members' streams (one, or the context's lanes): every host wait covers all of them
+    std::vector<hipStream_t> streams;
+    for (const auto &mb : ms)
+        if (std::find(streams.begin(), streams.end(), mb.pl->op.s) == streams.end()) streams.push_back(mb.pl->op.s);
+    auto sync_all = [&]() {
+        for (hipStream_t st : streams) TT_HIP(hipStreamSynchronize(st));
+    };
     std::vector<Krylov<double2> *> k64(n, nullptr);
     std::vector<Krylov<float2> *> k32(n, nullptr);
     auto enqueue_norms = [&](size_t i, const double2 *v) {
@@ -2307,13 +2315,12 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
     for (size_t i = 0; i < n; ++i) {
         KMember &mb = ms[i];
         SolvePlan &pl = *mb.pl;
-        TT_REQUIRE(pl.op.s == s, "lockstep members share one stream");
         mb.L = onshell_layout(*mb.os, pl.op.Nq, pl.op.Np);
         const int r = mb.L.r;
         pl.op.set_gidx(mb.L.row_g);
         pl.fit(r);
         on_shell_rows_of_A(pl.op, mb.L, pl.b.p, pl.rres.p, &pl.unit_idx);
-        TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * (size_t)pl.op.dim * r, s));
+        TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * (size_t)pl.op.dim * r, pl.op.s));
         if (mb.mixed)
             k32[i] = &pl.krylov<float2>(r, m, graphs);
         else
@@ -2323,7 +2330,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
         mb.iters = 0;
         mb.done = false;
     }
-    TT_HIP(hipStreamSynchronize(s));
+    sync_all();
     for (size_t i = 0; i < n; ++i) ms[i].bnorm = read_norms(i);
     auto rel_max = [](const std::vector<double> &a, const std::vector<double> &ref) {
         double v = 0.0;
@@ -2344,11 +2351,11 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
                 SolvePlan &pl = *ms[i].pl;
                 const int64_t ne = pl.op.dim * ms[i].L.r;
                 pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);
-                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, s, pl.rres.p, pl.b.p, pl.rres.p, 1.0,
-                                   -1.0, ne);
+                hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, pl.op.s, pl.rres.p, pl.b.p, pl.rres.p,
+                                   1.0, -1.0, ne);
                 enqueue_norms(i, pl.rres.p);
             }
-            TT_HIP(hipStreamSynchronize(s));
+            sync_all();
             for (size_t i : act) beta[i] = read_norms(i);
         }
         std::vector<KrylovBase *> ks;
@@ -2376,7 +2383,42 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
             if (!ms[i].done) ms[i].iters += ms[i].mixed ? k32[i]->used : k64[i]->used;
     }
     for (auto &mb : ms) mb.pl->enqueue_onshell(mb.pl->x.p, mb.L, *mb.os);
-    TT_HIP(hipStreamSynchronize(s));
+    sync_all();
+}
+
+// solve_lockstep with the members spread over the context's stream lanes (round robin), so that
+// independent channels' kernels overlap on the device.  Every lane first waits for the work
+// queued on the context stream (G, the set-up); afterwards the lanes are drained and the plans
+// go back to the context stream, also when the solve throws.
+constexpr int kLanes = 4;
+static void solve_on_lanes(tt_ctx *ctx, std::vector<KMember> &ms, const tt_solve_opts &o) {
+    const int L = std::min<int>((int)ms.size(), kLanes);
+    if (L <= 1) {
+        solve_lockstep(ms, o);
+        return;
+    }
+    while ((int)ctx->lanes.size() < L) {
+        hipStream_t st = nullptr;
+        TT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ctx->lanes.push_back(st);
+    }
+    hipEvent_t ready = nullptr;
+    TT_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    struct Restore {
+        tt_ctx *c;
+        std::vector<KMember> *m;
+        hipEvent_t ev;
+        int L;
+        ~Restore() {
+            for (int l = 0; l < L; ++l) (void)hipStreamSynchronize(c->lanes[l]);
+            for (auto &mb : *m) mb.pl->op.s = c->stream;
+            (void)hipEventDestroy(ev);
+        }
+    } restore{ctx, &ms, ready, L};
+    TT_HIP(hipEventRecord(ready, ctx->stream));
+    for (int l = 0; l < L; ++l) TT_HIP(hipStreamWaitEvent(ctx->lanes[l], ready, 0));
+    for (size_t i = 0; i < ms.size(); ++i) ms[i].pl->op.s = ctx->lanes[i % L];
+    solve_lockstep(ms, o);
 }
 
 // Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
@@ -2489,8 +2531,10 @@ void tt_ctx_destroy(tt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t st : c->lanes) (void)hipStreamSynchronize(st);
     free_plan_cache(c->plans);
     c->plans = nullptr;
+    for (hipStream_t st : c->lanes) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2501,6 +2545,7 @@ tt_status tt_ctx_synchronize(tt_ctx *c) {
     TT_GUARD_BEGIN
     TT_REQUIRE(c, "null context");
     TT_HIP(hipStreamSynchronize(c->stream));
+    for (hipStream_t st : c->lanes) TT_HIP(hipStreamSynchronize(st));
     TT_GUARD_END
 }
 
@@ -2764,7 +2809,7 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
         }
         i0 = i;
         try {
-            solve_lockstep(ms, o);
+            solve_on_lanes(ctx, ms, o);
         } catch (const Error &e) {
             for (int k : idx) record(k, e.code, e.msg);
             continue;
@@ -2956,7 +3001,7 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     ms[0].os = &os;
     ms[1].pl = &colp;   // (1 - A G) t_k = A e_k
     ms[1].os = &os;
-    solve_lockstep(ms, o);
+    solve_on_lanes(ctx, ms, o);
     {
         std::vector<cdouble> Uc((size_t)n);
         rowp.read_onshell(ms[0].L, Uc.data());

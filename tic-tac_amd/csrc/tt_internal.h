@@ -173,4 +173,7 @@ struct tt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     tt::PlanCache *plans = nullptr;
+    // extra streams (default priority) over which a lockstep group's channels are spread, so
+    // that independent channels' kernels overlap; created on first use
+    std::vector<hipStream_t> lanes;
 };
