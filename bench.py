@@ -28,6 +28,7 @@ import numpy as np  # noqa: E402
 import tictac as tt  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+LANES = 4   # streams a batched lockstep group is spread over (the library's kLanes)
 METRIC = "U-matrix channel-solves/s batched-mixed vs plain FP64 @|ΔU/U|≤1e-8; grad/fwd time"
 SOLVE_TOL = 1e-12
 BATCH_T = [float(t) for t in np.linspace(1.0, 200.0, 8)]
@@ -96,24 +97,40 @@ def time_batches(ctx, wl, method, steps, warmup):
     return time.perf_counter() - t0, [u for (u, _, _) in out], stats
 
 
-def product_roofline(stats, kernel, fp32=False, traffic=None):
+def product_roofline(stats, kernel, fp32=False, traffic=None, wall_s=None, lanes=1):
     """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
     algorithmic bytes per launch (every block value once, the S panel read and the U panel
     written once, the cell indexes) over the launch time from the HIP events that bracket each
-    launch on the solve stream during the timed steps; traffic = measured HBM bytes per launch."""
+    sampled launch on its stream during the timed steps; traffic = measured HBM bytes per launch.
+    With lanes > 1 the channels' streams run concurrently, so a launch shares the device with
+    the other lanes' kernels and its event time is its duration under that sharing.  With
+    wall_s, aggregate_achieved = the algorithmic bytes of every product launch of the timed
+    steps (sampled or not) over their wall time: a lower bound on the products' HBM rate."""
     if fp32:   # only the FP32 launches of the mixed solves (its FP64 residual applications are apart)
         ms = sum(s.ms_spmm_fp32 for s in stats)
         nb = sum(s.spmm_bytes_fp32 for s in stats)
         n = sum(s.n_spmm_fp32 for s in stats)
+        every = sum(s.n_apply_fp32 * s.spmm_bytes_fp32 / s.n_spmm_fp32 for s in stats if s.n_spmm_fp32)
     else:
         ms = sum(s.ms_spmm for s in stats)
         nb = sum(s.spmm_bytes for s in stats)
         n = sum(s.n_spmm for s in stats)
+        every = sum((s.n_apply - s.n_apply_fp32) * (s.spmm_bytes - s.spmm_bytes_fp32) / (s.n_spmm - s.n_spmm_fp32)
+                    for s in stats if s.n_spmm > s.n_spmm_fp32)
     achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
-            "bytes_per_launch": round(nb / max(n, 1))}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+           "bytes_per_launch": round(nb / max(n, 1)), "concurrent_streams": lanes}
+    if wall_s:
+        out["aggregate_achieved"] = round(every / wall_s / 1e9, 1)
+    return out
+
+
+def iteration_summary(stats, n_items):
+    """Krylov steps per channel in the last timed batch: mean and max."""
+    last = [s.iterations for s in stats[-n_items:]]
+    return {"mean": round(sum(last) / max(len(last), 1), 1), "max": max(last) if last else 0}
 
 
 TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
@@ -362,9 +379,11 @@ def main():
         "plain_fp64": None,
         "mixed_over_plain_fp64": None,
         "grad_over_fwd": None,
-        "roofline": product_roofline(st_mix, k32, True, measured_traffic(k32)),
+        "roofline": product_roofline(st_mix, k32, True, measured_traffic(k32), t_mix, LANES),
         "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, MT, NT, UN>", False,
-                                          measured_traffic("k_cellmfma<double, MT, NT, UN>")),
+                                          measured_traffic("k_cellmfma<double, MT, NT, UN>"), t_64, LANES),
+        "krylov_steps_per_channel": {"mixed": iteration_summary(st_mix, len(wl.chans)),
+                                     "fp64": iteration_summary(st_64, len(wl.chans))},
         "cell_autotune": tt.cell_tuning(),
     }
     U_plain = None
