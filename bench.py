@@ -299,6 +299,30 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=20.0):
             "max_rel_vs_gpu_plain_fp64": diff}
 
 
+WALK_WORKLOAD = ("same 18 channels x 8 energies for 8 parameter sets (the 12 strengths scaled by 0.99 .. 1.02): "
+                 "per set V_WP, SWP states, on-shell bins and G rebuilt, then one lockstep batch (tt_parameter_walk)")
+
+
+def parameter_walk(ctx, wl, n_sets=8):
+    """The reference's parameter walk on the batched workload: channel-solves/s over n_sets
+    parameter sets, the host rebuild of each set included (overlapped with the previous set's
+    solve), mixed precision to the same tolerance."""
+    th = tt.synthetic_params(7)
+    sets = []
+    for i in range(n_sets):
+        p = th.copy()
+        p[:12] *= 0.99 + 0.03 * i / max(n_sets - 1, 1)
+        sets.append(p)
+    chans = list(range(len(wl.chans)))
+    wl.prob.parameter_walk(ctx, chans, wl.Ps, sets[:2], BATCH_T, method=tt.SOLVE_MIXED, tol=SOLVE_TOL)  # warm-up
+    t0 = time.perf_counter()
+    U, status, code = wl.prob.parameter_walk(ctx, chans, wl.Ps, sets, BATCH_T, method=tt.SOLVE_MIXED, tol=SOLVE_TOL)
+    dt = time.perf_counter() - t0
+    ok = int((status == 0).sum())
+    return {"workload": WALK_WORKLOAD, "unit": "channel-solves/s", "value": round(ok * len(BATCH_T) / dt, 1),
+            "ms_per_set": round(1e3 * dt / n_sets, 2), "items_failed": int(status.size - ok)}
+
+
 def copy_calibration(mib=256, reps=3):
     """A plain device-to-device copy of known size (the HIP runtime's copy kernel behind
     hipMemcpy, mib MiB each way, reps launches): the reference point that scales rocprofv3's
@@ -324,7 +348,7 @@ def copy_calibration(mib=256, reps=3):
     return {"bytes_each_way": n, "launches": reps}
 
 
-LEGS = ("plain", "single", "n40", "grad", "cpu", "copy")
+LEGS = ("plain", "walk", "single", "n40", "grad", "cpu", "copy")
 
 
 def main():
@@ -403,6 +427,9 @@ def main():
         out["mixed_vs_plain_fp64_max_rel_norm"] = norm
         out["batched_fp64_vs_plain_fp64_max_rel_elem"] = elem64
         out["accuracy_target_met"] = bool(elem <= 1e-8)
+    if "walk" in legs:
+        log("parameter walk")
+        out["parameter_walk"] = parameter_walk(ctx, wl)
     if "single" in legs:
         log("single channel 1/2+")
         out["single_channel"] = single_channel(ctx, 20, 3)

@@ -285,6 +285,26 @@ tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_u
                          const double *V_coup, const tt_swp_states *swp, tt_batch_item *items,
                          int n_items, const tt_solve_opts *opts);
 
+/* Parameter walk: the PSI loop of CPP/main.cpp:126-136,254-466 (parameter_walk=true, parameter
+ * sets read by read_parameter_sample_list, CPP/disk_io_routines.cpp:43-78: one set of n_params values per
+ * row of params, n_sets rows) for the listed 3N channels at once.  For every set: the
+ * potential's parameters are set (update_parameters), V_WP, the SWP states, each channel's
+ * on-shell bins of T_lab and G are rebuilt, and all channels are solved in one lockstep batch
+ * (as tt_solve_batch, with opts).  The host work of set s + 1 runs while the device solves set
+ * s.  channels: indices into pw_full's 3N channels, P123[c] the P123 of channels[c] (parameter
+ * independent, built once).  U: n_sets * n_chn blocks of `capacity` complex values, block
+ * (s, c) at complex offset (s * n_chn + c) * capacity in the layout of
+ * tt_solve_faddeev_equations; n_out / status: n_sets * n_chn entries (elements written, and
+ * each item's status).  A set whose SWP states fail (e.g. no single 3S1 bound state) gets that
+ * status on all its items; the walk goes on.  Returns the first failure.  The potential keeps
+ * the last set's parameters.  No reference counterpart as one call: it replaces the PSI loop
+ * around solve_faddeev_equations. */
+tt_status tt_parameter_walk(tt_ctx *ctx, const tt_run_params *rp, const tt_fwp_states *fwp,
+                            const tt_pw_states *pw_full, tt_potential *pot, const double *params,
+                            int n_params, int n_sets, const int *channels, const tt_csc *const *P123,
+                            int n_chn, const double *T_lab, int n_T_lab, double *U, int capacity,
+                            int *n_out, tt_status *status, const tt_solve_opts *opts);
+
 /* ------------------------------------------------------------ derivatives
  * No reference counterpart (the reference has no derivatives).  For a potential that is
  * linear in its first n_theta parameters (the "synthetic" model: its 12 strengths, with the

@@ -450,3 +450,33 @@ def test_error_paths_on_device(ctx, c1):
     with pytest.raises(tt.TictacError) as e:
         c1.ch.solve(ctx, bad, c1.G)
     assert e.value.code == tt._lib.TT_ERR_ARG
+
+
+def test_parameter_walk_matches_per_set_solves(ctx):
+    """tt_parameter_walk (the reference's PSI loop for several channels at once, each set one
+    lockstep batch, the next set's host work overlapped) gives, per parameter set and channel,
+    the U of a separate per-set Problem solved channel by channel (FP64 GMRES: 1e-10; mixed:
+    1e-8).  A set without the single 3S1 bound state fails alone, with TT_ERR_PHYSICS on its
+    items, and the walk goes on."""
+    rp, pot, T = synthetic_config(N=12, two_J_3N_max=3, T_lab=(10.0, 50.0))
+    th = tt.synthetic_params()
+    sets = []
+    for scale in (1.0, 1.02, 0.3, 0.99):
+        p = th.copy()
+        p[:12] *= scale
+        sets.append(p)
+    sets = np.array(sets)
+    prob = tt.Problem(rp, tt.Potential("synthetic", th))
+    chans = [0, 2]
+    Ps = [prob.channel(c, T).build_p123(ctx) for c in chans]
+    for method, tol in ((tt.SOLVE_GMRES, 1e-10), (tt.SOLVE_MIXED, 1e-8)):
+        U, status, code = prob.parameter_walk(ctx, chans, Ps, sets, T, method=method)
+        assert code == tt._lib.TT_ERR_PHYSICS
+        assert (status[2] == tt._lib.TT_ERR_PHYSICS).all() and U[2][0] is None
+        for s in (0, 1, 3):
+            assert (status[s] == tt._lib.TT_OK).all()
+            ref = tt.Problem(rp, tt.Potential("synthetic", sets[s]))
+            for k, c in enumerate(chans):
+                Ur, _ = ref.channel(c, T).solve(ctx, Ps[k], None, method=tt.SOLVE_GMRES, device_resolvent=True)
+                assert U[s][k].shape == Ur.shape
+                assert np.abs(U[s][k] - Ur).max() <= tol * np.abs(Ur).max(), (method, s, c)

@@ -156,6 +156,43 @@ class Problem:
         return [(U.view(np.complex128).reshape(ch.nd, ch.nd, ch.nE), st, items[i].status)
                 for i, (U, st, ch) in enumerate(zip(Us, stats, channels))]
 
+    def parameter_walk(self, ctx, channels, P123s, param_sets, T_lab, method=L.SOLVE_MIXED, tol=1e-12,
+                       max_iter=2000, restart=40):
+        """tt_parameter_walk: the on-shell U of the listed channels (indices into the 3N channels,
+        with their P123) for every parameter set (rows of param_sets), each set's channels
+        solved as one lockstep batch.  Uses this problem's potential and gives it back its own
+        parameters afterwards.  Returns (U, status, code): U[s][c] complex [nd, nd, nE] (None
+        where the item failed), the status codes [n_sets, n_chn] and the call's status."""
+        sets = np.ascontiguousarray(np.atleast_2d(param_sets), np.float64)
+        n_sets, n_params = sets.shape
+        chans = np.ascontiguousarray(channels, np.int32)
+        n_chn = len(chans)
+        T = np.ascontiguousarray(T_lab, np.float64)
+        cap = 3 * 3 * 2 * (self.rp.Nq_WP + 1)
+        U = np.zeros(2 * cap * n_sets * n_chn)
+        n_out = np.zeros(n_sets * n_chn, np.int32)
+        status = np.zeros(n_sets * n_chn, np.int32)
+        Parr = (C.POINTER(L.CSC) * n_chn)(*[C.pointer(P.csc) for P in P123s])
+        o = L.SolveOpts()
+        L.lib().tt_solve_opts_default(C.byref(o))
+        o.method, o.tol, o.max_iter, o.restart = method, tol, max_iter, restart
+        try:
+            code = L.lib().tt_parameter_walk(ctx.handle, C.byref(self.rp), C.byref(self.fwp), C.byref(self.pw),
+                                             self.pot._h, dptr(sets), n_params, n_sets, iptr(chans), Parr, n_chn,
+                                             dptr(T), len(T), dptr(U), cap, iptr(n_out), iptr(status), C.byref(o))
+        finally:
+            self.pot.update(self.pot.params)
+        nds = [self.channel(int(c), T).nd for c in chans]
+        Uc = U.view(np.complex128).reshape(n_sets, n_chn, cap)
+        out = [[None] * n_chn for _ in range(n_sets)]
+        for s_ in range(n_sets):
+            for c in range(n_chn):
+                k = s_ * n_chn + c
+                if status[k] == L.TT_OK and n_out[k] > 0:
+                    nd = nds[c]
+                    out[s_][c] = Uc[s_, c, :n_out[k]].reshape(nd, nd, n_out[k] // (nd * nd)).copy()
+        return out, status.reshape(n_sets, n_chn), code
+
     def basis_matrices(self, n_theta=12):
         """V_i = V_WP(theta = e_i) for the potential's linear strengths (tt_potential_basis_matrices):
         arrays (n_theta, nu*Np*Np) and (n_theta, nc*4*Np*Np)."""
