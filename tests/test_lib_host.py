@@ -137,3 +137,29 @@ def test_synthetic_potential_is_linear_in_its_strengths():
     assert np.abs(np.tensordot(th, Vb_u, 1) - Vu).max() <= 1e-14 * np.abs(Vu).max()
     assert np.abs(np.tensordot(th, Vb_c, 1) - Vc).max() <= 1e-14 * np.abs(Vc).max()
     assert np.array_equal(prob.pot.params, tt.synthetic_params())   # restored
+
+
+_V_SCRIPT = """
+import sys, numpy as np
+sys.path[:0] = sys.argv[2:]
+import tictac as tt
+from helpers import synthetic_config
+rp, pot, _ = synthetic_config(N=12, J_2N_max=3, two_J_3N_max=1)
+prob = tt.Problem(rp, pot)
+np.save(sys.argv[1], np.concatenate([prob.V_unco, prob.V_coup]))
+"""
+
+
+def test_potential_matrices_do_not_depend_on_the_thread_count(tmp_path):
+    """V_WP rows are built on several host threads; each element is summed in one fixed order,
+    so one thread and eight give the same bits."""
+    import subprocess
+    import sys
+    out = {}
+    for n in (1, 8):
+        f = tmp_path / ("v%d.npy" % n)
+        env = dict(os.environ, TT_HOST_THREADS=str(n))
+        subprocess.run([sys.executable, "-c", _V_SCRIPT, str(f), os.path.join(ROOT, "tic-tac_amd"),
+                        os.path.join(ROOT, "tests")], env=env, check=True, timeout=120)
+        out[n] = np.load(f)
+    assert out[1].size > 0 and np.array_equal(out[1], out[8])

@@ -37,6 +37,7 @@ static double yukawa_pw(int L, double p_out, double p_in, double mu) {
     return legendre_Q_from_q0(L, z, q0) / (M_PI * p_out * p_in);
 }
 
+// V() is called from several host threads at once (the V_WP rows): it must not write state.
 struct Potential {
     virtual ~Potential() = default;
     virtual void update(const double *p, int n) = 0;
@@ -188,6 +189,12 @@ tt_status tt_calculate_potential_matrices(const tt_run_params *rp, const tt_fwp_
     const int Np = fwp->Np_WP, npw = rp->midpoint_approx ? 1 : fwp->Np_per_WP;
     std::vector<char> done_u(nu, 0), done_c(nc, 0);
     const Potential &P = *pot->impl;
+    // One task per unique 2N block: the (alpha, alpha') pair that first reaches it.
+    struct Block {
+        int Lr, Lc, Sr, Jr, Tr, iso, idx;
+        bool via_L, via_T, cm, coupled_model;
+    };
+    std::vector<Block> blocks;
     for (int ar = 0; ar < pw->Nalpha; ++ar) {
         int Lr = pw->L_2N[ar], Sr = pw->S_2N[ar], Jr = pw->J_2N[ar], Tr = pw->T_2N[ar], t3r = pw->two_T_3N[ar];
         for (int ac = 0; ac < pw->Nalpha; ++ac) {
@@ -220,74 +227,80 @@ tt_status tt_calculate_potential_matrices(const tt_run_params *rp, const tt_fwp_
                 if (done_u[idx]) continue;
                 done_u[idx] = 1;
             }
-            for (int br = 0; br < Np; ++br) {
-                double Nr = fwp->norm_p[br];
-                for (int bc = 0; bc < Np; ++bc) {
-                    double Nc = fwp->norm_p[bc];
-                    double acc[6] = {0, 0, 0, 0, 0, 0};
-                    for (int ir = 0; ir < npw; ++ir) {
-                        double pr, fr;
-                        if (rp->midpoint_approx) {
-                            pr = fwp->p[br];
-                            fr = pr * (fwp->p_WP[br + 1] - fwp->p_WP[br]);
-                        } else {
-                            pr = fwp->p[br * npw + ir];
-                            fr = fwp->wp[br * npw + ir] * pr;
-                        }
-                        for (int ic = 0; ic < npw; ++ic) {
-                            double pc, fc;
-                            if (rp->midpoint_approx) {
-                                pc = fwp->p[bc];
-                                fc = pc * (fwp->p_WP[bc + 1] - fwp->p_WP[bc]);
-                            } else {
-                                pc = fwp->p[bc * npw + ic];
-                                fc = fwp->wp[bc * npw + ic] * pc;
-                            }
-                            double vis[6] = {0, 0, 0, 0, 0, 0}, vnn[6], vnp[6];
-                            if (!via_T && iso == 0) {
-                                P.V(pc, pr, coupled_model, Sr, Jr, Tr, 0, vis);
-                            } else if (!via_T && iso == 1) {
-                                P.V(pc, pr, coupled_model, Sr, Jr, Tr, -1, vnn);
-                                P.V(pc, pr, coupled_model, Sr, Jr, Tr, 0, vnp);
-                                for (int k = 0; k < 6; ++k) vis[k] = (2. / 3) * vnn[k] + (1. / 3) * vnp[k];
-                            } else if (via_T) {
-                                P.V(pc, pr, coupled_model, Sr, Jr, Tr, -1, vnn);
-                                P.V(pc, pr, coupled_model, Sr, Jr, Tr, 0, vnp);
-                                double enn = extract_element(Lr, Lc, Jr, Sr, false, vnn);
-                                double enp = extract_element(Lr, Lc, Jr, Sr, false, vnp);
-                                vis[2] = (2. / 3) * enn + (1. / 3) * enp;
-                                vis[3] = (std::sqrt(2.0) / 3.) * (enn - enp);
-                                vis[4] = (std::sqrt(2.0) / 3.) * (enn - enp);
-                                vis[5] = (1. / 3) * enn + (2. / 3) * enp;
-                            } else {
-                                fail(TT_ERR_PHYSICS, "Unknown isoscalar potential encountered.");
-                            }
-                            double fac = fr * fc;
-                            for (int k = 0; k < 6; ++k) acc[k] += fac * vis[k] / (Nr * Nc);
-                        }
-                    }
-                    if (cm) {
-                        double *M = V_coup + (size_t)idx * 4 * Np * Np;
-                        size_t ul = (size_t)br * 2 * Np + bc, ur = ul + Np, ll = (size_t)(br + Np) * 2 * Np + bc,
-                               lr = ll + Np;
-                        if (via_L) {
-                            M[ul] = extract_element(Jr - 1, Jr - 1, Jr, Sr, true, acc);
-                            M[ur] = extract_element(Jr - 1, Jr + 1, Jr, Sr, true, acc);
-                            M[ll] = extract_element(Jr + 1, Jr - 1, Jr, Sr, true, acc);
-                            M[lr] = extract_element(Jr + 1, Jr + 1, Jr, Sr, true, acc);
-                        } else {
-                            M[ul] = acc[2];
-                            M[ur] = acc[3];
-                            M[ll] = acc[4];
-                            M[lr] = acc[5];
-                        }
-                    } else {
-                        V_unco[(size_t)idx * Np * Np + (size_t)br * Np + bc] = extract_element(Lr, Lc, Jr, Sr, false, acc);
-                    }
-                }
-            }
+            if (!via_T && iso != 0 && iso != 1) fail(TT_ERR_PHYSICS, "Unknown isoscalar potential encountered.");
+            blocks.push_back({Lr, Lc, Sr, Jr, Tr, iso, idx, via_L, via_T, cm, coupled_model});
         }
     }
+    // Row br of block b: Np elements, each a npw x npw quadrature over the two bins.
+    auto row = [&](const Block &B, int br) {
+        const double Nr = fwp->norm_p[br];
+        for (int bc = 0; bc < Np; ++bc) {
+            double Nc = fwp->norm_p[bc];
+            double acc[6] = {0, 0, 0, 0, 0, 0};
+            for (int ir = 0; ir < npw; ++ir) {
+                double pr, fr;
+                if (rp->midpoint_approx) {
+                    pr = fwp->p[br];
+                    fr = pr * (fwp->p_WP[br + 1] - fwp->p_WP[br]);
+                } else {
+                    pr = fwp->p[br * npw + ir];
+                    fr = fwp->wp[br * npw + ir] * pr;
+                }
+                for (int ic = 0; ic < npw; ++ic) {
+                    double pc, fc;
+                    if (rp->midpoint_approx) {
+                        pc = fwp->p[bc];
+                        fc = pc * (fwp->p_WP[bc + 1] - fwp->p_WP[bc]);
+                    } else {
+                        pc = fwp->p[bc * npw + ic];
+                        fc = fwp->wp[bc * npw + ic] * pc;
+                    }
+                    double vis[6] = {0, 0, 0, 0, 0, 0}, vnn[6], vnp[6];
+                    if (!B.via_T && B.iso == 0) {
+                        P.V(pc, pr, B.coupled_model, B.Sr, B.Jr, B.Tr, 0, vis);
+                    } else if (!B.via_T) {
+                        P.V(pc, pr, B.coupled_model, B.Sr, B.Jr, B.Tr, -1, vnn);
+                        P.V(pc, pr, B.coupled_model, B.Sr, B.Jr, B.Tr, 0, vnp);
+                        for (int k = 0; k < 6; ++k) vis[k] = (2. / 3) * vnn[k] + (1. / 3) * vnp[k];
+                    } else {
+                        P.V(pc, pr, B.coupled_model, B.Sr, B.Jr, B.Tr, -1, vnn);
+                        P.V(pc, pr, B.coupled_model, B.Sr, B.Jr, B.Tr, 0, vnp);
+                        double enn = extract_element(B.Lr, B.Lc, B.Jr, B.Sr, false, vnn);
+                        double enp = extract_element(B.Lr, B.Lc, B.Jr, B.Sr, false, vnp);
+                        vis[2] = (2. / 3) * enn + (1. / 3) * enp;
+                        vis[3] = (std::sqrt(2.0) / 3.) * (enn - enp);
+                        vis[4] = (std::sqrt(2.0) / 3.) * (enn - enp);
+                        vis[5] = (1. / 3) * enn + (2. / 3) * enp;
+                    }
+                    double fac = fr * fc;
+                    for (int k = 0; k < 6; ++k) acc[k] += fac * vis[k] / (Nr * Nc);
+                }
+            }
+            if (B.cm) {
+                double *M = V_coup + (size_t)B.idx * 4 * Np * Np;
+                size_t ul = (size_t)br * 2 * Np + bc, ur = ul + Np, ll = (size_t)(br + Np) * 2 * Np + bc,
+                       lr = ll + Np;
+                const int Jr = B.Jr, Sr = B.Sr;
+                if (B.via_L) {
+                    M[ul] = extract_element(Jr - 1, Jr - 1, Jr, Sr, true, acc);
+                    M[ur] = extract_element(Jr - 1, Jr + 1, Jr, Sr, true, acc);
+                    M[ll] = extract_element(Jr + 1, Jr - 1, Jr, Sr, true, acc);
+                    M[lr] = extract_element(Jr + 1, Jr + 1, Jr, Sr, true, acc);
+                } else {
+                    M[ul] = acc[2];
+                    M[ur] = acc[3];
+                    M[ll] = acc[4];
+                    M[lr] = acc[5];
+                }
+            } else {
+                V_unco[(size_t)B.idx * Np * Np + (size_t)br * Np + bc] =
+                    extract_element(B.Lr, B.Lc, B.Jr, B.Sr, false, acc);
+            }
+        }
+    };
+    // Rows are independent and each element is summed in the same order as a serial pass, so
+    // the matrices do not depend on the thread count.
+    parallel_rows((int64_t)blocks.size() * Np, [&](int64_t r) { row(blocks[r / Np], (int)(r % Np)); });
     TT_GUARD_END
 }
 

@@ -18,6 +18,18 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 void fail(tt_status code, const std::string &msg) { throw Error{code, msg}; }
 
+int host_threads() {
+    static const int n = [] {
+        if (const char *e = std::getenv("TT_HOST_THREADS")) {
+            const int v = std::atoi(e);
+            if (v >= 1) return v;
+        }
+        const unsigned hw = std::thread::hardware_concurrency();
+        return (int)std::max(1u, std::min(16u, hw));
+    }();
+    return n;
+}
+
 // ----------------------------------------------------------------- kinematics
 static double deuteron_mass(double Eb) { return kMn + kMp + Eb; }
 

@@ -3,10 +3,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "../../include/tictac.h"
@@ -42,6 +47,52 @@ struct Error {
     do {                                                       \
         if (!(cond)) ::tt::fail(TT_ERR_ARG, std::string(msg)); \
     } while (0)
+
+// ------------------------------------------------------------ host threads
+// Worker threads for host-side tables: TT_HOST_THREADS if set (>= 1), else the hardware
+// threads, at most 16.
+int host_threads();
+
+// f(r) for every r in [0, n), rows handed out one at a time to up to host_threads() threads
+// (the caller's among them).  The first exception a row throws stops the hand-out and is
+// rethrown on the caller once every worker has returned.
+template <class F>
+void parallel_rows(int64_t n, F &&f) {
+    const int nt = (int)std::min<int64_t>(host_threads(), n);
+    if (nt <= 1) {
+        for (int64_t r = 0; r < n; ++r) f(r);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::atomic<bool> stop{false};
+    std::exception_ptr err;
+    std::mutex m;
+    auto work = [&]() {
+        try {
+            while (!stop.load(std::memory_order_relaxed)) {
+                const int64_t r = next.fetch_add(1);
+                if (r >= n) return;
+                f(r);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(m);
+            if (!err) err = std::current_exception();
+            stop = true;
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int i = 1; i < nt; ++i) {
+        try {
+            th.emplace_back(work);
+        } catch (const std::system_error &) {
+            break;   // fewer workers; the rows still all run
+        }
+    }
+    work();
+    for (auto &t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
 
 // -------------------------------------------------------------- kinematics
 double mu1(double Eb);
