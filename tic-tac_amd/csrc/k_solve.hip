@@ -1103,6 +1103,39 @@ struct ResolventStaging {
     DevBuf<int> which, offset;
 };
 
+// V.C of every 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array,
+// CPP/solve_faddeev.cpp:1828-1855): row-major Np x Np (uncoupled) and 2Np x 2Np (coupled).
+struct VCProducts {
+    uint64_t content = 0;   // plan_content_key of the V and C they were made from
+    bool valid = false;
+    std::vector<double> u, c;
+};
+static void vc_products(const tt_run_params &rp, const double *V_unco, const double *V_coup,
+                        const tt_swp_states &swp, VCProducts &out) {
+    int nu, nc;
+    num_2N_states(rp, nu, nc);
+    const int Np = swp.Np_WP;
+    out.u.assign((size_t)nu * Np * Np, 0.0);
+    out.c.assign((size_t)nc * 4 * Np * Np, 0.0);
+    auto mm = [](const double *A, const double *B, double *C, int n) {
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                const double a = A[(size_t)i * n + k];
+                if (a == 0.0) continue;
+                for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+            }
+    };
+    // one 2N channel per row: the products do not depend on the thread count
+    parallel_rows(nu + nc, [&](int64_t c) {
+        if (c < nu)
+            mm(V_unco + (size_t)c * Np * Np, swp.C_SWP_unco + (size_t)c * Np * Np, &out.u[(size_t)c * Np * Np], Np);
+        else
+            mm(V_coup + (size_t)(c - nu) * 4 * Np * Np, swp.C_SWP_coup + (size_t)(c - nu) * 4 * Np * Np,
+               &out.c[(size_t)(c - nu) * 4 * Np * Np], 2 * Np);
+    });
+    out.valid = true;
+}
+
 struct RowOperator {
     hipStream_t s;
     ResolventStaging rstage;
@@ -1178,7 +1211,7 @@ struct RowOperator {
 
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
                const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn,
-               int p123_format = TT_P123_CELLS, bool col_ = false) {
+               int p123_format = TT_P123_CELLS, bool col_ = false, const VCProducts *vc = nullptr) {
         s = stream;
         cells = p123_format == TT_P123_CELLS;
         col = col_;
@@ -1197,31 +1230,21 @@ struct RowOperator {
             cv = cells_of(*csc, col);
             items_of_len(CELL_ITEM);
         }
-        set_blocks(rp, V_unco, V_coup, swp, chn);
+        set_blocks(rp, V_unco, V_coup, swp, chn, vc);
     }
 
     // C_op / VC_op partner blocks from V and the SWP eigenvectors.  The partner lists are
     // structural (zero blocks kept), so new V or C (another theta) only rewrite the block
-    // contents in the same buffers and captured graphs stay valid.
+    // contents in the same buffers and captured graphs stay valid.  vc: the V.C products of
+    // these V and C when the caller has them (they are the same for every 3N channel).
     void set_blocks(const tt_run_params &rp, const double *V_unco, const double *V_coup, const tt_swp_states &swp,
-                    const tt_pw_states &chn) {
-        int nu, nc;
-        num_2N_states(rp, nu, nc);
-        // V.C per 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array, lines 1828-1855)
-        std::vector<double> VCu((size_t)nu * Np * Np, 0.0), VCc((size_t)nc * 4 * Np * Np, 0.0);
-        auto mm = [](const double *A, const double *B, double *C, int n) {
-            for (int i = 0; i < n; ++i)
-                for (int k = 0; k < n; ++k) {
-                    const double a = A[(size_t)i * n + k];
-                    if (a == 0.0) continue;
-                    for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
-                }
-        };
-        for (int c = 0; c < nu; ++c)
-            mm(V_unco + (size_t)c * Np * Np, swp.C_SWP_unco + (size_t)c * Np * Np, &VCu[(size_t)c * Np * Np], Np);
-        for (int c = 0; c < nc; ++c)
-            mm(V_coup + (size_t)c * 4 * Np * Np, swp.C_SWP_coup + (size_t)c * 4 * Np * Np,
-               &VCc[(size_t)c * 4 * Np * Np], 2 * Np);
+                    const tt_pw_states &chn, const VCProducts *vc = nullptr) {
+        VCProducts own;
+        if (!vc) {
+            vc_products(rp, V_unco, V_coup, swp, own);
+            vc = &own;
+        }
+        const std::vector<double> &VCu = vc->u, &VCc = vc->c;
         std::vector<int> p1, a1, p2, a2;
         std::vector<double> b1, b2;
         const double *Cu = swp.C_SWP_unco, *Cc = swp.C_SWP_coup;
@@ -2188,6 +2211,7 @@ struct PlanCache {
     std::vector<std::unique_ptr<SolvePlan>> plans;
     uint64_t tick = 0;
     static constexpr size_t max_plans = 64;
+    VCProducts vc;   // of the last V and C seen: shared by every channel of a batch
 };
 
 void free_plan_cache(PlanCache *c) { delete c; }
@@ -2241,11 +2265,19 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
     PlanCache &c = *ctx->plans;
     const uint64_t key = plan_struct_key(rp, swp, chn, format);
     const uint64_t content = plan_content_key(rp, V_unco, V_coup, swp);
+    auto products = [&]() -> const VCProducts & {
+        if (!c.vc.valid || c.vc.content != content) {
+            c.vc.valid = false;
+            vc_products(rp, V_unco, V_coup, swp, c.vc);
+            c.vc.content = content;
+        }
+        return c.vc;
+    };
     for (auto &pl : c.plans)
         if (pl->p_gen == P->gen && pl->key == key && pl->format == format && pl->col == col) {
             pl->last_use = ++c.tick;
             if (pl->content != content) {
-                pl->op.set_blocks(rp, V_unco, V_coup, swp, chn);
+                pl->op.set_blocks(rp, V_unco, V_coup, swp, chn, &products());
                 pl->content = content;
             }
             return *pl;
@@ -2261,7 +2293,7 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
         }
     }
     std::unique_ptr<SolvePlan> pl(new SolvePlan);
-    pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format, col);
+    pl->op.setup(ctx->stream, rp, P, V_unco, V_coup, swp, chn, format, col, &products());
     pl->p_gen = P->gen;
     pl->key = key;
     pl->content = content;
