@@ -137,19 +137,37 @@ tt_status tt_make_swp_states(const tt_run_params *rp, const tt_fwp_states *fwp,
                              const tt_pw_states *pw_full, const double *V_unco,
                              const double *V_coup, tt_swp_states *swp);
 
-/* Mirrors channel_os_indexing (CPP/type_defs.h:76-85), elastic part. */
+/* Mirrors channel_os_indexing (CPP/type_defs.h:76-85). */
 typedef struct {
     int num_T_lab;              /* number of on-shell q bins        */
     int *q_com_idx;             /* num_T_lab                        */
     double *T_lab, *q_com, *E_com;  /* num_T_lab                    */
     int num_deuteron_states;
     int *deuteron_idx;          /* indices into the channel's alphas */
+    /* Breakup on-shell cells (tt_find_breakup_bins; 0 / NULL without): entries
+     * q_com_BU_idx[j] .. q_com_BU_idx[j+1]-1 belong to on-shell energy j, entry b is the
+     * state (alpha, q, p) = alphapq_idx[3b .. 3b+2]. */
+    int num_BU_chns;
+    int *q_com_BU_idx;          /* num_T_lab + 1                     */
+    int *alphapq_idx;           /* 3 * num_BU_chns                   */
 } tt_channel_os;
 
 /* Replaces find_on_shell_bins (CPP/run_organizer.cpp:4-131, elastic) and
  * find_deuteron_channels (CPP/run_organizer.cpp:342-378) for one channel. */
 tt_status tt_find_on_shell(const tt_swp_states *swp, const tt_pw_states *chn,
                            const double *T_lab_in, int n_T_lab_in, tt_channel_os *out);
+/* Replaces the breakup part of find_on_shell_bins (CPP/run_organizer.cpp:136-339,
+ * include_breakup_channels=true): for every on-shell energy E_com of os (E_com + E_bound
+ * >= 0), every (alpha, q, p) cell whose energy range [e_p,lo + E_q,lo, e_p,hi + E_q,hi]
+ * (SWP p-bin energies, E_q = q^2 / (2 mu1_free)) holds E_com with e_p,lo >= 0, followed by
+ * the three neighbour cells of the quadrant that holds E_com, as the reference lists them
+ * (a neighbour may lie one bin outside the grid: its flat index alpha*Nq*Np + q*Np + p is
+ * what the solves read, as the reference does; one outside the state space, where the
+ * reference reads outside the row, gets NaN in U_BU).  Fills the breakup
+ * fields of os (replacing earlier ones); TT_ERR_PHYSICS when an energy at or above the
+ * breakup threshold has no such cell. */
+tt_status tt_find_breakup_bins(const tt_run_params *rp, const tt_swp_states *swp,
+                               const tt_pw_states *chn, tt_channel_os *os);
 void tt_channel_os_free(tt_channel_os *os);
 
 /* ---------------------------------------------------------------- P123 CSC
@@ -228,6 +246,12 @@ typedef struct {
     int resolvent_on_device;    /* 1: ignore G (may be NULL) and compute the resolvent of the
                                  * on-shell energies E_com + E_bound on the device, straight
                                  * into the solver's buffer (no host round trip); 0: use G. */
+    /* Optional breakup amplitudes (U_BU_array of solve_faddeev_equations): when os carries
+     * breakup cells, nd * num_BU_chns complex values U_BU[d_row*num_BU_chns + b] = the U row
+     * of (deuteron d_row, energy of b) at b's state (CPP/solve_faddeev.cpp:867-891,
+     * 1177-1206, 1300-1365, 1388-1409 for Pade; read from the same solution for the other
+     * methods).  NULL: not computed. */
+    double *U_BU;
 } tt_solve_opts;
 void tt_solve_opts_default(tt_solve_opts *o);
 
@@ -280,6 +304,7 @@ typedef struct {
     double *U;                  /* out: nd*nd*nE complex, layout as above          */
     tt_solve_stats *stats;      /* optional per-item statistics                    */
     tt_status status;           /* out                                             */
+    double *U_BU;               /* optional out: breakup amplitudes as tt_solve_opts.U_BU */
 } tt_batch_item;
 tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_unco,
                          const double *V_coup, const tt_swp_states *swp, tt_batch_item *items,

@@ -10,6 +10,9 @@ Restates:
   CPP/solve_faddeev.cpp:305-438     dense solve (1 - A G_j) U = A
   CPP/solve_faddeev.cpp:440-1444    Neumann rows a_n = e_r^T A (G A)^n e_c and Pade selection
   CPP/General_functions/matrix_routines.cpp:82-101   determinant with pivot signs
+  CPP/solve_faddeev.cpp:867-891,1177-1206,1300-1365,1388-1409
+                                    breakup terms a_n at the breakup cells' columns and
+                                    their Pade selection (same rules as elastic)
 
 The block tables are written as embeddings of each 2N-channel matrix into the
 (alpha, p) space: the reference's pointer tables select, for a pair
@@ -21,9 +24,17 @@ reference (lines 1264, 1273: out of bounds).  The array is value-initialised
 to zero, so the oracle takes PA_{-1} = 0; this choice is "parity unpinned"
 (no reference test or run can pin undefined behaviour).
 
+Breakup amplitudes are U[row, cell] at the breakup cells' flat indices
+alpha*Nq*Np + q*Np + p (formed as the reference forms them, also for a neighbour one
+bin outside the grid; a flat index outside the state space, where the reference reads
+outside the row, gives NaN).  The reference fills them in its Pade path only; the oracle
+also gives the converged value, U = (1 - A G_j)^-1 A read at those columns.
+
 Pins (tests/test_oracle_pins.py): matrix-free row apply against the explicit
 dense A; dense solve residual; Pade of a geometric series a_n = r^n equals
-1/(1-r) exactly for N >= 1 and agrees with scipy.interpolate.pade.
+1/(1-r) exactly for N >= 1 and agrees with scipy.interpolate.pade; breakup values
+of the dense solve against A (1 - G A)^-1 by explicit inversion, and their Pade
+resummation against the dense values.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -214,3 +225,54 @@ def pade_solve(a):
             for j in range(nE):
                 U[dr, dc, j] = pade_select(a[dr, dc, j])[0]
     return U
+
+
+def breakup_flat(cell, Nq, Np):
+    """Flat state index of a breakup cell (CPP/solve_faddeev.cpp:879)."""
+    a, q, p = (int(x) for x in cell)
+    return a * Nq * Np + q * Np + p
+
+
+def dense_solve_breakup(A, G, deut, q_idx, Nq, Np, bu_ptr, bu_cells):
+    """U_BU[d_row, b] = U[(alpha_d, q_j, 0), cell_b] with (1 - A G_j) X = A[:, cell_b columns]."""
+    nd = len(deut)
+    n_bu = len(bu_cells)
+    U = np.zeros((nd, n_bu), dtype=complex)
+    dim = A.shape[0]
+    U[:] = np.nan
+    for j in range(len(q_idx)):
+        bs = [b for b in range(int(bu_ptr[j]), int(bu_ptr[j + 1])) if 0 <= breakup_flat(bu_cells[b], Nq, Np) < dim]
+        if not len(bs):
+            continue
+        cols = [breakup_flat(bu_cells[b], Nq, Np) for b in bs]
+        X = np.linalg.solve(np.eye(dim) - A * G[j][None, :], A[:, cols].astype(complex))
+        for dr in range(nd):
+            r = int(deut[dr]) * Nq * Np + int(q_idx[j]) * Np
+            for k, b in enumerate(bs):
+                U[dr, b] = X[r, k]
+    return U
+
+
+def neumann_terms_breakup(Kop, G, deut, q_idx, Nq, Np, bu_ptr, bu_cells, nterms):
+    """a_BU[d_row, b, n] = e_r^T A (G A)^n at cell_b's column, r = (deuteron d_row, energy of b)
+    (CPP/solve_faddeev.cpp:867-891,1177-1206)."""
+    nd, nE = len(deut), len(q_idx)
+    dim = Kop.dim
+    E = np.zeros((nd * nE, dim))
+    for dr in range(nd):
+        for j in range(nE):
+            E[dr * nE + j, int(deut[dr]) * Nq * Np + int(q_idx[j]) * Np] = 1.0
+    rows = Kop.row_apply(E).astype(complex)
+    n_bu = len(bu_cells)
+    a = np.full((nd, n_bu, nterms), np.nan, dtype=complex)
+    Gr = np.stack([G[j] for dr in range(nd) for j in range(nE)])
+    for n in range(nterms):
+        if n > 0:
+            rows = Kop.row_apply(rows * Gr)
+        for dr in range(nd):
+            for j in range(nE):
+                for b in range(int(bu_ptr[j]), int(bu_ptr[j + 1])):
+                    f = breakup_flat(bu_cells[b], Nq, Np)
+                    if 0 <= f < dim:
+                        a[dr, b, n] = rows[dr * nE + j, f]
+    return a

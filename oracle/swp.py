@@ -5,6 +5,8 @@ Restates:
                                         isoscalar nn/np mixing, isospin-breaking 1S0 block
   CPP/make_swp_states.cpp:28-183,221-471 H = H0 + V, eigenvectors C, SWP energy-bin edges
   CPP/run_organizer.cpp:4-131,342-378   on-shell q-bins and deuteron channels
+  CPP/run_organizer.cpp:136-339         breakup on-shell cells (alpha, q, p) and their
+                                        quadrant neighbours
 
 The reference diagonalises with LAPACKE_dspevd; here numpy.linalg.eigh.  The
 eigenvector sign is arbitrary in both, and every on-shell U element is
@@ -12,7 +14,13 @@ invariant under it (C enters as C ... C^T with the diagonal G in between).
 
 Pins (tests/test_oracle_pins.py): C^T C = 1 and C^T H C diagonal; MT-III deuteron
 near -2.23 MeV at large N (CPP/Interactions/malfliet_tjon.cpp:11); V_WP of the
-MT potential against scipy.integrate.dblquad over one bin pair.
+MT potential against scipy.integrate.dblquad over one bin pair.  Breakup cells: every
+listed primary cell brackets the energy and every bracketing cell is listed, in loop
+order; the neighbours follow from the geometry of the cell's four sub-rectangles (the
+energy ranges of their corner sums), computed independently in the test.  The energy
+convention itself (the kinetic E_com of the on-shell bin, compared with SWP p-bin energies
+plus E_q = q^2 / (2 mu1_free)) is the reference's choice at run_organizer.cpp:146-220 and no
+reference output pins it: "parity unpinned" for that convention.
 """
 import math
 from dataclasses import dataclass
@@ -276,3 +284,76 @@ def find_deuteron_channels(pw_chn):
     return np.array([a for a in range(pw_chn.Nalpha)
                      if pw_chn.L[a] == st.deuteron_L and pw_chn.S[a] == st.deuteron_S
                      and pw_chn.J[a] == st.deuteron_J and pw_chn.T[a] == st.deuteron_T], dtype=np.int64)
+
+
+def mu1_free():
+    """Spectator reduced mass against a free pair (CPP/General_functions/kinetic_conversion.cpp:16-19)."""
+    M12 = st.Mp + st.Mn
+    return st.Mn * M12 / (st.Mn + M12)
+
+
+def _p_edges_of_alpha(pw_chn, swp, a, tensor_force, isospin_breaking_1S0):
+    """The SWP p-bin energy edges of alpha (CPP/run_organizer.cpp:157-203)."""
+    L, S, J, T, t3 = (int(pw_chn.L[a]), int(pw_chn.S[a]), int(pw_chn.J[a]), int(pw_chn.T[a]),
+                      int(pw_chn.two_T3[a]))
+    s1S0 = S == 0 and J == 0 and L == 0
+    via_L = bool(tensor_force) and L != J and J != 0
+    via_T = s1S0 and bool(isospin_breaking_1S0)
+    if via_L and via_T:
+        raise ValueError("isospin breaking in a coupled channel")
+    Np = swp.Np
+    if via_L or via_T:
+        idx = st.unique_2N_idx(L, S, J, T, True, tensor_force, isospin_breaking_1S0)
+        if via_L:
+            off = 0 if L < J else Np + 1
+        else:
+            off = 0 if t3 == 1 else Np + 1
+        return swp.e_coup[idx, off:off + Np + 1]
+    idx = st.unique_2N_idx(L, S, J, T, False, tensor_force, isospin_breaking_1S0)
+    return swp.e_unco[idx]
+
+
+def find_breakup_bins(pw_chn, swp, E_com, tensor_force, isospin_breaking_1S0):
+    """CPP/run_organizer.cpp:136-339.  Returns (q_com_BU_idx [nE + 1], cells [n, 3] of
+    (alpha, q, p)).  Energies with E_com + E_bound < 0 get no cells; an energy above that
+    with no bracketing cell raises."""
+    Nq, Np = swp.Nq, swp.Np
+    qe = swp.q_edges
+    m = mu1_free()
+    cells, ptr = [], [0]
+    for E in E_com:
+        if E + swp.E_bound < 0:
+            ptr.append(len(cells))
+            continue
+        found = False
+        for a in range(pw_chn.Nalpha):
+            ep = _p_edges_of_alpha(pw_chn, swp, a, tensor_force, isospin_breaking_1S0)
+            for q in range(Nq):
+                Eql = 0.5 * (qe[q] * qe[q]) / m
+                Equ = 0.5 * (qe[q + 1] * qe[q + 1]) / m
+                for p in range(Np):
+                    Epl, Epu = ep[p], ep[p + 1]
+                    lo, hi = Epl + Eql, Epu + Equ
+                    if not (lo <= E <= hi and Epl >= 0 and Epu >= 0):
+                        continue
+                    found = True
+                    cells.append((a, q, p))
+                    Ep, Eq = 0.5 * (Epu + Epl), 0.5 * (Equ + Eql)
+                    if Eq + Ep <= E <= hi:                      # quadrant 1
+                        prange, qrange = (p, p + 1), (q, q + 1)
+                    elif Eq + Epl <= E <= Ep + Equ:             # quadrant 2
+                        prange, qrange = (p - 1, p), (q, q + 1)
+                    elif lo <= E <= Eq + Ep:                    # quadrant 3
+                        prange, qrange = (p - 1, p), (q - 1, q)
+                    elif Eql + Ep <= E <= Epu + Eq:             # quadrant 4
+                        prange, qrange = (p, p + 1), (q - 1, q)
+                    else:
+                        continue
+                    for pi in range(prange[0], prange[1] + 1):
+                        for qj in range(qrange[0], qrange[1] + 1):
+                            if pi != p or qj != q:
+                                cells.append((a, qj, pi))
+        if not found:
+            raise ValueError("on-shell breakup energy outside the WP state space")
+        ptr.append(len(cells))
+    return np.array(ptr, dtype=np.int64), np.array(cells, dtype=np.int64).reshape(-1, 3)

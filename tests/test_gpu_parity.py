@@ -480,3 +480,52 @@ def test_parameter_walk_matches_per_set_solves(ctx):
                 Ur, _ = ref.channel(c, T).solve(ctx, Ps[k], None, method=tt.SOLVE_GMRES, device_resolvent=True)
                 assert U[s][k].shape == Ur.shape
                 assert np.abs(U[s][k] - Ur).max() <= tol * np.abs(Ur).max(), (method, s, c)
+
+
+# ------------------------------------------------------------------ breakup amplitudes
+@pytest.fixture(scope="module")
+def c1bu(ctx, c1):
+    """C1 with breakup cells at energies below (1 MeV) and above the breakup threshold; the
+    oracle's converged breakup values from the dense A (NaN where a neighbour cell's flat
+    index leaves the state space)."""
+    N = c1.prob.rp.Np_WP
+    T = [1.0, 13.0, 60.0]
+    ch = c1.prob.channel(0, T, breakup=True)
+    G = ch.resolvent(ctx)
+    Ud = so.dense_solve_breakup(c1.A, G, ch.deut, ch.q_idx, N, N, ch.bu_ptr, ch.bu_cells)
+    return ch, G, Ud, T
+
+
+@pytest.mark.parametrize("method", [tt.SOLVE_DENSE, tt.SOLVE_GMRES, tt.SOLVE_MIXED])
+def test_breakup_amplitudes_converged_c1(ctx, c1, c1bu, method):
+    ch, G, Ud, _ = c1bu
+    print("breakup", method, ch.n_bu, flush=True)
+    ch.solve(ctx, c1.P, G, method=method)
+    ok = ~np.isnan(Ud)
+    assert ok.sum() > 0 and np.array_equal(np.isnan(ch.U_BU), ~ok)
+    assert np.abs(ch.U_BU[ok] - Ud[ok]).max() <= 1e-10 * np.abs(Ud[ok]).max()
+
+
+def test_breakup_amplitudes_pade_c1(ctx, c1, c1bu):
+    """Pade breakup values: Neumann terms at the breakup columns resummed by the same rules as
+    the elastic ones; within 1e-5 of the converged values (as the elastic Pade test), and the
+    elastic U equals that of the same solve without breakup cells."""
+    ch, G, Ud, T = c1bu
+    U, st = ch.solve(ctx, c1.P, G, method=tt.SOLVE_PADE)
+    ok = ~np.isnan(Ud)
+    assert np.array_equal(np.isnan(ch.U_BU), ~ok)
+    assert np.abs(ch.U_BU[ok] - Ud[ok]).max() <= 1e-5 * np.abs(Ud[ok]).max()
+    plain = c1.prob.channel(0, T)
+    U0, _ = plain.solve(ctx, c1.P, G, method=tt.SOLVE_PADE)
+    assert np.array_equal(U, U0)
+
+
+@pytest.mark.parametrize("method", [tt.SOLVE_GMRES, tt.SOLVE_MIXED])
+def test_breakup_amplitudes_in_batch(ctx, c1, c1bu, method):
+    ch, G, Ud, _ = c1bu
+    ok = ~np.isnan(Ud)
+    ch.U_BU = None
+    res = c1.prob.solve_batch(ctx, [ch], [c1.P], method=method, tol=1e-12)
+    assert res[0][2] == tt._lib.TT_OK
+    assert np.array_equal(np.isnan(ch.U_BU), ~ok)
+    assert np.abs(ch.U_BU[ok] - Ud[ok]).max() <= 1e-9 * np.abs(Ud[ok]).max()

@@ -163,3 +163,20 @@ def test_potential_matrices_do_not_depend_on_the_thread_count(tmp_path):
                         os.path.join(ROOT, "tests")], env=env, check=True, timeout=120)
         out[n] = np.load(f)
     assert out[1].size > 0 and np.array_equal(out[1], out[8])
+
+
+def test_breakup_cells_match_oracle():
+    """tt_find_breakup_bins against the oracle on the library's own SWP states (tensor force
+    and isospin-breaking 1S0: coupled p-bin edges), energies below and above the breakup
+    threshold."""
+    from helpers import oracle_pw, oracle_swp
+    rp, pot_, _ = synthetic_config(N=10, two_J_3N_max=3)
+    prob = tt.Problem(rp, pot_)
+    T = [0.5, 10.0, 60.0]
+    for c in (0, 2):
+        ch = prob.channel(c, T, breakup=True)
+        ptr, cells = osw.find_breakup_bins(oracle_pw(ch), oracle_swp(prob), ch.E_com, True, True)
+        assert ch.n_bu == len(cells) > 0
+        assert np.array_equal(ch.bu_ptr, ptr)
+        assert np.array_equal(ch.bu_cells, cells)
+        assert ptr[1] == 0   # the lowest bin lies below the threshold

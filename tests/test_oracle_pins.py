@@ -351,3 +351,91 @@ def test_row_apply_dense_and_dense_solve_residual():
     Xfull = np.linalg.solve(np.eye(dim) - A * G[0][None, :], A)
     assert abs(U[0, 0, 0] - Xfull[r, r]) < 1e-12 * abs(Xfull[r, r])
     assert np.allclose((np.eye(dim) - A * G[0][None, :]) @ Xfull, A, atol=1e-12)
+
+
+# ---------------------------------------------------------------- breakup
+def _expected_breakup_cells(ep, Eq, E):
+    """Breakup cells from the geometry of each cell: the cell [ep_p, ep_p+1] x [Eq_q, Eq_q+1]
+    brackets E when E lies between its smallest and largest corner sum; its four
+    sub-rectangles (split at the midpoints) are tried in the order (+p, +q), (-p, +q),
+    (-p, -q), (+p, -q), and the first whose corner sums bracket E names the 2x2 block of
+    neighbours, listed p-major with q inside."""
+    out = []
+    Na, Np1 = ep.shape
+    for a in range(Na):
+        for q in range(len(Eq) - 1):
+            for p in range(Np1 - 1):
+                if ep[a, p] < 0 or ep[a, p + 1] < 0:
+                    continue
+                corners = [x + y for x in ep[a, p:p + 2] for y in Eq[q:q + 2]]
+                if not (min(corners) <= E <= max(corners)):
+                    continue
+                out.append((a, q, p))
+                pm_, qm_ = 0.5 * (ep[a, p + 1] + ep[a, p]), 0.5 * (Eq[q + 1] + Eq[q])
+                for dp, dq in ((1, 1), (-1, 1), (-1, -1), (1, -1)):
+                    pr = (pm_, ep[a, p + 1]) if dp > 0 else (ep[a, p], pm_)
+                    qr = (qm_, Eq[q + 1]) if dq > 0 else (Eq[q], qm_)
+                    sums = [x + y for x in pr for y in qr]
+                    if min(sums) <= E <= max(sums):
+                        block = sorted({(p + i, q + k) for i in (0, dp) for k in (0, dq)} - {(p, q)})
+                        out += [(a, qq, pp) for pp, qq in block]
+                        break
+    return out
+
+
+def _mt_breakup_case(N=10, T_lab=(0.5, 13.0, 40.0)):
+    pwf = st.construct_pw_states(1, 1, False, swave_only=True)
+    pw = pwf.channel(0)
+    fwp = st.make_fwp(N, N, st.chebyshev_scale_for_top(N, 8 * st.hbarc))
+    Vu, Vc = sw.potential_matrices(fwp, pwf, pot.MalflietTjon(), False, False, 1)
+    swp = sw.make_swp_states(fwp, pwf, Vu, Vc, False, False)
+    os_ = sw.find_on_shell_bins(fwp.q_edges, swp.E_bound, list(T_lab))
+    return pw, fwp, Vu, Vc, swp, os_
+
+
+def test_breakup_cells_against_cell_geometry():
+    pw, fwp, _, _, swp, os_ = _mt_breakup_case()
+    ptr, cells = sw.find_breakup_bins(pw, swp, os_.E_com, False, False)
+    ep = np.stack([sw._p_edges_of_alpha(pw, swp, a, False, False) for a in range(pw.Nalpha)])
+    Eq = 0.5 * fwp.q_edges ** 2 / sw.mu1_free()
+    assert ptr[0] == 0 and len(ptr) == len(os_.E_com) + 1
+    for j, E in enumerate(os_.E_com):
+        got = [tuple(c) for c in cells[ptr[j]:ptr[j + 1]]]
+        if E + swp.E_bound < 0:
+            assert got == []
+        else:
+            assert got and got == _expected_breakup_cells(ep, Eq, E)
+    assert ptr[-1] > 0 and (os_.E_com + swp.E_bound < 0).any()   # both branches exercised
+
+
+def test_breakup_dense_values_against_explicit_inverse_and_pade():
+    """U_BU from (1 - A G_j) X = A[:, cells] equals A (1 - G_j A)^-1 by explicit inversion at
+    the same entries (the other order of the factors), and the Pade resummation of the breakup
+    Neumann terms reaches it."""
+    from oracle import p123 as pm_
+    pw, fwp, Vu, Vc, swp, os_ = _mt_breakup_case()
+    N = fwp.Np
+    deut = sw.find_deuteron_channels(pw)
+    ptr, cells = sw.find_breakup_bins(pw, swp, os_.E_com, False, False)
+    ent, _ = pm_.p123_rows(pw, fwp.p_edges, fwp.q_edges, 16, 16)
+    dim = pw.Nalpha * N * N
+    K = so.KernelA(pw, swp, Vu, Vc, so.p_matrix(*pm_.to_csc(ent, dim), dim), False, False)
+    A = K.dense()
+    G = np.stack([rs.resolvent_array(E + swp.E_bound, pw, swp, False, False) for E in os_.E_com])
+    Ubu = so.dense_solve_breakup(A, G, deut, os_.q_idx, N, N, ptr, cells)
+    assert len(cells) > 0
+    for j in range(len(os_.E_com)):
+        Ufull = A @ np.linalg.inv(np.eye(dim) - G[j][:, None] * A)
+        for dr, a in enumerate(deut):
+            r = int(a) * N * N + int(os_.q_idx[j]) * N
+            for b in range(ptr[j], ptr[j + 1]):
+                c = so.breakup_flat(cells[b], N, N)
+                if 0 <= c < dim:
+                    assert abs(Ubu[dr, b] - Ufull[r, c]) <= 1e-11 * np.abs(Ufull[r]).max()
+                else:
+                    assert np.isnan(Ubu[dr, b])
+    ok = ~np.isnan(Ubu)
+    a_bu = so.neumann_terms_breakup(K, G, deut, os_.q_idx, N, N, ptr, cells, 29)
+    Up = np.array([[so.pade_select(a_bu[d, b])[0] if ok[d, b] else np.nan for b in range(len(cells))]
+                   for d in range(len(deut))])
+    assert np.abs(Up[ok] - Ubu[ok]).max() <= 1e-6 * np.abs(Ubu[ok]).max()

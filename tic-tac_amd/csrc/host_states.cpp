@@ -484,6 +484,80 @@ tt_status tt_find_on_shell(const tt_swp_states *swp, const tt_pw_states *chn, co
     TT_GUARD_END
 }
 
+tt_status tt_find_breakup_bins(const tt_run_params *rp, const tt_swp_states *swp, const tt_pw_states *chn,
+                               tt_channel_os *os) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(rp && swp && chn && os && (os->num_T_lab == 0 || os->E_com), "null argument");
+    const int Na = chn->Nalpha, Nq = swp->Nq_WP, Np = swp->Np_WP, nE = os->num_T_lab;
+    const double *qe = swp->q_WP;
+    // the p-bin energy edges of every alpha (the same choice as the resolvent's)
+    std::vector<int> which, offset;
+    swp_edge_offsets(*chn, *swp, *rp, which, offset);
+    // E_q of the q-bin edges with the free spectator mass (mu1_free, kinetic_conversion.cpp:16-19)
+    const double M12 = kMp + kMn, mu_free = kMn * M12 / (kMn + M12);
+    std::vector<int> apq, ptr(1, 0);
+    for (int j = 0; j < nE; ++j) {
+        const double E = os->E_com[j];
+        if (E + swp->E_bound < 0) {   // below the breakup threshold: no entries
+            ptr.push_back((int)(apq.size() / 3));
+            continue;
+        }
+        bool found = false;
+        auto add = [&](int a, int q, int p) {
+            apq.push_back(a);
+            apq.push_back(q);
+            apq.push_back(p);
+        };
+        for (int a = 0; a < Na; ++a) {
+            const double *ep = (which[a] ? swp->e_SWP_coup : swp->e_SWP_unco) + offset[a];
+            for (int q = 0; q < Nq; ++q) {
+                const double Eql = 0.5 * qe[q] * qe[q] / mu_free, Equ = 0.5 * qe[q + 1] * qe[q + 1] / mu_free;
+                for (int p = 0; p < Np; ++p) {
+                    const double Epl = ep[p], Epu = ep[p + 1];
+                    const double lo = Epl + Eql, hi = Epu + Equ;
+                    if (!(lo <= E && E <= hi && Epl >= 0 && Epu >= 0)) continue;
+                    found = true;
+                    add(a, q, p);
+                    // the quadrant of the cell (split at its bin midpoints) whose energy range
+                    // holds E, tried in the order upper-upper, lower-p/upper-q, lower-lower,
+                    // upper-p/lower-q; its three neighbours follow (run_organizer.cpp:250-315)
+                    const double Epm = 0.5 * (Epu + Epl), Eqm = 0.5 * (Equ + Eql);
+                    int dp = 0, dq = 0;
+                    if (Eqm + Epm <= E && E <= hi) {
+                        dp = 1;
+                        dq = 1;
+                    } else if (Eqm + Epl <= E && E <= Epm + Equ) {
+                        dp = -1;
+                        dq = 1;
+                    } else if (lo <= E && E <= Eqm + Epm) {
+                        dp = -1;
+                        dq = -1;
+                    } else if (Eql + Epm <= E && E <= Epu + Eqm) {
+                        dp = 1;
+                        dq = -1;
+                    }
+                    if (dp == 0) continue;
+                    // the 2x2 block spanned by the cell and (p + dp, q + dq), p-major, q inner,
+                    // both ascending, without the cell itself
+                    const int p0 = std::min(p, p + dp), q0 = std::min(q, q + dq);
+                    for (int pi = p0; pi <= p0 + 1; ++pi)
+                        for (int qj = q0; qj <= q0 + 1; ++qj)
+                            if (pi != p || qj != q) add(a, qj, pi);
+                }
+            }
+        }
+        if (!found) fail(TT_ERR_PHYSICS, "on-shell breakup energy E_com=" + std::to_string(E) +
+                                             " MeV has no cell in the WP state space");
+        ptr.push_back((int)(apq.size() / 3));
+    }
+    std::free(os->q_com_BU_idx);
+    std::free(os->alphapq_idx);
+    os->q_com_BU_idx = dup_ints(ptr);
+    os->alphapq_idx = dup_ints(apq.empty() ? std::vector<int>(3, 0) : apq);
+    os->num_BU_chns = (int)(apq.size() / 3);
+    TT_GUARD_END
+}
+
 void tt_channel_os_free(tt_channel_os *os) {
     if (!os) return;
     std::free(os->q_com_idx);
@@ -491,6 +565,8 @@ void tt_channel_os_free(tt_channel_os *os) {
     std::free(os->q_com);
     std::free(os->E_com);
     std::free(os->deuteron_idx);
+    std::free(os->q_com_BU_idx);
+    std::free(os->alphapq_idx);
     std::memset(os, 0, sizeof(*os));
 }
 

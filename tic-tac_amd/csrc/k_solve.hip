@@ -1984,17 +1984,55 @@ static OnshellLayout onshell_layout(const tt_channel_os &os, int Nq, int Np) {
     return L;
 }
 
-// values rows[(dr, j)][(alpha_dc, q_j, 0)] -> out[dr][dc][j]
+// Flat state index of breakup cell b as the reference forms it (CPP/solve_faddeev.cpp:879):
+// a neighbour one bin outside the grid lands on another state or outside the state space.
+static inline int64_t breakup_flat(const tt_channel_os &os, int b, int Nq, int Np) {
+    const int *t = os.alphapq_idx + 3 * (size_t)b;
+    return (int64_t)t[0] * Nq * Np + (int64_t)t[1] * Np + t[2];
+}
+
+// Elements read from the row solutions x[state][k] (k = d_row * nE + j): first the elastic
+// U[dr][dc][j] = x[(alpha_dc, q_j, 0)][dr * nE + j], then with breakup
+// U_BU[dr][b] = x[(alpha_b, q_b, p_b)][dr * nE + j(b)] (CPP/solve_faddeev.cpp:867-891), the
+// state's flat index alpha * Nq * Np + q * Np + p as the reference forms it.  A neighbour cell
+// whose flat index falls outside the state space (where the reference reads outside the row)
+// is left out; bu_pos gets the U_BU position (dr * num_BU_chns + b) of each breakup element
+// listed, and the positions left out are set to NaN by the callers.
+static void readout_list(const tt_channel_os &os, int Nq, int Np, int64_t dim, bool breakup,
+                         std::vector<int64_t> &idx, std::vector<int> &kk, std::vector<int> *bu_pos = nullptr) {
+    const int nd = os.num_deuteron_states, nE = os.num_T_lab;
+    idx.clear();
+    kk.clear();
+    for (int dr = 0; dr < nd; ++dr)
+        for (int dc = 0; dc < nd; ++dc)
+            for (int j = 0; j < nE; ++j) {
+                idx.push_back(((int64_t)os.deuteron_idx[dc] * Nq + os.q_com_idx[j]) * Np);
+                kk.push_back(dr * nE + j);
+            }
+    if (bu_pos) bu_pos->clear();
+    if (!breakup) return;
+    TT_REQUIRE(os.q_com_BU_idx && os.alphapq_idx, "breakup cells missing (tt_find_breakup_bins)");
+    for (int dr = 0; dr < nd; ++dr)
+        for (int j = 0; j < nE; ++j)
+            for (int b = os.q_com_BU_idx[j]; b < os.q_com_BU_idx[j + 1]; ++b) {
+                const int64_t flat = breakup_flat(os, b, Nq, Np);
+                if (flat < 0 || flat >= dim) continue;
+                idx.push_back(flat);
+                kk.push_back(dr * nE + j);
+                if (bu_pos) bu_pos->push_back(dr * os.num_BU_chns + b);
+            }
+}
+
+static inline bool wants_breakup(const tt_channel_os &os, const double *U_BU) {
+    return U_BU && os.num_BU_chns > 0;
+}
+
+// values of the readout list of rows (elastic, then breakup) -> out
 static void gather_onshell(RowOperator &op, const double2 *rows, const OnshellLayout &L, const tt_channel_os &os,
-                           std::vector<cdouble> &out) {
+                           std::vector<cdouble> &out, bool breakup = false, std::vector<int> *bu_pos = nullptr) {
     std::vector<int64_t> idx;
     std::vector<int> kk;
-    for (int dr = 0; dr < L.nd; ++dr)
-        for (int dc = 0; dc < L.nd; ++dc)
-            for (int j = 0; j < L.nE; ++j) {
-                idx.push_back(((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np);
-                kk.push_back(dr * L.nE + j);
-            }
+    readout_list(os, op.Nq, op.Np, op.dim, breakup, idx, kk, bu_pos);
     int n = (int)idx.size();
     DevBuf<int64_t> di;
     DevBuf<int> dk;
@@ -2027,17 +2065,29 @@ static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 
     if (own.p) TT_HIP(hipStreamSynchronize(op.s));   // own buffers are freed on return
 }
 
+// Elastic and breakup elements alike get Neumann terms and Pade selection; the series runs
+// until every element has converged.  (The reference stops once the count of converged elastic
+// plus breakup elements equals the elastic count, lines 920-924, which can leave elastic
+// elements unselected; here all of them are selected.)
 static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_opts &o, cdouble *U,
-                       tt_solve_stats &st) {
+                       tt_solve_stats &st, cdouble *U_BU = nullptr) {
     OnshellLayout L = onshell_layout(os, op.Nq, op.Np);
-    const int nel = L.nd * L.nd * L.nE;
+    const bool bu = U_BU && os.num_BU_chns > 0;
+    const int n_el = L.nd * L.nd * L.nE;
+    std::vector<int> bu_pos;
+    {
+        std::vector<int64_t> idx;
+        std::vector<int> kk;
+        readout_list(os, op.Nq, op.Np, op.dim, bu, idx, kk, &bu_pos);
+    }
+    const int nel = n_el + (int)bu_pos.size();
     const int NM_max = o.NM_max > 0 ? o.NM_max : 14;
     const int nterms = 2 * NM_max + 1;
     op.set_gidx(L.row_g);
     DevBuf<double2> rows((size_t)op.dim * L.r), rows2((size_t)op.dim * L.r);
     on_shell_rows_of_A(op, L, rows.p);
     std::vector<cdouble> a((size_t)nel * nterms, 0.0), an;
-    gather_onshell(op, rows.p, L, os, an);
+    gather_onshell(op, rows.p, L, os, an, bu);
     for (int e = 0; e < nel; ++e) a[(size_t)e * nterms] = an[e];
     std::vector<cdouble> PA((size_t)nel * (NM_max + 1), 0.0);
     std::vector<int> best_idx(nel, 0);
@@ -2049,7 +2099,7 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
             if (n <= 0) continue;
             op.apply(rows.p, rows2.p, L.r, true, false);
             std::swap(rows.p, rows2.p);
-            gather_onshell(op, rows.p, L, os, an);
+            gather_onshell(op, rows.p, L, os, an, bu);
             for (int e = 0; e < nel; ++e)
                 if (!conv[e]) a[(size_t)e * nterms + n] = an[e];
             n_done = n;
@@ -2082,8 +2132,12 @@ static void solve_pade(RowOperator &op, const tt_channel_os &os, const tt_solve_
             }
         }
     }
-    for (int e = 0; e < nel; ++e) U[e] = PA[(size_t)e * (NM_max + 1) + best_idx[e]];
-    if (o.neumann_terms) std::memcpy(o.neumann_terms, a.data(), sizeof(cdouble) * a.size());
+    for (int e = 0; e < n_el; ++e) U[e] = PA[(size_t)e * (NM_max + 1) + best_idx[e]];
+    if (bu) {
+        std::fill(U_BU, U_BU + (size_t)L.nd * os.num_BU_chns, cdouble(NAN, NAN));
+        for (int e = n_el; e < nel; ++e) U_BU[bu_pos[e - n_el]] = PA[(size_t)e * (NM_max + 1) + best_idx[e]];
+    }
+    if (o.neumann_terms) std::memcpy(o.neumann_terms, a.data(), sizeof(cdouble) * n_el * nterms);
     st.iterations = n_done + 1;
 }
 
@@ -2140,6 +2194,9 @@ struct SolvePlan {
     DevBuf<double2> os_out;
     double2 *os_out_h = nullptr;
     size_t os_out_cap = 0;
+    int os_out_n = 0;      // elements of the last enqueue_onshell
+    int os_bu_total = 0;   // its U_BU size and the U_BU position of each breakup element
+    std::vector<int> os_bu_pos;
     uint64_t in_use = 0;   // mark of the lockstep solve holding this plan
     // work buffers of the gradient assembly (row plans of tt_solve_gradient)
     struct GradWork {
@@ -2179,16 +2236,14 @@ struct SolvePlan {
     // Enqueue the on-shell elements of x (rows (d_row, j), columns (alpha_dc, q_j, 0)) into
     // pinned host memory, read by read_onshell once the stream has passed:
     // out[dr][dc][j] = x[(alpha_dc, q_j, 0)][dr * nE + j].
-    void enqueue_onshell(const double2 *xv, const OnshellLayout &L, const tt_channel_os &os) {
+    // With breakup the breakup elements follow (readout_list).
+    void enqueue_onshell(const double2 *xv, const OnshellLayout &L, const tt_channel_os &os, bool breakup = false) {
         std::vector<int64_t> idx;
         std::vector<int> kk;
-        for (int dr = 0; dr < L.nd; ++dr)
-            for (int dc = 0; dc < L.nd; ++dc)
-                for (int j = 0; j < L.nE; ++j) {
-                    idx.push_back(((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np);
-                    kk.push_back(dr * L.nE + j);
-                }
+        readout_list(os, op.Nq, op.Np, op.dim, breakup, idx, kk, &os_bu_pos);
         const int n = (int)idx.size();
+        os_out_n = n;
+        os_bu_total = breakup ? L.nd * os.num_BU_chns : 0;
         if ((size_t)n > os_out_cap) {
             if (os_out_h) TT_HIP(hipHostFree(os_out_h));
             os_out_h = nullptr;
@@ -2201,9 +2256,13 @@ struct SolvePlan {
         hipLaunchKernelGGL(k_gather, dim3(nblk(n, 64)), dim3(64), 0, op.s, xv, os_idx.p, os_k.p, os_out.p, n, L.r);
         TT_HIP(hipMemcpyAsync(os_out_h, os_out.p, sizeof(double2) * n, hipMemcpyDeviceToHost, op.s));
     }
-    void read_onshell(const OnshellLayout &L, cdouble *U) const {
+    void read_onshell(const OnshellLayout &L, cdouble *U, cdouble *U_BU = nullptr) const {
         const int n = L.nd * L.nd * L.nE;
         for (int i = 0; i < n; ++i) U[i] = cdouble(os_out_h[i].x, os_out_h[i].y);
+        if (U_BU) {
+            std::fill(U_BU, U_BU + os_bu_total, cdouble(NAN, NAN));
+            for (int i = n; i < os_out_n; ++i) U_BU[os_bu_pos[i - n]] = cdouble(os_out_h[i].x, os_out_h[i].y);
+        }
     }
 };
 
@@ -2311,6 +2370,7 @@ struct KMember {
     SolvePlan *pl = nullptr;
     const tt_channel_os *os = nullptr;
     bool mixed = false;
+    bool breakup = false;   // also read the breakup elements of the solution
     OnshellLayout L;
     std::vector<double> bnorm;
     double res = 0.0;
@@ -2414,7 +2474,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
         for (size_t i : act)
             if (!ms[i].done) ms[i].iters += ms[i].mixed ? k32[i]->used : k64[i]->used;
     }
-    for (auto &mb : ms) mb.pl->enqueue_onshell(mb.pl->x.p, mb.L, *mb.os);
+    for (auto &mb : ms) mb.pl->enqueue_onshell(mb.pl->x.p, mb.L, *mb.os, mb.breakup);
     sync_all();
 }
 
@@ -2454,7 +2514,10 @@ static void solve_on_lanes(tt_ctx *ctx, std::vector<KMember> &ms, const tt_solve
 }
 
 // Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
-static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt_solve_stats &st) {
+// With U_BU, the breakup states' columns of A join the right-hand sides (U = (1 - A G)^-1 A
+// column by column); the reference's dense path leaves U_BU unset.
+static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt_solve_stats &st,
+                        cdouble *U_BU = nullptr) {
     const int64_t dim = op.dim;
     TT_REQUIRE(dim <= 4096, "dense solve is a debug path for dim <= 4096");
     std::vector<cdouble> A((size_t)dim * dim);
@@ -2480,7 +2543,22 @@ static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt
     TT_HIP(hipMemcpyAsync(Gh.data(), op.G.p, sizeof(double2) * Gh.size(), hipMemcpyDeviceToHost, op.s));
     TT_HIP(hipStreamSynchronize(op.s));
     const int nd = os.num_deuteron_states, nE = os.num_T_lab;
+    const bool bu = U_BU && os.num_BU_chns > 0;
+    if (bu) std::fill(U_BU, U_BU + (size_t)nd * os.num_BU_chns, cdouble(NAN, NAN));
     for (int j = 0; j < nE; ++j) {
+        // breakup cells of energy j inside the state space: extra right-hand-side columns
+        std::vector<int> bsel;
+        std::vector<int64_t> bcol;
+        if (bu)
+            for (int b = os.q_com_BU_idx[j]; b < os.q_com_BU_idx[j + 1]; ++b) {
+                const int64_t flat = breakup_flat(os, b, op.Nq, op.Np);
+                if (flat < 0 || flat >= dim) continue;
+                bsel.push_back(b);
+                bcol.push_back(flat);
+            }
+        const int nb = (int)bsel.size();
+        const int nrhs = nd + nb;
+        auto bu_col = [&](int k) { return bcol[k]; };
         // L = 1 - A G_j (columns scaled), RHS = on-shell columns of A (CPP/solve_faddeev.cpp:372-388)
         std::vector<cdouble> M((size_t)dim * dim);
         for (int64_t rr = 0; rr < dim; ++rr)
@@ -2488,12 +2566,14 @@ static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt
                 const double2 g = Gh[(size_t)j * dim + c];
                 M[(size_t)rr * dim + c] = (rr == c ? 1.0 : 0.0) - A[(size_t)rr * dim + c] * cdouble(g.x, g.y);
             }
-        std::vector<cdouble> Bm((size_t)dim * nd);
-        for (int64_t rr = 0; rr < dim; ++rr)
+        std::vector<cdouble> Bm((size_t)dim * nrhs);
+        for (int64_t rr = 0; rr < dim; ++rr) {
             for (int dc = 0; dc < nd; ++dc) {
                 int64_t c = ((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np;
-                Bm[(size_t)rr * nd + dc] = A[(size_t)rr * dim + c];
+                Bm[(size_t)rr * nrhs + dc] = A[(size_t)rr * dim + c];
             }
+            for (int k = 0; k < nb; ++k) Bm[(size_t)rr * nrhs + nd + k] = A[(size_t)rr * dim + bu_col(k)];
+        }
         // Gaussian elimination with partial pivoting
         for (int64_t k = 0; k < dim; ++k) {
             int64_t piv = k;
@@ -2505,27 +2585,27 @@ static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt
                 }
             if (piv != k) {
                 for (int64_t c = 0; c < dim; ++c) std::swap(M[(size_t)k * dim + c], M[(size_t)piv * dim + c]);
-                for (int c = 0; c < nd; ++c) std::swap(Bm[(size_t)k * nd + c], Bm[(size_t)piv * nd + c]);
+                for (int c = 0; c < nrhs; ++c) std::swap(Bm[(size_t)k * nrhs + c], Bm[(size_t)piv * nrhs + c]);
             }
             cdouble d = M[(size_t)k * dim + k];
             for (int64_t i = k + 1; i < dim; ++i) {
                 cdouble f = M[(size_t)i * dim + k] / d;
                 if (f == cdouble(0.0)) continue;
                 for (int64_t c = k + 1; c < dim; ++c) M[(size_t)i * dim + c] -= f * M[(size_t)k * dim + c];
-                for (int c = 0; c < nd; ++c) Bm[(size_t)i * nd + c] -= f * Bm[(size_t)k * nd + c];
+                for (int c = 0; c < nrhs; ++c) Bm[(size_t)i * nrhs + c] -= f * Bm[(size_t)k * nrhs + c];
             }
         }
         for (int64_t i = dim - 1; i >= 0; --i)
-            for (int c = 0; c < nd; ++c) {
-                cdouble s = Bm[(size_t)i * nd + c];
-                for (int64_t l = i + 1; l < dim; ++l) s -= M[(size_t)i * dim + l] * Bm[(size_t)l * nd + c];
-                Bm[(size_t)i * nd + c] = s / M[(size_t)i * dim + i];
+            for (int c = 0; c < nrhs; ++c) {
+                cdouble s = Bm[(size_t)i * nrhs + c];
+                for (int64_t l = i + 1; l < dim; ++l) s -= M[(size_t)i * dim + l] * Bm[(size_t)l * nrhs + c];
+                Bm[(size_t)i * nrhs + c] = s / M[(size_t)i * dim + i];
             }
-        for (int dr = 0; dr < nd; ++dr)
-            for (int dc = 0; dc < nd; ++dc) {
-                int64_t rr = ((int64_t)os.deuteron_idx[dr] * op.Nq + os.q_com_idx[j]) * op.Np;
-                U[(size_t)dr * nd * nE + (size_t)dc * nE + j] = Bm[(size_t)rr * nd + dc];
-            }
+        for (int dr = 0; dr < nd; ++dr) {
+            int64_t rr = ((int64_t)os.deuteron_idx[dr] * op.Nq + os.q_com_idx[j]) * op.Np;
+            for (int dc = 0; dc < nd; ++dc) U[(size_t)dr * nd * nE + (size_t)dc * nE + j] = Bm[(size_t)rr * nrhs + dc];
+            for (int k = 0; k < nb; ++k) U_BU[(size_t)dr * os.num_BU_chns + bsel[k]] = Bm[(size_t)rr * nrhs + nd + k];
+        }
     }
     st.iterations = 0;
 }
@@ -2591,6 +2671,7 @@ void tt_solve_opts_default(tt_solve_opts *o) {
     o->p123_format = TT_P123_CELLS;
     o->use_graphs = 0;
     o->resolvent_on_device = 0;
+    o->U_BU = nullptr;
 }
 
 tt_status tt_csc_upload(tt_ctx *ctx, int64_t dim, int64_t nnz, const double *val, const int32_t *row,
@@ -2748,12 +2829,15 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
     SolvePlan &pl = channel_plan(ctx, rp, G, P123, V_unco, V_coup, swp, os, chn, o);
     RowOperator &op = pl.op;
     std::vector<cdouble> Uc((size_t)os->num_deuteron_states * os->num_deuteron_states * os->num_T_lab);
+    const bool bu = wants_breakup(*os, o.U_BU);
+    std::vector<cdouble> Ubu(bu ? (size_t)os->num_deuteron_states * os->num_BU_chns : 0);
+    cdouble *bu_out = bu ? Ubu.data() : nullptr;
     switch (o.method) {
         case TT_SOLVE_PADE:
-            solve_pade(op, *os, o, Uc.data(), st);
+            solve_pade(op, *os, o, Uc.data(), st, bu_out);
             break;
         case TT_SOLVE_DENSE:
-            solve_dense(op, *os, Uc.data(), st);
+            solve_dense(op, *os, Uc.data(), st, bu_out);
             break;
         case TT_SOLVE_GMRES:
         case TT_SOLVE_MIXED: {
@@ -2761,8 +2845,9 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
             ms[0].pl = &pl;
             ms[0].os = os;
             ms[0].mixed = o.method == TT_SOLVE_MIXED;
+            ms[0].breakup = bu;
             solve_lockstep(ms, o);
-            pl.read_onshell(ms[0].L, Uc.data());
+            pl.read_onshell(ms[0].L, Uc.data(), bu_out);
             st.iterations = ms[0].iters;
             st.residual = ms[0].res;
             break;
@@ -2772,6 +2857,7 @@ static void solve_channel(tt_ctx *ctx, const tt_run_params *rp, double *U, const
     }
     TT_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(U, Uc.data(), sizeof(cdouble) * Uc.size());
+    if (bu) std::memcpy(o.U_BU, Ubu.data(), sizeof(cdouble) * Ubu.size());
     channel_stats(pl, st);
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
@@ -2800,7 +2886,9 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
         for (int i = 0; i < n_items; ++i) {
             tt_batch_item &it = items[i];
             try {
-                solve_channel(ctx, rp, it.U, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o, it.stats);
+                tt_solve_opts oi = o;
+                oi.U_BU = it.U_BU;
+                solve_channel(ctx, rp, it.U, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, oi, it.stats);
             } catch (const Error &e) {
                 record(i, e.code, e.msg);
             }
@@ -2833,6 +2921,7 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
                 mb.pl = &pl;
                 mb.os = it.os;
                 mb.mixed = o.method == TT_SOLVE_MIXED;
+                mb.breakup = wants_breakup(*it.os, it.U_BU);
                 ms.push_back(std::move(mb));
                 idx.push_back(i);
             } catch (const Error &e) {
@@ -2851,8 +2940,10 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
             tt_batch_item &it = items[idx[k]];
             SolvePlan &pl = *ms[k].pl;
             std::vector<cdouble> Uc((size_t)ms[k].L.nd * ms[k].L.nd * ms[k].L.nE);
-            pl.read_onshell(ms[k].L, Uc.data());
+            std::vector<cdouble> Ubu(ms[k].breakup ? (size_t)ms[k].L.nd * it.os->num_BU_chns : 0);
+            pl.read_onshell(ms[k].L, Uc.data(), ms[k].breakup ? Ubu.data() : nullptr);
             std::memcpy(it.U, Uc.data(), sizeof(cdouble) * Uc.size());
+            if (ms[k].breakup) std::memcpy(it.U_BU, Ubu.data(), sizeof(cdouble) * Ubu.size());
             tt_solve_stats st;
             std::memset(&st, 0, sizeof(st));
             channel_stats(pl, st);
@@ -3166,6 +3257,7 @@ tt_status tt_solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_u
     tt_solve_opts_default(&o);
     if (opts) o = *opts;
     o.resolvent_on_device = 1;
+    o.U_BU = nullptr;   // per item (tt_batch_item.U_BU)
     tt::solve_batch(ctx, rp, V_unco, V_coup, swp, items, n_items, o);
     TT_GUARD_END
 }

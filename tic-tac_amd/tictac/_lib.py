@@ -46,7 +46,8 @@ class SwpStates(C.Structure):
 
 class ChannelOS(C.Structure):
     _fields_ = [("num_T_lab", C.c_int), ("q_com_idx", _ip), ("T_lab", _dp), ("q_com", _dp), ("E_com", _dp),
-                ("num_deuteron_states", C.c_int), ("deuteron_idx", _ip)]
+                ("num_deuteron_states", C.c_int), ("deuteron_idx", _ip), ("num_BU_chns", C.c_int),
+                ("q_com_BU_idx", _ip), ("alphapq_idx", _ip)]
 
 
 class CSC(C.Structure):
@@ -62,7 +63,7 @@ class P123Stats(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int),
                 ("NM_max", C.c_int), ("neumann_terms", _dp), ("p123_format", C.c_int), ("use_graphs", C.c_int),
-                ("resolvent_on_device", C.c_int)]
+                ("resolvent_on_device", C.c_int), ("U_BU", _dp)]
 
 
 class SolveStats(C.Structure):
@@ -79,7 +80,7 @@ class GradStats(C.Structure):
 
 class BatchItem(C.Structure):
     _fields_ = [("chn", C.POINTER(PwStates)), ("os", C.POINTER(ChannelOS)), ("P123", C.POINTER(CSC)),
-                ("U", _dp), ("stats", C.POINTER(SolveStats)), ("status", C.c_int)]
+                ("U", _dp), ("stats", C.POINTER(SolveStats)), ("status", C.c_int), ("U_BU", _dp)]
 
 
 # exported symbols and their prototypes (restype, argtypes)
@@ -108,6 +109,8 @@ PROTOTYPES = {
                                      C.POINTER(SwpStates)]),
     "tt_find_on_shell": (C.c_int, [C.POINTER(SwpStates), C.POINTER(PwStates), _dp, C.c_int,
                                    C.POINTER(ChannelOS)]),
+    "tt_find_breakup_bins": (C.c_int, [C.POINTER(RunParams), C.POINTER(SwpStates), C.POINTER(PwStates),
+                                       C.POINTER(ChannelOS)]),
     "tt_channel_os_free": (None, [C.POINTER(ChannelOS)]),
     "tt_csc_upload": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _dp, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_uint64), C.POINTER(CSC)]),

@@ -126,8 +126,8 @@ class Problem:
     def n_channels(self):
         return self.pw.N_chn_3N
 
-    def channel(self, c, T_lab):
-        return Channel(self, c, T_lab)
+    def channel(self, c, T_lab, breakup=False):
+        return Channel(self, c, T_lab, breakup)
 
     def solve_batch(self, ctx, channels, P123s, method=L.SOLVE_MIXED, tol=1e-12, max_iter=2000, restart=40,
                     graphs=False):
@@ -140,7 +140,7 @@ class Problem:
         o.use_graphs = int(bool(graphs))
         n = len(channels)
         items = (L.BatchItem * max(n, 1))()
-        Us, stats = [], []
+        Us, stats, Ubus = [], [], []
         for i, (ch, P) in enumerate(zip(channels, P123s)):
             U = np.zeros(ch.nd * ch.nd * ch.nE * 2)
             st = L.SolveStats()
@@ -151,8 +151,14 @@ class Problem:
             items[i].P123 = C.pointer(P.csc)
             items[i].U = dptr(U)
             items[i].stats = C.pointer(st)
+            Ubus.append(np.zeros(max(ch.nd * ch.n_bu, 1) * 2))
+            if ch.n_bu:
+                items[i].U_BU = dptr(Ubus[-1])
         check(L.lib().tt_solve_batch(ctx.handle, C.byref(self.rp), dptr(self.V_unco), dptr(self.V_coup),
                                      C.byref(self.swp), items, n, C.byref(o)))
+        for ch, Ubu in zip(channels, Ubus):
+            if ch.n_bu:
+                ch.U_BU = Ubu.view(np.complex128)[:ch.nd * ch.n_bu].reshape(ch.nd, ch.n_bu).copy()
         return [(U.view(np.complex128).reshape(ch.nd, ch.nd, ch.nE), st, items[i].status)
                 for i, (U, st, ch) in enumerate(zip(Us, stats, channels))]
 
@@ -297,9 +303,10 @@ class P123:
 
 
 class Channel:
-    """One 3N channel J^Pi with its elastic on-shell indexing."""
+    """One 3N channel J^Pi with its on-shell indexing (elastic, and with breakup=True the
+    breakup cells of tt_find_breakup_bins)."""
 
-    def __init__(self, prob, c, T_lab):
+    def __init__(self, prob, c, T_lab, breakup=False):
         lib = L.lib()
         self.prob = prob
         self.index = c
@@ -319,6 +326,13 @@ class Channel:
         self.deut = _arr(self.os.deuteron_idx, self.nd, np.int64)
         self.two_J = int(self.view.two_J_3N[0])
         self.parity = int(self.view.P_3N[0])
+        self.n_bu = 0
+        if breakup:
+            check(lib.tt_find_breakup_bins(C.byref(prob.rp), C.byref(prob.swp), C.byref(self.view),
+                                           C.byref(self.os)))
+            self.n_bu = self.os.num_BU_chns
+            self.bu_ptr = _arr(self.os.q_com_BU_idx, nE + 1, np.int64)
+            self.bu_cells = _arr(self.os.alphapq_idx, 3 * self.n_bu, np.int64).reshape(self.n_bu, 3)
 
     def pw_arrays(self):
         n = self.Nalpha
@@ -348,7 +362,9 @@ class Channel:
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
 
         With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n].
-        With ``device_resolvent=True`` G is computed on the device inside the call (G ignored)."""
+        With ``device_resolvent=True`` G is computed on the device inside the call (G ignored).
+        A channel made with breakup=True also fills ``self.U_BU[d_row, b]``, the breakup
+        amplitudes of its breakup cells."""
         if device_resolvent:
             G = None
         else:
@@ -368,6 +384,9 @@ class Channel:
             o.neumann_terms = dptr(a)
         st = L.SolveStats()
         U = np.zeros(self.nd * self.nd * self.nE * 2)
+        Ubu = np.zeros(max(self.nd * self.n_bu, 1) * 2)
+        if self.n_bu:
+            o.U_BU = dptr(Ubu)
         Gf = None if G is None else G.view(np.float64).ravel()
         check(L.lib().tt_solve_faddeev_equations(ctx.handle, C.byref(self.prob.rp), dptr(U),
                                                  dptr(Gf) if Gf is not None else None,
@@ -375,6 +394,8 @@ class Channel:
                                                  C.byref(self.prob.swp), C.byref(self.os), C.byref(self.view),
                                                  C.byref(o), C.byref(st)))
         U = U.view(np.complex128).reshape(self.nd, self.nd, self.nE)
+        if self.n_bu:
+            self.U_BU = Ubu.view(np.complex128)[:self.nd * self.n_bu].reshape(self.nd, self.n_bu).copy()
         if neumann:
             return U, st, a.view(np.complex128).reshape(self.nd, self.nd, self.nE, nterms)
         return U, st
