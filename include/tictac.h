@@ -168,6 +168,12 @@ tt_status tt_find_on_shell(const tt_swp_states *swp, const tt_pw_states *chn,
  * breakup threshold has no such cell. */
 tt_status tt_find_breakup_bins(const tt_run_params *rp, const tt_swp_states *swp,
                                const tt_pw_states *chn, tt_channel_os *os);
+/* The SWP p-bin energy edges of every alpha of a channel, edges[alpha*(Np+1) + p]: the
+ * per-alpha choice of e_SWP_unco / e_SWP_coup (upper or lower half) that the reference makes
+ * inline in CPP/make_resolvent.cpp:118-156, CPP/run_organizer.cpp:157-203 and
+ * CPP/disk_io_routines.cpp:507-540 (the breakup output's p-bin boundaries). */
+tt_status tt_channel_p_edges(const tt_run_params *rp, const tt_swp_states *swp, const tt_pw_states *chn,
+                             double *edges);
 void tt_channel_os_free(tt_channel_os *os);
 
 /* ---------------------------------------------------------------- P123 CSC

@@ -1,0 +1,123 @@
+"""tictac_run, the reference's run driver over the library: option handling on the CPU, and on
+the GPU the U_PW_elements / U_PW_breakup_elements files against the same solves made through
+the Python binding."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import tictac as tt
+from helpers import PKG
+
+RUN = os.path.join(PKG, "tictac", "tictac_run")
+
+
+def _run(args, cwd, timeout=300):
+    return subprocess.run([RUN] + list(args), cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def _input(tmp_path, **kw):
+    """An input list file (energy_input_file has to come from one: a command-line argument ending
+    in .txt is read as an input list, as in the reference)."""
+    f = tmp_path / "input.txt"
+    f.write_text("# run options\n" + "".join("%s=%s\n" % kv for kv in kw.items()))
+    return str(f)
+
+
+def test_cli_options_and_errors(tmp_path):
+    assert os.path.exists(RUN), "tictac_run not built (make -C tic-tac_amd)"
+    r = _run(["-h"], tmp_path)
+    assert r.returncode == 0 and "solve_method" in r.stdout
+    r = _run(["two_J_3N_max=2"], tmp_path)
+    assert r.returncode == 1 and "Cannot have even two_J_3N_max" in r.stderr
+    r = _run(["potential_model=malfliet_tjon", "parallel_run=true"], tmp_path)
+    assert r.returncode == 1 and "no channel index" in r.stderr
+    r = _run(["tensor_force=maybe"], tmp_path)
+    assert r.returncode == 1 and "Invalid value for input parameter tensor_force" in r.stderr
+    r = _run(["no_such_option=1", "potential_model=LO_internal"], tmp_path)
+    assert "Unrecognised run option entered: no_such_option=1" in r.stdout
+    assert r.returncode == 1 and "Invalid potential model entered: LO_internal" in r.stderr
+    # the last value read wins: the command line after the input file
+    inp = _input(tmp_path, two_J_3N_max=2, potential_model="malfliet_tjon")
+    r = _run([inp, "two_J_3N_max=1", "potential_model=synthetic"], tmp_path)
+    assert r.returncode == 1 and "parameter_file required" in r.stderr
+
+
+def _read_elastic(path):
+    """Rows of table 2 of a U_PW_elements file: (T_lab, E_com, q_idx, [U elements])."""
+    rows = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    tables = [i for i, l in enumerate(lines) if l.startswith("# ####")]
+    for l in lines[tables[1] + 2:tables[2]]:
+        tok = l.split()
+        rows.append((float(tok[0]), float(tok[1]), int(tok[2]), [complex(t) for t in tok[3:]]))
+    return lines, rows
+
+
+def _mt_rp(N):
+    return tt.run_params(two_J_3N_max=1, J_2N_max=1, Np_WP=N, Nq_WP=N, chebyshev_s=100.0, tensor_force=0,
+                         isospin_breaking_1S0=0)
+
+
+@pytest.mark.gpu
+def test_cli_elastic_and_breakup_files_match_the_binding(ctx, tmp_path):
+    N = 12
+    (tmp_path / "e.txt").write_text("13.0\n40.0\n")
+    out = tmp_path / "out"
+    inp = _input(tmp_path, potential_model="malfliet_tjon", Np_WP=N, Nq_WP=N, J_2N_max=1, tensor_force="false",
+                 isospin_breaking_1S0="false", include_breakup_channels="true", energy_input_file=tmp_path / "e.txt",
+                 output_folder=out)
+    print("tictac_run MT N=12 Pade with breakup", flush=True)
+    r = _run([inp], tmp_path)
+    assert r.returncode == 0, r.stderr
+    prob = tt.Problem(_mt_rp(N), tt.Potential("malfliet_tjon"))
+    for c in range(prob.n_channels):
+        ch = prob.channel(c, [13.0, 40.0], breakup=True)
+        P = ch.build_p123(ctx)
+        U, _ = ch.solve(ctx, P, None, method=tt.SOLVE_PADE, device_resolvent=True)
+        name = "_Np_%d_Nq_%d_JP_%d_%d_Jmax_1_PSI_0.txt" % (N, N, ch.two_J, ch.parity)
+        lines, rows = _read_elastic(out / ("U_PW_elements" + name))
+        assert lines[3] == "# JP:          1/2" + ("+" if ch.parity == 1 else "-")
+        assert len(rows) == ch.nE
+        for j, (T, E, q, u) in enumerate(rows):
+            assert (T, E, q) == (ch.T_lab[j], ch.E_com[j], ch.q_idx[j])
+            assert len(u) == ch.nd ** 2
+            # two processes: the autotuner may pick other launch shapes, so sums differ in order
+            assert np.abs(np.array(u) - U[:, :, j].ravel()).max() <= 1e-12 * np.abs(U).max()
+        # breakup file: every finite breakup value of the solve appears in it
+        bu = (out / ("U_PW_breakup_elements" + name)).read_text()
+        vals = np.array([complex(t) for l in bu.splitlines() if not l.startswith("#")
+                         for t in l.split() if t.endswith("j")])
+        fin = ch.U_BU[np.isfinite(ch.U_BU)].ravel()
+        if ch.n_bu and fin.size:
+            scale = np.abs(fin).max()
+            assert all(np.abs(vals - v).min() <= 1e-12 * scale for v in fin)
+
+
+@pytest.mark.gpu
+def test_cli_parameter_walk_matches_the_binding(ctx, tmp_path):
+    N = 10
+    th = tt.synthetic_params(7)
+    sets = np.stack([th, th * np.r_[np.full(12, 1.01), np.ones(3)]])
+    np.savetxt(tmp_path / "params.txt", sets, fmt="%.17g")
+    (tmp_path / "e.txt").write_text("20.0\n")
+    out = tmp_path / "out"
+    inp = _input(tmp_path, potential_model="synthetic", Np_WP=N, Nq_WP=N, J_2N_max=1, parameter_walk="true",
+                 parameter_file=tmp_path / "params.txt", PSI_start=0, PSI_end=2, solve_method="gmres",
+                 energy_input_file=tmp_path / "e.txt", output_folder=out, parallel_run="true", channel_idx=0)
+    print("tictac_run synthetic N=10 parameter walk", flush=True)
+    r = _run([inp], tmp_path)
+    assert r.returncode == 0, r.stderr
+    rp = tt.run_params(two_J_3N_max=1, J_2N_max=1, Np_WP=N, Nq_WP=N, chebyshev_s=100.0)
+    for psi in range(2):
+        prob = tt.Problem(rp, tt.Potential("synthetic", sets[psi]))
+        ch = prob.channel(0, [20.0])
+        P = ch.build_p123(ctx)
+        U, _ = ch.solve(ctx, P, None, method=tt.SOLVE_GMRES, device_resolvent=True)
+        name = "U_PW_elements_Np_%d_Nq_%d_JP_%d_%d_Jmax_1_PSI_%d.txt" % (N, N, ch.two_J, ch.parity, psi)
+        _, rows = _read_elastic(out / name)
+        got = np.array([u for _, _, _, u in rows]).T.reshape(ch.nd, ch.nd, ch.nE)
+        assert np.abs(got - U).max() <= 1e-10 * np.abs(U).max()
+    assert len(os.listdir(out)) == 2   # parallel_run: channel 0 only, two parameter sets

@@ -558,6 +558,20 @@ tt_status tt_find_breakup_bins(const tt_run_params *rp, const tt_swp_states *swp
     TT_GUARD_END
 }
 
+tt_status tt_channel_p_edges(const tt_run_params *rp, const tt_swp_states *swp, const tt_pw_states *chn,
+                             double *edges) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(rp && swp && chn && edges, "null argument");
+    std::vector<int> which, offset;
+    swp_edge_offsets(*chn, *swp, *rp, which, offset);
+    const int Np = swp->Np_WP;
+    for (int a = 0; a < chn->Nalpha; ++a) {
+        const double *e = (which[a] ? swp->e_SWP_coup : swp->e_SWP_unco) + offset[a];
+        std::memcpy(edges + (size_t)a * (Np + 1), e, sizeof(double) * (Np + 1));
+    }
+    TT_GUARD_END
+}
+
 void tt_channel_os_free(tt_channel_os *os) {
     if (!os) return;
     std::free(os->q_com_idx);

@@ -111,6 +111,7 @@ PROTOTYPES = {
                                    C.POINTER(ChannelOS)]),
     "tt_find_breakup_bins": (C.c_int, [C.POINTER(RunParams), C.POINTER(SwpStates), C.POINTER(PwStates),
                                        C.POINTER(ChannelOS)]),
+    "tt_channel_p_edges": (C.c_int, [C.POINTER(RunParams), C.POINTER(SwpStates), C.POINTER(PwStates), _dp]),
     "tt_channel_os_free": (None, [C.POINTER(ChannelOS)]),
     "tt_csc_upload": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _dp, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_uint64), C.POINTER(CSC)]),
