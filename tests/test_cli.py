@@ -68,15 +68,15 @@ def test_cli_elastic_and_breakup_files_match_the_binding(ctx, tmp_path):
     out = tmp_path / "out"
     inp = _input(tmp_path, potential_model="malfliet_tjon", Np_WP=N, Nq_WP=N, J_2N_max=1, tensor_force="false",
                  isospin_breaking_1S0="false", include_breakup_channels="true", energy_input_file=tmp_path / "e.txt",
-                 output_folder=out)
-    print("tictac_run MT N=12 Pade with breakup", flush=True)
+                 output_folder=out, solve_method="gmres")
+    print("tictac_run MT N=12 GMRES with breakup", flush=True)
     r = _run([inp], tmp_path)
     assert r.returncode == 0, r.stderr
     prob = tt.Problem(_mt_rp(N), tt.Potential("malfliet_tjon"))
     for c in range(prob.n_channels):
         ch = prob.channel(c, [13.0, 40.0], breakup=True)
         P = ch.build_p123(ctx)
-        U, _ = ch.solve(ctx, P, None, method=tt.SOLVE_PADE, device_resolvent=True)
+        U, _ = ch.solve(ctx, P, None, method=tt.SOLVE_GMRES, device_resolvent=True)
         name = "_Np_%d_Nq_%d_JP_%d_%d_Jmax_1_PSI_0.txt" % (N, N, ch.two_J, ch.parity)
         lines, rows = _read_elastic(out / ("U_PW_elements" + name))
         assert lines[3] == "# JP:          1/2" + ("+" if ch.parity == 1 else "-")
@@ -85,7 +85,8 @@ def test_cli_elastic_and_breakup_files_match_the_binding(ctx, tmp_path):
             assert (T, E, q) == (ch.T_lab[j], ch.E_com[j], ch.q_idx[j])
             assert len(u) == ch.nd ** 2
             # two processes: the autotuner may pick other launch shapes, so sums differ in order
-            assert np.abs(np.array(u) - U[:, :, j].ravel()).max() <= 1e-12 * np.abs(U).max()
+            # (converged solves: 1e-10; a Pade selection could amplify such differences)
+            assert np.abs(np.array(u) - U[:, :, j].ravel()).max() <= 1e-10 * np.abs(U).max()
         # breakup file: every finite breakup value of the solve appears in it
         bu = (out / ("U_PW_breakup_elements" + name)).read_text()
         vals = np.array([complex(t) for l in bu.splitlines() if not l.startswith("#")
@@ -93,7 +94,7 @@ def test_cli_elastic_and_breakup_files_match_the_binding(ctx, tmp_path):
         fin = ch.U_BU[np.isfinite(ch.U_BU)].ravel()
         if ch.n_bu and fin.size:
             scale = np.abs(fin).max()
-            assert all(np.abs(vals - v).min() <= 1e-12 * scale for v in fin)
+            assert all(np.abs(vals - v).min() <= 1e-10 * scale for v in fin)
 
 
 @pytest.mark.gpu
