@@ -695,6 +695,37 @@ struct CellItems {
         items.upload(it.data(), std::max<size_t>(it.size(), 1), s);
         ket_item_ptr.upload(kip.data(), kip.size(), s);
     }
+    // Items over only the cells whose partner bin (q', p') has q' marked in qmask: runs of
+    // consecutive such cells of one group, at most item_len long.  For inputs that are zero
+    // outside those q bins (unit rows at on-shell states) the other cells add exact zeros.
+    // n = 0 when no cell qualifies.
+    void build_on_q(const CellView &P, hipStream_t s, int item_len, int Np, const std::vector<char> &qmask) {
+        TT_REQUIRE(item_len >= 1 && item_len <= 64, "work item length must be in [1, 64]");
+        std::vector<int> kp(P.NqNp + 1), partner((size_t)std::max<int64_t>(P.ncell, 1));
+        TT_HIP(hipMemcpyAsync(kp.data(), P.grp_ptr, sizeof(int) * (P.NqNp + 1), hipMemcpyDeviceToHost, s));
+        if (P.ncell)
+            TT_HIP(hipMemcpyAsync(partner.data(), P.partner, sizeof(int) * P.ncell, hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+        auto on = [&](int c) { return qmask[partner[c] / Np] != 0; };
+        std::vector<int2> it;
+        std::vector<int> kip(P.NqNp + 1, 0);
+        for (int k = 0; k < P.NqNp; ++k) {
+            int c = kp[k];
+            while (c < kp[k + 1]) {
+                if (!on(c)) {
+                    ++c;
+                    continue;
+                }
+                const int c0 = c;
+                while (c < kp[k + 1] && c - c0 < item_len && on(c)) ++c;
+                it.push_back(make_int2(c0, c));
+            }
+            kip[k + 1] = (int)it.size();
+        }
+        n = (int)it.size();
+        items.upload(it.data(), std::max<size_t>(it.size(), 1), s);
+        ket_item_ptr.upload(kip.data(), kip.size(), s);
+    }
 };
 
 template <typename T, int MT, int NT, int UN>
@@ -1178,6 +1209,9 @@ struct RowOperator {
     uint64_t layout_gen = 0;
     // a buffer the first block op of the next application zeroes (for accumulations that follow)
     double2 *zero_next = nullptr;
+    // work items for the next application only (items_on_q), else the tuned ones
+    const CellItems *next_items = nullptr;
+    std::map<std::vector<int>, std::unique_ptr<CellItems>> items_q;   // key: item length, then q bins
     int zero_next_n = 0;
 
     ~RowOperator() {
@@ -1355,6 +1389,24 @@ struct RowOperator {
         return it == tuned.end() ? dflt : it->second;
     }
     CellItems &items_for(bool fp32, int R2) { return items_of_len(choice(fp32, R2).len); }
+    // Work items over the cells whose input bin lies in one of the q bins (row operator), or
+    // nullptr when none does.
+    const CellItems *items_on_q(const std::vector<int> &qbins, int len) {
+        std::vector<int> key(1, len);
+        key.insert(key.end(), qbins.begin(), qbins.end());
+        auto it = items_q.find(key);
+        if (it == items_q.end()) {
+            std::vector<char> mask(Nq, 0);
+            for (int q : qbins) {
+                TT_REQUIRE(q >= 0 && q < Nq, "q bin out of range");
+                mask[q] = 1;
+            }
+            std::unique_ptr<CellItems> ci(new CellItems);
+            ci->build_on_q(cv, s, len, Np, mask);
+            it = items_q.emplace(key, std::move(ci)).first;
+        }
+        return it->second->n > 0 ? it->second.get() : nullptr;
+    }
 
     // Autotuner of the cell-block product: on the first application of a shape (precision, r),
     // each candidate item length and batch depth is timed, together with the block op that
@@ -1513,7 +1565,12 @@ struct RowOperator {
         const int cm = cells ? 1 : 0;
         // cell path: S -> work-item partial panels; the item sum and the factor 2 are folded
         // into the input staging of the V.C block op
-        CellItems *ci = cells ? &items_for(fp32, R2) : nullptr;
+        const CellItems *ci = cells ? (next_items ? next_items : &items_for(fp32, R2)) : nullptr;
+        next_items = nullptr;
+        if (ci && upart.n < (size_t)std::max(ci->n, 1) * Na * R2) {
+            upart.alloc((size_t)std::max(ci->n, 1) * Na * R2);
+            ++layout_gen;
+        }
         const int un = cells ? choice(fp32, R2).un : 0;
         const int *kip = cells ? ci->ket_item_ptr.p : nullptr;
         const double in_scale = cells ? 2.0 : 1.0;
@@ -2061,7 +2118,20 @@ static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 
     TT_HIP(hipMemsetAsync(scratch, 0, sizeof(double2) * (size_t)op.dim * L.r, op.s));
     idx->upload(L.row_idx.data(), L.r, op.s);
     hipLaunchKernelGGL(k_unit_rows, dim3(nblk(L.r, 64)), dim3(64), 0, op.s, scratch, idx->p, L.r);
-    op.apply(scratch, Bout, L.r, false, false);
+    // The unit rows, and so the input of the cell product, are zero outside the on-shell q
+    // bins: only the cells reading those bins take part (untimed, as it moves fewer bytes than
+    // a full product)
+    bool timed = true;
+    if (op.cells && !op.col) {
+        op.prepare(L.r, false);
+        std::vector<int> qb;
+        for (int64_t ri : L.row_idx) qb.push_back((int)((ri / op.Np) % op.Nq));
+        std::sort(qb.begin(), qb.end());
+        qb.erase(std::unique(qb.begin(), qb.end()), qb.end());
+        op.next_items = op.items_on_q(qb, op.choice(false, 2 * L.r).len);
+        timed = op.next_items == nullptr;
+    }
+    op.apply(scratch, Bout, L.r, false, false, timed);
     if (own.p) TT_HIP(hipStreamSynchronize(op.s));   // own buffers are freed on return
 }
 
