@@ -28,7 +28,9 @@ import numpy as np  # noqa: E402
 import tictac as tt  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-LANES = 4   # streams a batched lockstep group is spread over (the library's kLanes)
+# streams a batched lockstep group is spread over: the library reads TT_LANES (1..16), else 8
+LANES = int(os.environ["TT_LANES"]) if os.environ.get("TT_LANES", "").isdigit() and \
+    1 <= int(os.environ["TT_LANES"]) <= 16 else 8
 METRIC = "U-matrix channel-solves/s batched-mixed vs plain FP64 @|ΔU/U|≤1e-8; grad/fwd time"
 SOLVE_TOL = 1e-12
 BATCH_T = [float(t) for t in np.linspace(1.0, 200.0, 8)]
