@@ -536,6 +536,7 @@ struct Run {
     T v[N];
 };
 
+constexpr int XCD_CHUNK = 32;   // items per chunk of the XCD-aware order
 template <typename T, int MT, int NT, int UN>
 __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
                                                   const T *__restrict__ B, const T *__restrict__ S,
@@ -545,10 +546,13 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so workgroup b runs on
-    // XCD b % 8; it takes item start(b % 8) + b / 8, giving each XCD one contiguous range of
-    // items (neighbouring kets, which read largely the same bra rows of S from that XCD's L2)
-    const int nit = gridDim.x, xq = nit / 8, xr = nit % 8, xcd = blockIdx.x % 8;
-    const int item = xcd * xq + min(xcd, xr) + blockIdx.x / 8;
+    // XCD b % 8.  The items are cut into chunks of XCD_CHUNK consecutive items (neighbouring
+    // kets, which read largely the same bra rows of S), dealt to the XCDs in turn: each chunk
+    // is read through one XCD's L2, and every XCD gets work from the whole item range, so the
+    // XCDs finish together.  The last partial round keeps the launch order.
+    const int nit = gridDim.x, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+    const int full = nit / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    const int item = (int)blockIdx.x < full ? ((w / XCD_CHUNK) * 8 + xcd) * XCD_CHUNK + w % XCD_CHUNK : (int)blockIdx.x;
     const int v0 = blockIdx.y * 16 * NT;
     const int c_beg = items[item].x, ncell = items[item].y - c_beg;
     const int K = ncell * Na;
