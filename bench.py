@@ -298,7 +298,23 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=20.0):
             "sample": "%d channel-solve(s) of the workload's J^Pi=1/2+ channel (the smallest; T_lab %s MeV, "
                       "2 on-shell bins x 2 deuteron channels each, GMRES rtol 1e-12), scipy.sparse on 1 thread"
                       % (done, ", ".join("%.4g" % t for t in BATCH_T[:done])),
-            "max_rel_vs_gpu_plain_fp64": diff}
+            "max_rel_vs_gpu_plain_fp64": diff, "host": host_info()}
+
+
+def host_info():
+    """The host the CPU baseline ran on: CPU model, the CPUs this process may use, and the
+    OpenMP / BLAS thread setting (scipy's sparse products use one thread)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 WALK_WORKLOAD = ("same 18 channels x 8 energies for 8 parameter sets (the 12 strengths scaled by 0.99 .. 1.02): "
