@@ -193,19 +193,24 @@ void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, cons
                 continue;
             bool via_L, via_T;
             coupling_kind(Lr, Sr, Jr, Lc, rp, via_L, via_T);
+            // the 2N block this pair reads: element (pr, pc) at base[pr * ld + pc]
+            const double *base;
+            size_t ld;
+            if (via_L || via_T) {
+                const int idx = unique_2N_idx(Lr, Sr, Jr, Tr, true, rp);
+                const int hr = half_of(chn, ar, via_L), hc = half_of(chn, ac, via_L);
+                base = M_coup + (size_t)idx * 4 * Np * Np + (size_t)hr * Np * 2 * Np + (size_t)hc * Np;
+                ld = 2 * (size_t)Np;
+            } else {
+                const int idx = unique_2N_idx(Lr, Sr, Jr, Tr, false, rp);
+                base = M_unco + (size_t)idx * Np * Np;
+                ld = Np;
+            }
             bool nz = false;
             for (int p = 0; p < Np; ++p)
                 for (int pp = 0; pp < Np; ++pp) {
                     const int pr = transpose ? pp : p, pc = transpose ? p : pp;
-                    double v;
-                    if (via_L || via_T) {
-                        int idx = unique_2N_idx(Lr, Sr, Jr, Tr, true, rp);
-                        int hr = half_of(chn, ar, via_L), hc = half_of(chn, ac, via_L);
-                        v = M_coup[(size_t)idx * 4 * Np * Np + (size_t)(hr * Np + pr) * 2 * Np + hc * Np + pc];
-                    } else {
-                        int idx = unique_2N_idx(Lr, Sr, Jr, Tr, false, rp);
-                        v = M_unco[(size_t)idx * Np * Np + (size_t)pr * Np + pc];
-                    }
+                    const double v = base[(size_t)pr * ld + pc];
                     blk[(size_t)p * Np + pp] = v;
                     nz = nz || v != 0.0;
                 }
