@@ -2391,13 +2391,14 @@ static uint64_t plan_content_key(const tt_run_params &rp, const double *V_unco, 
 // The context's plan for these inputs: found by the P123 object and the structure, its block
 // contents refreshed when V or C changed (parameter sweeps, gradients), else built (evicting
 // the least recently used plan beyond max_plans).
+// content_key: plan_content_key of V and C when the caller has it (one per batch call).
 static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC *P, const double *V_unco,
                            const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format,
-                           bool col = false) {
+                           bool col = false, const uint64_t *content_key = nullptr) {
     if (!ctx->plans) ctx->plans = new PlanCache;
     PlanCache &c = *ctx->plans;
     const uint64_t key = plan_struct_key(rp, swp, chn, format);
-    const uint64_t content = plan_content_key(rp, V_unco, V_coup, swp);
+    const uint64_t content = content_key ? *content_key : plan_content_key(rp, V_unco, V_coup, swp);
     auto products = [&]() -> const VCProducts & {
         if (!c.vc.valid || c.vc.content != content) {
             c.vc.valid = false;
@@ -2840,14 +2841,15 @@ namespace tt {
 // The plan of one channel's solve (checked arguments).
 static SolvePlan &find_plan(tt_ctx *ctx, const tt_run_params *rp, const double *G, const tt_csc *P123,
                             const double *V_unco, const double *V_coup, const tt_swp_states *swp,
-                            const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o) {
+                            const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o,
+                            const uint64_t *content_key = nullptr) {
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && os && chn, "null argument");
     TT_REQUIRE(G || o.resolvent_on_device, "G required unless opts->resolvent_on_device is set");
     TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
     TT_HIP(hipSetDevice(ctx->device));
     const DeviceCSC *P = dev_of(P123);
     TT_REQUIRE(P->dim == (int64_t)chn->Nalpha * swp->Nq_WP * swp->Np_WP, "P123 dimension does not match Nalpha*Nq*Np");
-    return get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format);
+    return get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format, false, content_key);
 }
 
 // G of the solve in place in the plan (computed on the device or uploaded) and the plan's
@@ -2981,6 +2983,8 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
         return;
     }
     const int group_max = (int)PlanCache::max_plans;
+    // V and C are the same for every item: hash them once
+    const uint64_t content = plan_content_key(*rp, V_unco, V_coup, *swp);
     int i0 = 0;
     while (i0 < n_items) {
         auto t0 = std::chrono::steady_clock::now();
@@ -2997,7 +3001,7 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
             tt_batch_item &it = items[i];
             try {
                 TT_REQUIRE(it.U, "null U");
-                SolvePlan &pl = find_plan(ctx, rp, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o);
+                SolvePlan &pl = find_plan(ctx, rp, nullptr, it.P123, V_unco, V_coup, swp, it.os, it.chn, o, &content);
                 if (pl.in_use) break;   // the same channel twice: it starts the next group
                 load_plan(ctx, pl, rp, nullptr, swp, it.os, it.chn, o);
                 pl.in_use = 1;
