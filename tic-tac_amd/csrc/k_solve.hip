@@ -1136,7 +1136,22 @@ static TuneTable &tune_table() {
 struct ResolventStaging {
     DevBuf<double> eu, ec, qe, E;
     DevBuf<int> which, offset;
+    // host copies of what the buffers hold: inputs equal to them are not sent again (the SWP
+    // edges and the per-alpha tables stay the same from one solve of a channel to the next)
+    std::vector<double> h_eu, h_ec, h_qe;
+    std::vector<int> h_which, h_offset;
+    template <typename T>
+    static void put(DevBuf<T> &d, std::vector<T> &last, const T *h, size_t n, hipStream_t s) {
+        if (d.p && d.n >= n && last.size() == n && std::equal(h, h + n, last.begin())) return;
+        d.upload(h, n, s);
+        last.assign(h, h + n);
+    }
 };
+// d <- h[0..n) unless d already holds exactly that (last: host copy of what d holds)
+template <typename T>
+static void upload_if_changed(DevBuf<T> &d, std::vector<T> &last, const T *h, size_t n, hipStream_t s) {
+    ResolventStaging::put(d, last, h, n, s);
+}
 
 // V.C of every 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array,
 // CPP/solve_faddeev.cpp:1828-1855): row-major Np x Np (uncoupled) and 2Np x 2Np (coupled).
@@ -1184,6 +1199,7 @@ struct RowOperator {
     DevBuf<double2> S, U;      // FP64 scratch (dim * r)
     DevBuf<float2> S32, U32;   // FP32 scratch
     DevBuf<int> gidx;
+    std::vector<int> h_gidx;   // what gidx holds (set_gidx sends only changes)
     // Event pool: four events per timed application (before the first block op, around the
     // SpMM, after the last block op); read back once in collect(), no per-apply synchronisation.
     std::vector<hipEvent_t> evs;
@@ -1347,6 +1363,7 @@ struct RowOperator {
             S.alloc((size_t)dim * r);
             U.alloc((size_t)dim * r);
             gidx.alloc(r);
+            h_gidx.clear();
             r_cap = r;
             S32.release();
             U32.release();
@@ -1502,7 +1519,7 @@ struct RowOperator {
 
     void set_gidx(const std::vector<int> &g) {
         reserve((int)g.size(), false);
-        gidx.upload(g.data(), g.size(), s);
+        upload_if_changed(gidx, h_gidx, g.data(), g.size(), s);
     }
 
     void ensure_f32() {
@@ -2231,12 +2248,12 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
     ResolventStaging &st = stage ? *stage : own;
     DevBuf<double> &eu = st.eu, &ec = st.ec, &qe = st.qe, &dE = st.E;
     DevBuf<int> &dw = st.which, &doff = st.offset;
-    eu.upload(swp.e_SWP_unco, std::max<size_t>(nu, 1), s);
-    if (nc) ec.upload(swp.e_SWP_coup, nc, s);
-    qe.upload(swp.q_WP, Nq + 1, s);
+    ResolventStaging::put(eu, st.h_eu, swp.e_SWP_unco, std::max<size_t>(nu, 1), s);
+    if (nc) ResolventStaging::put(ec, st.h_ec, swp.e_SWP_coup, nc, s);
+    ResolventStaging::put(qe, st.h_qe, swp.q_WP, (size_t)Nq + 1, s);
     dE.upload(E, nE, s);
-    dw.upload(which.data(), Na, s);
-    doff.upload(offset.data(), Na, s);
+    ResolventStaging::put(dw, st.h_which, which.data(), (size_t)Na, s);
+    ResolventStaging::put(doff, st.h_offset, offset.data(), (size_t)Na, s);
     if (partials)
         hipLaunchKernelGGL(k_resolvent_partials, dim3(nblk(dim * nE, 256)), dim3(256), 0, s, dG, eu.p, ec.p, dw.p,
                            doff.p, qe.p, dE.p, Na, Nq, Np, nE);
@@ -2265,6 +2282,8 @@ struct SolvePlan {
     // on-shell index lists and the gathered U elements (pinned host copy)
     DevBuf<int64_t> unit_idx, os_idx;
     DevBuf<int> os_k;
+    std::vector<int64_t> h_os_idx;   // what os_idx and os_k hold
+    std::vector<int> h_os_k;
     DevBuf<double2> os_out;
     double2 *os_out_h = nullptr;
     size_t os_out_cap = 0;
@@ -2325,8 +2344,8 @@ struct SolvePlan {
             os_out_cap = n;
             os_out.alloc(n);
         }
-        os_idx.upload(idx.data(), n, op.s);
-        os_k.upload(kk.data(), n, op.s);
+        upload_if_changed(os_idx, h_os_idx, idx.data(), (size_t)n, op.s);
+        upload_if_changed(os_k, h_os_k, kk.data(), (size_t)n, op.s);
         hipLaunchKernelGGL(k_gather, dim3(nblk(n, 64)), dim3(64), 0, op.s, xv, os_idx.p, os_k.p, os_out.p, n, L.r);
         TT_HIP(hipMemcpyAsync(os_out_h, os_out.p, sizeof(double2) * n, hipMemcpyDeviceToHost, op.s));
     }
