@@ -2126,9 +2126,9 @@ static void gather_onshell(RowOperator &op, const double2 *rows, const OnshellLa
 }
 
 // Bout = the on-shell rows e_k^T A (k = (d_row, j)); with scratch (dim * r elements) and idx the
-// unit rows are formed there instead of in buffers of their own.
+// unit rows are formed there instead of in buffers of their own (h_idx: what idx holds).
 static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 *Bout, double2 *scratch = nullptr,
-                               DevBuf<int64_t> *idx = nullptr) {
+                               DevBuf<int64_t> *idx = nullptr, std::vector<int64_t> *h_idx = nullptr) {
     DevBuf<double2> own;
     DevBuf<int64_t> own_idx;
     if (!scratch) {
@@ -2137,7 +2137,10 @@ static void on_shell_rows_of_A(RowOperator &op, const OnshellLayout &L, double2 
     }
     if (!idx) idx = &own_idx;
     TT_HIP(hipMemsetAsync(scratch, 0, sizeof(double2) * (size_t)op.dim * L.r, op.s));
-    idx->upload(L.row_idx.data(), L.r, op.s);
+    if (h_idx)
+        upload_if_changed(*idx, *h_idx, L.row_idx.data(), (size_t)L.r, op.s);
+    else
+        idx->upload(L.row_idx.data(), L.r, op.s);
     hipLaunchKernelGGL(k_unit_rows, dim3(nblk(L.r, 64)), dim3(64), 0, op.s, scratch, idx->p, L.r);
     // The unit rows, and so the input of the cell product, are zero outside the on-shell q
     // bins: only the cells reading those bins take part (untimed, as it moves fewer bytes than
@@ -2282,7 +2285,7 @@ struct SolvePlan {
     // on-shell index lists and the gathered U elements (pinned host copy)
     DevBuf<int64_t> unit_idx, os_idx;
     DevBuf<int> os_k;
-    std::vector<int64_t> h_os_idx;   // what os_idx and os_k hold
+    std::vector<int64_t> h_os_idx, h_unit_idx;   // what os_idx, os_k and unit_idx hold
     std::vector<int> h_os_k;
     DevBuf<double2> os_out;
     double2 *os_out_h = nullptr;
@@ -2505,7 +2508,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
         const int r = mb.L.r;
         pl.op.set_gidx(mb.L.row_g);
         pl.fit(r);
-        on_shell_rows_of_A(pl.op, mb.L, pl.b.p, pl.rres.p, &pl.unit_idx);
+        on_shell_rows_of_A(pl.op, mb.L, pl.b.p, pl.rres.p, &pl.unit_idx, &pl.h_unit_idx);
         TT_HIP(hipMemsetAsync(pl.x.p, 0, sizeof(double2) * (size_t)pl.op.dim * r, pl.op.s));
         if (mb.mixed)
             k32[i] = &pl.krylov<float2>(r, m, graphs);
