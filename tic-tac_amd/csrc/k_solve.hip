@@ -331,7 +331,8 @@ struct Mfma16<float> {
 // (cell-major per item) and Xg = in_scale * (sum of the ket's item panels, in item order).
 // CT: the type the tile is staged and summed in (double; float for the FP32 vectors of the inner
 // solves, whose blocks are then FP32 copies: half the LDS, so more workgroups per CU hide the
-// op's load latency).
+// op's load latency).  GT: the type of G (double2; float2, an FP32 copy of G, for the FP32
+// vectors, whose tile is rounded to FP32 anyway: half the bytes of the op's largest input).
 template <typename CT>
 struct Cplx;
 template <>
@@ -343,10 +344,10 @@ struct Cplx<float> {
     typedef float2 type;
 };
 
-template <typename TI, typename TO, typename CT = double>
+template <typename TI, typename TO, typename CT = double, typename GT = double2>
 __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
                                                const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
-                                               const CT *__restrict__ blocks, const double2 *__restrict__ G,
+                                               const CT *__restrict__ blocks, const GT *__restrict__ G,
                                                const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
                                                int ne_max, const TO *__restrict__ Zsub, int in_cell,
                                                int out_cell, const int *__restrict__ ket_item_ptr,
@@ -390,7 +391,8 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         if (G) {
             const int g = gidx[k];
             if (g >= 0) {
-                const double2 gv = G[(int64_t)g * dim + src];
+                const GT gv0 = G[(int64_t)g * dim + src];
+                const double2 gv = make_double2((double)gv0.x, (double)gv0.y);
                 const double tr = xr * gv.x - xi * gv.y;
                 xi = xr * gv.y + xi * gv.x;
                 xr = tr;
@@ -426,9 +428,15 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
 // (every chunk stages all partner blocks, so a narrow remainder chunk costs a full one).
-template <typename TI, typename TO, typename CT = double>
+// G's type comes from the template arguments only (G may be passed as nullptr)
+template <typename T>
+struct NoDeduce {
+    typedef T type;
+};
+template <typename TI, typename TO, typename CT = double, typename GT = double2>
 static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
-                         const CT *blocks, int ne_max, const double2 *G, const int *gidx, int Na, int Nq, int Np,
+                         const CT *blocks, int ne_max, const typename NoDeduce<GT>::type *G, const int *gidx, int Na,
+                         int Nq, int Np,
                          int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
                          double in_scale, double2 *zero_buf = nullptr, int zero_n = 0) {
     const int nch = (r + 23) / 24;
@@ -437,7 +445,7 @@ static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr,
                        (size_t)ne_max * Np * kc * sizeof(typename Cplx<CT>::type);
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
-    hipLaunchKernelGGL((k_block<TI, TO, CT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
+    hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
                        Nq, Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n);
 }
 
@@ -1133,6 +1141,11 @@ static TuneTable &tune_table() {
 
 // Device copies of the resolvent's inputs (SWP edges, q edges, energies, per-alpha edge
 // offsets), kept by an operator so that computing its G allocates nothing and needs no wait.
+__global__ void k_g_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = make_float2((float)in[t].x, (float)in[t].y);
+}
+
 struct ResolventStaging {
     DevBuf<double> eu, ec, qe, E;
     DevBuf<int> which, offset;
@@ -1196,6 +1209,8 @@ struct RowOperator {
     DevBuf<double> c_blocks, vc_blocks;
     DevBuf<float> c_blocks32, vc_blocks32;   // FP32 copies for the block ops on FP32 vectors
     DevBuf<double2> G;
+    DevBuf<float2> G32;                 // FP32 copy of G (the FP32 vectors' block op)
+    uint64_t g_gen = 0, g32_gen = ~0ull;
     DevBuf<double2> S, U;      // FP64 scratch (dim * r)
     DevBuf<float2> S32, U32;   // FP32 scratch
     DevBuf<int> gidx;
@@ -1339,6 +1354,7 @@ struct RowOperator {
 
     // Device buffer for the resolvent of nE energies (contents written by the caller).
     double2 *G_buffer(int nE_) {
+        ++g_gen;
         nE = nE_;
         const size_t n = (size_t)std::max(nE, 1) * dim;
         if (n > G.n) {
@@ -1350,6 +1366,7 @@ struct RowOperator {
 
     // Resolvent of nE energies (host, [j][idx] complex), copied into the device buffer.
     void set_G(const double *Gh, int nE_) {
+        ++g_gen;
         nE = nE_;
         if (nE <= 0 || !Gh) return;
         const size_t n = (size_t)nE * dim;
@@ -1383,6 +1400,14 @@ struct RowOperator {
     void prepare(int r, bool fp32) {
         reserve(r, fp32);
         if (fp32) ensure_f32();
+        if (fp32 && G.p && g32_gen != g_gen) {   // FP32 copy of G for the FP32 block ops
+            if (G32.n < G.n) {
+                G32.alloc(G.n);
+                ++layout_gen;
+            }
+            hipLaunchKernelGGL(k_g_to_f32, dim3(nblk((int64_t)G.n, 256)), dim3(256), 0, s, G.p, G32.p, (int64_t)G.n);
+            g32_gen = g_gen;
+        }
         if (cells) {
             tune(r, fp32);
             const size_t need = (size_t)std::max(items_for(fp32, 2 * r).n, 1) * Na * 2 * r;
@@ -1626,8 +1651,8 @@ struct RowOperator {
                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
                                          zb, zn);
             else
-                launch_block<VT, float2, float>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks32.p, c_npart_max,
-                                                use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
+                launch_block<VT, float2, float, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks32.p, c_npart_max,
+                                                use_G ? G32.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
                                                 1.0, zb, zn);
             if (timed) record(e1);
             if (cells)
