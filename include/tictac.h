@@ -225,7 +225,10 @@ typedef enum {
     TT_SOLVE_MIXED = 3          /* FP32 products + FP64 iterative refinement                 */
 } tt_solve_method;
 
-/* How the product over P123 is formed (results agree to rounding). */
+/* How the product over P123 is formed (results agree to rounding).  The cell-block form covers
+ * channels with Nalpha <= 64 (J_2N_max <= 3); a solve of a larger channel (J_2N_max >= 4 gives
+ * Nalpha up to 139) asking for TT_P123_CELLS takes the CSC gather instead.  tt_solve_gradient and
+ * the column operator of tt_apply_rows need the cell-block form: TT_ERR_UNSUPPORTED above 64. */
 typedef enum {
     TT_P123_CELLS = 0,          /* dense Nalpha x Nalpha blocks per overlapping bin cell (default) */
     TT_P123_CSC = 1             /* gather over the reference's CSC arrays                          */

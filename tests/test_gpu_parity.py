@@ -529,3 +529,79 @@ def test_breakup_amplitudes_in_batch(ctx, c1, c1bu, method):
     assert res[0][2] == tt._lib.TT_OK
     assert np.array_equal(np.isnan(ch.U_BU), ~ok)
     assert np.abs(ch.U_BU[ok] - Ud[ok]).max() <= 1e-9 * np.abs(Ud[ok]).max()
+
+
+# ------------------------------------------------------------------ plan identity and failure paths
+def _oracle_dense_U(prob, ch, P, G, Vu=None, Vc=None):
+    val, row, ptr = P.to_host()
+    tf, ib = flags(prob)
+    if Vu is None:
+        Vu, Vc = oracle_V(prob)
+    K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
+    return so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, prob.rp.Nq_WP, prob.rp.Np_WP)
+
+
+def test_plan_follows_sign_flips_of_V(ctx):
+    """The cached plan compares V and C word for word: flipping the sign of V (or of one
+    symmetric pair of its elements) with the SWP basis unchanged gives the oracle's result
+    for the new V, not the cached operator's."""
+    rp, pot, T = synthetic_config(N=10, J_2N_max=1)
+    prob = tt.Problem(rp, pot)
+    ch = prob.channel(0, T)
+    P = ch.build_p123(ctx)
+    G = ch.resolvent(ctx)
+    Np = rp.Np_WP
+    V0u, V0c = prob.V_unco.copy(), prob.V_coup.copy()
+    try:
+        U0, _ = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES)
+        prob.V_unco[:] = -V0u
+        prob.V_coup[:] = -V0c
+        U1, _ = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES)
+        assert np.abs(U1 - U0).max() > 1e-3 * np.abs(U0).max()
+        Uo = _oracle_dense_U(prob, ch, P, G)
+        assert np.abs(U1 - Uo).max() <= 1e-10 * np.abs(Uo).max()
+        prob.V_unco[:] = V0u
+        prob.V_coup[:] = V0c
+        i, j = 2, 5   # one symmetric pair of the first uncoupled block
+        prob.V_unco[i * Np + j] *= -1.0
+        prob.V_unco[j * Np + i] *= -1.0
+        U2, _ = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES)
+        Uo = _oracle_dense_U(prob, ch, P, G)
+        assert np.abs(U2 - Uo).max() <= 1e-10 * np.abs(Uo).max()
+    finally:
+        prob.V_unco[:] = V0u
+        prob.V_coup[:] = V0c
+
+
+@pytest.mark.parametrize("method", [tt.SOLVE_GMRES, tt.SOLVE_MIXED])
+def test_nan_resolvent_is_reported_not_converged(ctx, c1, method):
+    """A NaN in G makes the residual NaN, which no tolerance accepts: the solve returns
+    TT_ERR_NOT_CONVERGED instead of a NaN (or zero) U with TT_OK."""
+    G = c1.G.copy()
+    G[0, 7] = np.nan
+    with pytest.raises(tt.TictacError) as e:
+        c1.ch.solve(ctx, c1.P, G, method=method)
+    assert e.value.code == tt._lib.TT_ERR_NOT_CONVERGED
+    U, st = c1.ch.solve(ctx, c1.P, c1.G, method=method)   # the plan still solves good input
+    assert np.abs(U - c1.Ud).max() <= 1e-10 * np.abs(c1.Ud).max()
+
+
+def test_channel_above_64_alphas_takes_the_csc_gather(ctx):
+    """J_2N_max = 4 gives a 5/2+ channel with N_alpha = 83, beyond the cell-block product:
+    the default solve takes the CSC gather and agrees with the oracle's dense solve; the
+    gradient, which needs the cell blocks, reports TT_ERR_UNSUPPORTED."""
+    rp, pot, T = synthetic_config(N=6, J_2N_max=4, two_J_3N_max=5, T_lab=(10.0,))
+    prob = tt.Problem(rp, pot)
+    c = max(range(prob.n_channels), key=lambda i: prob.channel(i, T).Nalpha)
+    ch = prob.channel(c, T)
+    assert ch.Nalpha > 64
+    P = ch.build_p123(ctx)
+    G = ch.resolvent(ctx)
+    U, st = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES)
+    Uo = _oracle_dense_U(prob, ch, P, G)
+    assert np.abs(U - Uo).max() <= 1e-10 * np.abs(Uo).max()
+    Umx, _ = ch.solve(ctx, P, G, method=tt.SOLVE_MIXED)
+    assert np.abs(Umx - Uo).max() <= 1e-8 * np.abs(Uo).max()
+    with pytest.raises(tt.TictacError) as e:
+        prob.gradient(ctx, c, P, tt.synthetic_params()[:12], T, basis=prob.basis_matrices(12))
+    assert e.value.code == tt._lib.TT_ERR_UNSUPPORTED

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -527,6 +528,9 @@ __global__ __launch_bounds__(256) void k_spmm_gather(const uint64_t *__restrict_
 // autotuner (RowOperator::tune); the defaults below are its starting point (measured at
 // Nalpha = 27, R2 = 8 on 22k cells).
 constexpr int CELL_ITEM = 16;
+// Largest N_alpha of the cell-block product (four 16-row M tiles).  Channels with more partial
+// waves (J_2N_max >= 4 gives N_alpha up to 139) take the gather over the CSC instead.
+constexpr int CELL_NA_MAX = 64;
 template <typename T>
 constexpr int cell_un() {
     return sizeof(T) == 8 ? 2 : 4;
@@ -804,7 +808,7 @@ static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems
 template <typename T>
 static void launch_cellmm(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S, T *U,
                           T *Upart, int R2, T scale, int un = 0) {
-    TT_REQUIRE(P->Na >= 1 && P->Na <= 64, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
+    TT_REQUIRE(P->Na >= 1 && P->Na <= CELL_NA_MAX, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
     if (ci.n > 0) {
         if (P->Na <= 16)
             launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2, un);
@@ -1166,10 +1170,32 @@ static void upload_if_changed(DevBuf<T> &d, std::vector<T> &last, const T *h, si
     ResolventStaging::put(d, last, h, n, s);
 }
 
+// Exact identity of the inputs a plan's block contents are built from (V and the SWP
+// eigenvectors): a copy of their words, with a hash that mixes every bit of every word into the
+// whole key, so that most mismatches are rejected without comparing the words.  Two contents
+// are the same only when their words are equal bit for bit.
+struct Content {
+    std::vector<double> words;
+    uint64_t hash = 0;
+};
+typedef std::shared_ptr<const Content> ContentRef;
+
+static inline uint64_t mix64(uint64_t z) {   // splitmix64 finaliser (a bijection of 64-bit words)
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+static bool same_content(const ContentRef &a, const ContentRef &b) {
+    if (a == b) return true;
+    if (!a || !b || a->hash != b->hash || a->words.size() != b->words.size()) return false;
+    return a->words.empty() || std::memcmp(a->words.data(), b->words.data(), sizeof(double) * a->words.size()) == 0;
+}
+
 // V.C of every 2N channel (dot_MM in create_VC_col_maj_3N_pointer_array,
 // CPP/solve_faddeev.cpp:1828-1855): row-major Np x Np (uncoupled) and 2Np x 2Np (coupled).
 struct VCProducts {
-    uint64_t content = 0;   // plan_content_key of the V and C they were made from
+    ContentRef content;     // the V and C they were made from
     bool valid = false;
     std::vector<double> u, c;
 };
@@ -1758,6 +1784,9 @@ static cdouble pade_approximant(const cdouble *a, int N, int M, cdouble z) {
     return det_lu(P, M + 1) / det_lu(Q, M + 1);
 }
 
+// sqrt of a squared norm: rounding below zero gives 0, NaN stays NaN (std::max(0.0, NaN) is 0)
+static inline double sqrt_nonneg(double n2) { return std::isnan(n2) ? n2 : std::sqrt(n2 > 0.0 ? n2 : 0.0); }
+
 // ------------------------------------------------------------------ Krylov engine
 // GMRES on r independent systems M y_k = b_k, M y = y - (y .* G_k) A, that share every
 // operator application: one Arnoldi basis per system (column k of the basis vectors).  VT is
@@ -1863,9 +1892,11 @@ struct Krylov : KrylovBase {
         dots_dev(Wv, 1, Wv, n2d.p);
         TT_HIP(hipMemcpyAsync(n2h, n2d.p, sizeof(double2) * r, hipMemcpyDeviceToHost, op.s));
     }
+    // (a NaN squared norm stays NaN, so that a NaN residual or right-hand side is never taken
+    // for a converged one)
     std::vector<double> read_norms() const {
         std::vector<double> n(r);
-        for (int k = 0; k < r; ++k) n[k] = std::sqrt(std::max(0.0, n2h[k].x));
+        for (int k = 0; k < r; ++k) n[k] = sqrt_nonneg(n2h[k].x);
         return n;
     }
     std::vector<double> norms(const double2 *Wv) {
@@ -1986,7 +2017,7 @@ struct Krylov : KrylovBase {
                 const double2 a1 = hj[(size_t)i * r + k], a2 = hj[(size_t)(m + 1 + i) * r + k];
                 Hk(i, j) = cdouble(a1.x + a2.x, a1.y + a2.y);
             }
-            Hk(j + 1, j) = std::sqrt(std::max(0.0, hj[(size_t)2 * (m + 1) * r + k].x));
+            Hk(j + 1, j) = sqrt_nonneg(hj[(size_t)2 * (m + 1) * r + k].x);
             for (int i = 0; i < j; ++i) {
                 cdouble c = cs[(size_t)i * r + k], sv = sn[(size_t)i * r + k];
                 cdouble t0 = std::conj(c) * Hk(i, j) + std::conj(sv) * Hk(i + 1, j);
@@ -2299,7 +2330,9 @@ static void resolvent_on_device(hipStream_t s, const tt_run_params &rp, const tt
 // solution vectors.  Cached per context, keyed by the P123 object and a hash of the inputs
 // the operator is built from; only G (the energies) changes between reuses.
 struct SolvePlan {
-    uint64_t p_gen = 0, key = 0, content = 0;
+    uint64_t p_gen = 0;
+    std::vector<int> key;   // plan_struct_key
+    ContentRef content;     // the V and C its blocks hold
     int format = TT_P123_CELLS;
     bool col = false;   // column operator (adjoint solves)
     uint64_t last_use = 0;
@@ -2396,58 +2429,53 @@ struct PlanCache {
 
 void free_plan_cache(PlanCache *c) { delete c; }
 
-static uint64_t hash_bytes(uint64_t h, const void *data, size_t n) {
-    // FNV-1a over 64-bit words (tail bytes folded in one at a time)
-    const unsigned char *b = static_cast<const unsigned char *>(data);
-    size_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        uint64_t w;
-        std::memcpy(&w, b + i, 8);
-        h = (h ^ w) * 0x100000001b3ull;
-    }
-    for (; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
-    return h;
-}
-
 // Structure of a plan: everything that fixes the operator's layout (flags, sizes, the
-// channel's quantum numbers).  Its contents (V and the SWP eigenvectors) are hashed apart.
-static uint64_t plan_struct_key(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
-                                int format) {
-    uint64_t h = 0xcbf29ce484222325ull;
-    const int flags[6] = {rp.tensor_force, rp.isospin_breaking_1S0, swp.Np_WP, swp.Nq_WP, chn.Nalpha, format};
-    h = hash_bytes(h, flags, sizeof(flags));
+// channel's quantum numbers), compared word for word.  Its contents (V and the SWP
+// eigenvectors) are compared apart (Content).
+static std::vector<int> plan_struct_key(const tt_run_params &rp, const tt_swp_states &swp, const tt_pw_states &chn,
+                                        int format) {
+    std::vector<int> k = {rp.tensor_force, rp.isospin_breaking_1S0, swp.Np_WP, swp.Nq_WP, chn.Nalpha, format};
     const int *arrs[] = {chn.L_2N, chn.S_2N, chn.J_2N, chn.T_2N, chn.L_1N, chn.two_J_1N, chn.two_J_3N,
                          chn.two_T_3N, chn.P_3N};
-    for (const int *a : arrs) h = hash_bytes(h, a, sizeof(int) * chn.Nalpha);
-    return h;
+    for (const int *a : arrs) k.insert(k.end(), a, a + chn.Nalpha);
+    return k;
 }
 
-static uint64_t plan_content_key(const tt_run_params &rp, const double *V_unco, const double *V_coup,
-                                 const tt_swp_states &swp) {
+static ContentRef plan_content(const tt_run_params &rp, const double *V_unco, const double *V_coup,
+                               const tt_swp_states &swp) {
     int nu, nc;
     num_2N_states(rp, nu, nc);
-    const size_t Np = (size_t)swp.Np_WP;
-    uint64_t h = 0x84222325cbf29ce4ull;
-    h = hash_bytes(h, V_unco, sizeof(double) * nu * Np * Np);
-    if (nc) h = hash_bytes(h, V_coup, sizeof(double) * nc * 4 * Np * Np);
-    h = hash_bytes(h, swp.C_SWP_unco, sizeof(double) * nu * Np * Np);
-    if (nc) h = hash_bytes(h, swp.C_SWP_coup, sizeof(double) * nc * 4 * Np * Np);
-    return h;
+    const size_t Np = (size_t)swp.Np_WP, nu2 = nu * Np * Np, nc2 = nc * 4 * Np * Np;
+    std::shared_ptr<Content> c(new Content);
+    std::vector<double> &w = c->words;
+    w.reserve(2 * (nu2 + nc2));
+    w.insert(w.end(), V_unco, V_unco + nu2);
+    if (nc) w.insert(w.end(), V_coup, V_coup + nc2);
+    w.insert(w.end(), swp.C_SWP_unco, swp.C_SWP_unco + nu2);
+    if (nc) w.insert(w.end(), swp.C_SWP_coup, swp.C_SWP_coup + nc2);
+    uint64_t h = mix64(w.size());
+    for (size_t i = 0; i < w.size(); ++i) {
+        uint64_t b;
+        std::memcpy(&b, &w[i], 8);
+        h = mix64(h ^ mix64(b + 0x9e3779b97f4a7c15ull * (i + 1)));
+    }
+    c->hash = h;
+    return c;
 }
 
 // The context's plan for these inputs: found by the P123 object and the structure, its block
 // contents refreshed when V or C changed (parameter sweeps, gradients), else built (evicting
 // the least recently used plan beyond max_plans).
-// content_key: plan_content_key of V and C when the caller has it (one per batch call).
+// content_key: plan_content of V and C when the caller has it (one per batch call).
 static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC *P, const double *V_unco,
                            const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn, int format,
-                           bool col = false, const uint64_t *content_key = nullptr) {
+                           bool col = false, const ContentRef *content_key = nullptr) {
     if (!ctx->plans) ctx->plans = new PlanCache;
     PlanCache &c = *ctx->plans;
-    const uint64_t key = plan_struct_key(rp, swp, chn, format);
-    const uint64_t content = content_key ? *content_key : plan_content_key(rp, V_unco, V_coup, swp);
+    const std::vector<int> key = plan_struct_key(rp, swp, chn, format);
+    const ContentRef content = content_key ? *content_key : plan_content(rp, V_unco, V_coup, swp);
     auto products = [&]() -> const VCProducts & {
-        if (!c.vc.valid || c.vc.content != content) {
+        if (!c.vc.valid || !same_content(c.vc.content, content)) {
             c.vc.valid = false;
             vc_products(rp, V_unco, V_coup, swp, c.vc);
             c.vc.content = content;
@@ -2457,7 +2485,7 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
     for (auto &pl : c.plans)
         if (pl->p_gen == P->gen && pl->key == key && pl->format == format && pl->col == col) {
             pl->last_use = ++c.tick;
-            if (pl->content != content) {
+            if (!same_content(pl->content, content)) {
                 pl->op.set_blocks(rp, V_unco, V_coup, swp, chn, &products());
                 pl->content = content;
             }
@@ -2546,9 +2574,15 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
     }
     sync_all();
     for (size_t i = 0; i < n; ++i) ms[i].bnorm = read_norms(i);
+    // largest relative residual over the systems; NaN when any ratio is not finite (a NaN or
+    // infinite norm), which no tolerance accepts
     auto rel_max = [](const std::vector<double> &a, const std::vector<double> &ref) {
         double v = 0.0;
-        for (size_t k = 0; k < a.size(); ++k) v = std::max(v, a[k] / std::max(ref[k], 1e-300));
+        for (size_t k = 0; k < a.size(); ++k) {
+            const double q = a[k] / (ref[k] > 1e-300 ? ref[k] : 1e-300);
+            if (!std::isfinite(q)) return std::numeric_limits<double>::quiet_NaN();
+            v = std::max(v, q);
+        }
         return v;
     };
     std::vector<std::vector<double>> beta(n);
@@ -2576,7 +2610,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
         for (size_t i : act) {
             KMember &mb = ms[i];
             mb.res = rel_max(beta[i], mb.bnorm);
-            if (mb.res <= o.tol || mb.iters >= o.max_iter || (mb.mixed && outer >= 20)) {
+            if (mb.res <= o.tol || std::isnan(mb.res) || mb.iters >= o.max_iter || (mb.mixed && outer >= 20)) {
                 mb.done = true;
                 continue;
             }
@@ -2889,14 +2923,16 @@ namespace tt {
 static SolvePlan &find_plan(tt_ctx *ctx, const tt_run_params *rp, const double *G, const tt_csc *P123,
                             const double *V_unco, const double *V_coup, const tt_swp_states *swp,
                             const tt_channel_os *os, const tt_pw_states *chn, const tt_solve_opts &o,
-                            const uint64_t *content_key = nullptr) {
+                            const ContentRef *content_key = nullptr) {
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && os && chn, "null argument");
     TT_REQUIRE(G || o.resolvent_on_device, "G required unless opts->resolvent_on_device is set");
     TT_REQUIRE(os->num_deuteron_states > 0 && os->num_T_lab > 0, "no on-shell elements to solve for");
     TT_HIP(hipSetDevice(ctx->device));
     const DeviceCSC *P = dev_of(P123);
     TT_REQUIRE(P->dim == (int64_t)chn->Nalpha * swp->Nq_WP * swp->Np_WP, "P123 dimension does not match Nalpha*Nq*Np");
-    return get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, o.p123_format, false, content_key);
+    // the cell-block form covers N_alpha <= CELL_NA_MAX; larger channels take the CSC gather
+    const int format = o.p123_format == TT_P123_CELLS && chn->Nalpha > CELL_NA_MAX ? TT_P123_CSC : o.p123_format;
+    return get_plan(ctx, *rp, P, V_unco, V_coup, *swp, *chn, format, false, content_key);
 }
 
 // G of the solve in place in the plan (computed on the device or uploaded) and the plan's
@@ -3030,8 +3066,8 @@ static void solve_batch(tt_ctx *ctx, const tt_run_params *rp, const double *V_un
         return;
     }
     const int group_max = (int)PlanCache::max_plans;
-    // V and C are the same for every item: hash them once
-    const uint64_t content = plan_content_key(*rp, V_unco, V_coup, *swp);
+    // V and C are the same for every item: take their snapshot once
+    const ContentRef content = plan_content(*rp, V_unco, V_coup, *swp);
     int i0 = 0;
     while (i0 < n_items) {
         auto t0 = std::chrono::steady_clock::now();
@@ -3144,6 +3180,9 @@ static void solve_gradient(tt_ctx *ctx, const tt_run_params &rp, const tt_fwp_st
     check(tt_make_swp_states(&rp, &fwp, &pw_full, Vu.data(), Vc.data(), &swp));
     tt_pw_states chn;
     check(tt_pw_channel(&pw_full, channel, &chn));
+    if (dU && chn.Nalpha > CELL_NA_MAX)
+        fail(TT_ERR_UNSUPPORTED, "the adjoint's column operator uses the cell-block form, which covers N_alpha <= 64 "
+                                 "(this channel has " + std::to_string(chn.Nalpha) + ")");
     tt_channel_os os;
     check(tt_find_on_shell(&swp, &chn, T_lab, nT, &os));
     struct OsGuard {
@@ -3443,7 +3482,10 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
     TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
     TT_HIP(hipSetDevice(ctx->device));
     RowOperator op;
-    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, (flags & 2) ? TT_P123_CSC : TT_P123_CELLS,
+    const bool csc = (flags & 2) != 0 || chn->Nalpha > CELL_NA_MAX;   // cells cover N_alpha <= 64
+    if ((flags & 4) && chn->Nalpha > CELL_NA_MAX)
+        fail(TT_ERR_UNSUPPORTED, "the column operator uses the cell-block form, which covers N_alpha <= 64");
+    op.setup(ctx->stream, *rp, dev_of(P123), V_unco, V_coup, *swp, *chn, csc ? TT_P123_CSC : TT_P123_CELLS,
              (flags & 4) != 0);
     op.set_G(G, nE);
     std::vector<int> g(r, -1);
