@@ -8,11 +8,13 @@ the resolvent of every on-shell bin on the device and solves all channels throug
 P123, V and the SWP basis are built once before timing and stay resident in HBM.
 
 The headline value is that batched solve in mixed precision (FP32 P.V.G products, FP64 iterative
-refinement).  Beside it, on the same 144 channel-solves: the batched solve in FP64, and the plain
-FP64 path (one tt_solve_faddeev_equations call per channel and T_lab, FP64 GMRES), with the
-relative error of every mixed U element against the plain FP64 one (target <= 1e-8).  Further
-lines: the single-channel J^Pi = 1/2+ workload, Np = Nq = 40 precision, the gradient cost and the
-CPU baseline.  One JSON line on stdout.
+refinement).  Beside it, on the same 144 channel-solves: the batched solve in FP64, and three
+plain paths: FP64 GMRES with one tt_solve_faddeev_equations call per channel and T_lab, FP64
+GMRES with one call per channel holding all 8 T_lab (the reference driver's granularity), and
+the reference's own method (Neumann series + Pade) at that granularity; with the relative error
+of every mixed U element against the plain FP64 one (target <= 1e-8).  Further lines: the
+single-channel J^Pi = 1/2+ workload, Np = Nq = 40 precision, the gradient cost and the CPU
+baseline.  One JSON line on stdout.
 """
 import argparse
 import json
@@ -28,6 +30,7 @@ import numpy as np  # noqa: E402
 import tictac as tt  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # dense MFMA peaks of MI355X
 # streams a batched lockstep group is spread over: the library reads TT_LANES (1..16), else 8
 LANES = int(os.environ["TT_LANES"]) if os.environ.get("TT_LANES", "").isdigit() and \
     1 <= int(os.environ["TT_LANES"]) <= 16 else 8
@@ -72,6 +75,20 @@ class BatchWorkload:
         return [[ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)[0] for ch in row]
                 for row, P in zip(self.plain, self.Ps)]
 
+    def solve_per_channel(self, ctx, method=tt.SOLVE_GMRES):
+        """The reference driver's call granularity (CPP/main.cpp:386-435): one
+        tt_solve_faddeev_equations call per channel holding all 8 T_lab, channels one after
+        another on one stream."""
+        return [ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)[0]
+                for ch, P in zip(self.chans, self.Ps)]
+
+    def channel_table(self):
+        """Per channel: J^Pi, N_alpha, dim, overlapping bin cells, nonzeros of P123 and the
+        number of right-hand sides r = n_d x on-shell bins of the batched solve."""
+        return [{"JP": "%d/2%s" % (ch.two_J, "+" if ch.parity > 0 else "-"), "Nalpha": ch.Nalpha, "dim": ch.dim,
+                 "ncell": int(P.stats.n_cells), "nnz": int(P.nnz), "r": int(ch.nd * ch.nE)}
+                for ch, P in zip(self.chans, self.Ps)]
+
     def accuracy(self, U_batch, U_plain):
         """Relative error of every on-shell element of the batched U against the plain one of the same
         channel and bin: max |dU|/|U| over elements, and max |dU| / max |U| per channel."""
@@ -99,31 +116,60 @@ def time_batches(ctx, wl, method, steps, warmup):
     return time.perf_counter() - t0, [u for (u, _, _) in out], stats
 
 
-def product_roofline(stats, kernel, fp32=False, traffic=None, wall_s=None, lanes=1):
-    """HBM roofline of the P123 cell-block product (the dominant kernel of the A-application):
-    algorithmic bytes per launch (every block value once, the S panel read and the U panel
-    written once, the cell indexes) over the launch time from the HIP events that bracket each
-    sampled launch on its stream during the timed steps; traffic = measured HBM bytes per launch.
+def s8d_bytes(nnz, dim, r, fp32):
+    """Algorithmic bytes of one product over P123 for r complex right-hand sides, as a CSR
+    SpMM must move them: values and int32 indices of the nonzeros, the row pointers, the
+    input panel read and the output panel written once (nnz (s_v + 4) + 4 (dim + 1) +
+    2 r dim s_c, s_v / s_c the value and complex element sizes)."""
+    sv, sc = (4.0, 8.0) if fp32 else (8.0, 16.0)
+    return nnz * (sv + 4.0) + 4.0 * (dim + 1) + 2.0 * r * dim * sc
+
+
+def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=None, lanes=1):
+    """HBM roofline of the product over P123 (the dominant kernel of the A-application).
+
+    achieved = the algorithmic bytes per launch by s8d_bytes (per-channel nonzeros, dim and r)
+    over the launch time from the HIP events that bracket each sampled launch on its stream
+    during the timed steps.  achieved_cell_bytes uses the bytes the cell-block form itself must
+    move instead (every block value once, structural zeros included, no index per value, the
+    cell indexes and the two panels), which is fewer.  traffic = measured HBM bytes per launch
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None.  mfma: the same launches
+    against the MFMA roof of their precision, at 4 real flops per nonzero and complex
+    right-hand side (a real value times a complex element: two fused multiply-adds).
     With lanes > 1 the channels' streams run concurrently, so a launch shares the device with
     the other lanes' kernels and its event time is its duration under that sharing.  With
-    wall_s, aggregate_achieved = the algorithmic bytes of every product launch of the timed
-    steps (sampled or not) over their wall time: a lower bound on the products' HBM rate."""
-    if fp32:   # only the FP32 launches of the mixed solves (its FP64 residual applications are apart)
-        ms = sum(s.ms_spmm_fp32 for s in stats)
-        nb = sum(s.spmm_bytes_fp32 for s in stats)
-        n = sum(s.n_spmm_fp32 for s in stats)
-        every = sum(s.n_apply_fp32 * s.spmm_bytes_fp32 / s.n_spmm_fp32 for s in stats if s.n_spmm_fp32)
-    else:
-        ms = sum(s.ms_spmm for s in stats)
-        nb = sum(s.spmm_bytes for s in stats)
-        n = sum(s.n_spmm for s in stats)
-        every = sum((s.n_apply - s.n_apply_fp32) * (s.spmm_bytes - s.spmm_bytes_fp32) / (s.n_spmm - s.n_spmm_fp32)
-                    for s in stats if s.n_spmm > s.n_spmm_fp32)
-    achieved = nb / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    wall_s, aggregate_achieved = the s8d bytes of every product launch of the timed steps
+    (sampled or not) over their wall time: a lower bound on the products' HBM rate.
+    stats[i] belongs to the channel of shapes[i % len(shapes)] (nnz, dim, r)."""
+    ms = nb = b8 = fl = every = 0.0
+    n = 0
+    for i, s in enumerate(stats):
+        nnz, dim, r = shapes[i % len(shapes)]
+        if fp32:   # only the FP32 launches of the mixed solves (its FP64 residual applications are apart)
+            k, t, cb, tot = s.n_spmm_fp32, s.ms_spmm_fp32, s.spmm_bytes_fp32, s.n_apply_fp32
+        else:
+            k, t, cb = s.n_spmm - s.n_spmm_fp32, s.ms_spmm - s.ms_spmm_fp32, s.spmm_bytes - s.spmm_bytes_fp32
+            tot = s.n_apply - s.n_apply_fp32
+        n += k
+        ms += t
+        nb += cb
+        b8 += k * s8d_bytes(nnz, dim, r, fp32)
+        fl += k * 4.0 * nnz * r
+        every += tot * s8d_bytes(nnz, dim, r, fp32)
+    sec = ms * 1e-3
+    achieved = b8 / sec / 1e9 if ms > 0 else 0.0
+    peak_tf = MFMA_PEAK_TFLOPS["fp32" if fp32 else "fp64"]
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
            "kernel": kernel, "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
-           "bytes_per_launch": round(nb / max(n, 1)), "concurrent_streams": lanes}
+           "bytes_per_launch": round(b8 / max(n, 1)),
+           "achieved_cell_bytes": round(nb / sec / 1e9, 1) if ms > 0 else 0.0,
+           "cell_bytes_per_launch": round(nb / max(n, 1)),
+           "mfma": {"flops_per_launch": round(fl / max(n, 1)),
+                    "achieved_tflops": round(fl / sec / 1e12, 2) if ms > 0 else 0.0,
+                    "peak_tflops": peak_tf,
+                    "frac": round(fl / sec / 1e12 / peak_tf, 4) if ms > 0 else 0.0},
+           "concurrent_streams": lanes}
     if wall_s:
         out["aggregate_achieved"] = round(every / wall_s / 1e9, 1)
     return out
@@ -135,8 +181,9 @@ def iteration_summary(stats, n_items):
     return {"mean": round(sum(last) / max(len(last), 1), 1), "max": max(last) if last else 0}
 
 
+# rocprofv3 summaries made by this bench.py's own options (each file gives its commands)
 TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
-                 ("pmc_cellmfma_synthetic_np30_batch.md", "pmc_cellmfma_synthetic_np30.md")]
+                 ("pmc_cellmfma_synthetic_np30_batch.md", "pmc_cellmfma_synthetic_np30_single.md")]
 
 
 def measured_traffic(kernel):
@@ -177,13 +224,14 @@ def single_channel(ctx, steps, warmup):
     v_mix, U_mix, st_mix = run(tt.SOLVE_MIXED, False, steps, warmup)
     v_64, U_64, st_64 = run(tt.SOLVE_GMRES, False, steps, warmup)
     v_pcie, _, _ = run(tt.SOLVE_MIXED, True, steps, 1)
+    shape = [(int(P.nnz), ch.dim, ch.nd * ch.nE)]
     return {"workload": SINGLE_WORKLOAD, "unit": "channel-solves/s", "nnz_P123": int(P.nnz),
             "p123_build_ms": round(P.stats.ms_total, 2), "mixed": round(v_mix, 2), "fp64": round(v_64, 2),
             "mixed_pcie_inclusive": round(v_pcie, 2),
             "mixed_vs_fp64_max_rel": float(np.max(np.abs(U_mix - U_64)) / np.max(np.abs(U_64))),
-            "roofline_fp32": product_roofline(st_mix, "k_cellmfma<float, 2, 1, 4>", True,
+            "roofline_fp32": product_roofline(st_mix, shape, "k_cellmfma<float, 2, 1, 4>", True,
                                               measured_traffic("k_cellmfma<float, 2, 1, 4>")),
-            "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, 2, 1, 2>", False,
+            "roofline_fp64": product_roofline(st_64, shape, "k_cellmfma<double, 2, 1, 2>", False,
                                               measured_traffic("k_cellmfma<double, 2, 1, 2>"))}
 
 
@@ -400,6 +448,8 @@ def main():
     value = wl.n_solves * args.steps / t_mix
     fp64 = wl.n_solves * n64 / t_64
     k32 = "k_cellmfma<float, MT, NT, UN>"
+    table = wl.channel_table()
+    shapes = [(c["nnz"], c["dim"], c["r"]) for c in table]
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -415,14 +465,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": BATCH_WORKLOAD, "channel_solves_per_step": wl.n_solves,
                    "nnz_P123_total": int(sum(P.nnz for P in wl.Ps)),
-                   "p123_build_s_all_channels": round(wl.build_s, 2), "solve_tol": SOLVE_TOL},
+                   "p123_build_s_all_channels": round(wl.build_s, 2), "solve_tol": SOLVE_TOL,
+                   "channels": table},
         "batched_fp64": round(fp64, 2),
         "mixed_over_batched_fp64": round(value / fp64, 3),
         "plain_fp64": None,
         "mixed_over_plain_fp64": None,
         "grad_over_fwd": None,
-        "roofline": product_roofline(st_mix, k32, True, measured_traffic(k32), t_mix, LANES),
-        "roofline_fp64": product_roofline(st_64, "k_cellmfma<double, MT, NT, UN>", False,
+        "roofline": product_roofline(st_mix, shapes, k32, True, measured_traffic(k32), t_mix, LANES),
+        "roofline_fp64": product_roofline(st_64, shapes, "k_cellmfma<double, MT, NT, UN>", False,
                                           measured_traffic("k_cellmfma<double, MT, NT, UN>"), t_64, LANES),
         "krylov_steps_per_channel": {"mixed": iteration_summary(st_mix, len(wl.chans)),
                                      "fp64": iteration_summary(st_64, len(wl.chans))},
@@ -445,6 +496,23 @@ def main():
         out["mixed_vs_plain_fp64_max_rel_norm"] = norm
         out["batched_fp64_vs_plain_fp64_max_rel_elem"] = elem64
         out["accuracy_target_met"] = bool(elem <= 1e-8)
+        # the reference driver's granularity: one call per channel with all 8 T_lab
+        log("plain FP64 and Pade: one call per channel with all energies")
+        for method, key in ((tt.SOLVE_GMRES, "plain_fp64_per_channel"), (tt.SOLVE_PADE, "plain_pade_per_channel")):
+            wl.solve_per_channel(ctx, method)   # warm-up
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            Uc = wl.solve_per_channel(ctx, method)
+            ctx.synchronize()
+            rate = wl.n_solves / (time.perf_counter() - t0)
+            ref = U_64 if method == tt.SOLVE_PADE else U_mix
+            err = max(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) for a, b in zip(ref, Uc))
+            out[key] = round(rate, 2)
+            out["mixed_over_" + key] = round(value / rate, 3)
+            if method == tt.SOLVE_PADE:
+                out["plain_pade_per_channel_max_rel_elem_vs_fp64"] = err
+            else:
+                out["mixed_vs_plain_fp64_per_channel_max_rel_elem"] = err
     if "walk" in legs:
         log("parameter walk")
         out["parameter_walk"] = parameter_walk(ctx, wl)

@@ -69,3 +69,61 @@ def oracle_V(prob):
 
 def flags(prob):
     return bool(prob.rp.tensor_force), bool(prob.rp.isospin_breaking_1S0)
+
+
+def oracle_kernel(prob, ch, P, Vu=None, Vc=None):
+    """oracle.solve.KernelA of a library channel on its P123 (host copy), V and SWP basis."""
+    from oracle import solve as so
+    val, row, ptr = P.to_host()
+    tf, ib = flags(prob)
+    if Vu is None:
+        Vu, Vc = oracle_V(prob)
+    return so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
+
+
+def oracle_dense_U(prob, ch, P, G, Vu=None, Vc=None):
+    """The oracle's dense solve (1 - A G_j) X = A of a library channel: U[d_row, d_col, j]."""
+    from oracle import solve as so
+    K = oracle_kernel(prob, ch, P, Vu, Vc)
+    return so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, prob.rp.Nq_WP, prob.rp.Np_WP)
+
+
+def oracle_gmres_row(K, ch, G, dr, j, Nq, Np, rtol=1e-13):
+    """The oracle's GMRES (scipy) solution of one on-shell row system y (1 - G_j A) = e^T A,
+    e = (alpha_deut[dr], q_j, 0), read at the on-shell columns: U[dr, :, j].  The real operator
+    is applied to real and imaginary parts apart (no complex copy of P123)."""
+    import scipy.sparse.linalg as sla
+    e = np.zeros(ch.dim)
+    e[int(ch.deut[dr]) * Nq * Np + int(ch.q_idx[j]) * Np] = 1.0
+    b = K.row_apply(e[None, :])[0].astype(complex)
+    g = G[j]
+
+    def mv(v):
+        w = v * g
+        return v - (K.row_apply(w.real[None, :])[0] + 1j * K.row_apply(w.imag[None, :])[0])
+
+    op = sla.LinearOperator((ch.dim, ch.dim), dtype=complex, matvec=mv)
+    y, info = sla.gmres(op, b, rtol=rtol, restart=80, maxiter=40)
+    assert info == 0, info
+    return np.array([y[int(ch.deut[dc]) * Nq * Np + int(ch.q_idx[j]) * Np] for dc in range(ch.nd)])
+
+
+def oracle_chain_U(prob, ch, P, T_lab):
+    """The oracle's own chain from the library's V_WP of ``prob``: numpy SWP eigenvectors and
+    edges, on-shell bins, closed-form resolvent, dense A = C^T 2P VC on the library's P123 and
+    the dense solve.  Returns U[d_row, d_col, j]."""
+    from oracle import resolvent as rs
+    from oracle import solve as so
+    rp = prob.rp
+    Np = rp.Np_WP
+    tf, ib = flags(prob)
+    fwp = ost.make_fwp(Np, rp.Nq_WP, rp.chebyshev_s)
+    Vu, Vc = oracle_V(prob)
+    swp = osw.make_swp_states(fwp, oracle_pw_full(prob), Vu, Vc, tf, ib)
+    pw = oracle_pw(ch)
+    os_ = osw.find_on_shell_bins(fwp.q_edges, swp.E_bound, list(T_lab))
+    deut = osw.find_deuteron_channels(pw)
+    G = np.stack([rs.resolvent_array(e, pw, swp, tf, ib) for e in os_.E_com + swp.E_bound])
+    val, row, ptr = P.to_host()
+    A = so.KernelA(pw, swp, Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib).dense()
+    return so.dense_solve(A, G, deut, os_.q_idx, rp.Nq_WP, Np)

@@ -1,6 +1,6 @@
 """tictac_run, the reference's run driver over the library: option handling on the CPU, and on
 the GPU the U_PW_elements / U_PW_breakup_elements files against the same solves made through
-the Python binding."""
+the Python binding, and the elastic values against the oracle's dense solve."""
 import os
 import subprocess
 
@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import tictac as tt
-from helpers import PKG
+from helpers import PKG, oracle_dense_U
 
 RUN = os.path.join(PKG, "tictac", "tictac_run")
 
@@ -87,6 +87,10 @@ def test_cli_elastic_and_breakup_files_match_the_binding(ctx, tmp_path):
             # two processes: the autotuner may pick other launch shapes, so sums differ in order
             # (converged solves: 1e-10; a Pade selection could amplify such differences)
             assert np.abs(np.array(u) - U[:, :, j].ravel()).max() <= 1e-10 * np.abs(U).max()
+        # the file's values against the oracle's dense solve of the same channel
+        Uo = oracle_dense_U(prob, ch, P, ch.resolvent(ctx))
+        got = np.array([u for _, _, _, u in rows]).T.reshape(ch.nd, ch.nd, ch.nE)
+        assert np.abs(got - Uo).max() <= 1e-10 * np.abs(Uo).max()
         # breakup file: every finite breakup value of the solve appears in it
         bu = (out / ("U_PW_breakup_elements" + name)).read_text()
         vals = np.array([complex(t) for l in bu.splitlines() if not l.startswith("#")

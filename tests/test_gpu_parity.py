@@ -158,6 +158,21 @@ def test_pade_c1(ctx, c1):
         if m.any():
             assert np.abs(a[..., n][m] - ao[..., n][m]).max() <= 1e-10 * np.abs(ao[..., n]).max()
     assert np.abs(U - c1.Ud).max() <= 1e-5 * np.abs(c1.Ud).max()
+    # the library's resummation is the reference's Pade rules applied to its own terms: U is the
+    # oracle's [k/k] approximant of the same terms (CPP/solve_faddeev.cpp:271-296) for the k the
+    # oracle's selection rules pick (lines 1236-1298), or for a k whose difference to its
+    # predecessor ties with the picked one within 20%.  The rule compares differences of
+    # approximants whose Hankel determinants lose digits at high orders, so near-ties are
+    # decided by rounding (here [7/7] and [9/9] differ from their predecessors by 9.0e-7 and
+    # 9.8e-7); no tolerance on U alone can pin the choice between them.
+    for dr in range(a.shape[0]):
+        for dc in range(a.shape[1]):
+            for j in range(a.shape[2]):
+                _, PA, best = so.pade_select(a[dr, dc, j])
+                d = [abs(PA[k] - (PA[k - 1] if k else 0.0)) for k in range(len(PA))]
+                ok = [k for k in range(len(PA)) if d[k] <= 1.2 * d[best]]
+                err = min(abs(U[dr, dc, j] - PA[k]) for k in ok)
+                assert err <= 1e-9 * abs(PA[best]), (dr, dc, j, best, ok, err)
 
 
 # ---------------------------------------------------------- synthetic potential, N = 30, N_alpha = 27
@@ -458,6 +473,7 @@ def test_parameter_walk_matches_per_set_solves(ctx):
     the U of a separate per-set Problem solved channel by channel (FP64 GMRES: 1e-10; mixed:
     1e-8).  A set without the single 3S1 bound state fails alone, with TT_ERR_PHYSICS on its
     items, and the walk goes on."""
+    from helpers import oracle_chain_U
     rp, pot, T = synthetic_config(N=12, two_J_3N_max=3, T_lab=(10.0, 50.0))
     th = tt.synthetic_params()
     sets = []
@@ -480,6 +496,10 @@ def test_parameter_walk_matches_per_set_solves(ctx):
                 Ur, _ = ref.channel(c, T).solve(ctx, Ps[k], None, method=tt.SOLVE_GMRES, device_resolvent=True)
                 assert U[s][k].shape == Ur.shape
                 assert np.abs(U[s][k] - Ur).max() <= tol * np.abs(Ur).max(), (method, s, c)
+                if s == 1 and k == 0 and method == tt.SOLVE_GMRES:
+                    # and against the oracle's own chain for that set (SWP states, bins, G, dense A)
+                    Uo = oracle_chain_U(ref, ref.channel(c, T), Ps[k], T)
+                    assert np.abs(U[s][k] - Uo).max() <= 1e-9 * np.abs(Uo).max()
 
 
 # ------------------------------------------------------------------ breakup amplitudes
@@ -533,12 +553,8 @@ def test_breakup_amplitudes_in_batch(ctx, c1, c1bu, method):
 
 # ------------------------------------------------------------------ plan identity and failure paths
 def _oracle_dense_U(prob, ch, P, G, Vu=None, Vc=None):
-    val, row, ptr = P.to_host()
-    tf, ib = flags(prob)
-    if Vu is None:
-        Vu, Vc = oracle_V(prob)
-    K = so.KernelA(oracle_pw(ch), oracle_swp(prob), Vu, Vc, so.p_matrix(val, row, ptr, ch.dim), tf, ib)
-    return so.dense_solve(K.dense(), G, ch.deut, ch.q_idx, prob.rp.Nq_WP, prob.rp.Np_WP)
+    from helpers import oracle_dense_U
+    return oracle_dense_U(prob, ch, P, G, Vu, Vc)
 
 
 def test_plan_follows_sign_flips_of_V(ctx):
