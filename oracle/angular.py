@@ -11,6 +11,10 @@ the square root taken last.  Arguments are doubled (two_j), as in GSL.
 
 Pins (tests/test_oracle_pins.py): 3j/6j/9j against sympy.physics.wigner;
 Plm against scipy.special.sph_harm at phi = 0 (Condon-Shortley phase included).
+tests/test_oracle_recoupling.py: Atilde against explicit overlaps of coupled states (LS-jj
+recouplings by Clebsch-Gordan sums in the product basis, and the spin and isospin overlaps of
+the cyclic permutation in the 8-dim space of three spin-1/2 particles), for channels with L, l
+up to 4; the S-wave block against the textbook [[1/4, -3/4], [-3/4, 1/4]].
 """
 import math
 from fractions import Fraction

@@ -18,6 +18,10 @@ Pins (tests/test_oracle_pins.py): the S-wave special case G-tilde = A0/2 (the
 closed form of CPP/auxiliary.cpp:991-1063 at L=l=L'=l'=0); for constant
 G-tilde the element equals a brute-force Monte-Carlo/nested quadrature of the
 bin-overlap volume; the mask is compared with a dense point test.
+tests/test_oracle_recoupling.py: G-tilde for L > 0 against the rotation-invariant contraction
+of two bipolar harmonics (complex spherical harmonics in randomly rotated frames) at the Jacobi
+momenta pi1 = -p/2 - 3q/4, pi2 = p - q/2; the whole P123 against the symmetry P132 = P123^T =
+P123 on antisymmetric states (the asymmetry falls >= 3x per doubling of Nphi = Nx).
 """
 import math
 

@@ -31,7 +31,10 @@ outside the row, gives NaN).  The reference fills them in its Pade path only; th
 also gives the converged value, U = (1 - A G_j)^-1 A read at those columns.
 
 Pins (tests/test_oracle_pins.py): matrix-free row apply against the explicit
-dense A; dense solve residual; Pade of a geometric series a_n = r^n equals
+dense A, and elements of A by explicit sums over the per-alpha-pair blocks and P;
+tests/test_oracle_recoupling.py: the embedded C_op is orthogonal, diagonalises the embedded
+H0 + V with E_bound at the deuteron state (alpha_d, p = 0), carries the deuteron mostly in its
+S-wave rows, and VC_op = V_op C_op (a swapped half or a transposed off-diagonal quarter fails); dense solve residual; Pade of a geometric series a_n = r^n equals
 1/(1-r) exactly for N >= 1 and agrees with scipy.interpolate.pade; breakup values
 of the dense solve against A (1 - G A)^-1 by explicit inversion, and their Pade
 resummation against the dense values.

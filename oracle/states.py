@@ -7,7 +7,9 @@ Restates:
   CPP/make_wp_states.cpp:3-44,77-211        Chebyshev bins, per-bin GL meshes, normalisation
   CPP/make_pw_symm_states.cpp:4-283         partial-wave enumeration and unique 2N index
 
-Pins (tests/test_oracle_pins.py): GL against numpy.polynomial.legendre.leggauss,
+Pins (tests/test_oracle_pins.py): kinematics against the Mandelstam-s closed form and a
+Lorentz boost, the masses and hbar c against PDG values (tests/test_oracle_recoupling.py);
+GL against numpy.polynomial.legendre.leggauss,
 Chebyshev edges against the closed form, kinematics round trips, and the channel
 counts of the enumeration against a brute-force enumeration of angular-momentum
 couplings.

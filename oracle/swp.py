@@ -12,7 +12,9 @@ The reference diagonalises with LAPACKE_dspevd; here numpy.linalg.eigh.  The
 eigenvector sign is arbitrary in both, and every on-shell U element is
 invariant under it (C enters as C ... C^T with the diagonal G in between).
 
-Pins (tests/test_oracle_pins.py): C^T C = 1 and C^T H C diagonal; MT-III deuteron
+Pins (tests/test_oracle_pins.py): C^T C = 1 and C^T H C diagonal; the nn/np mixing
+weights against isospin Clebsch-Gordan coefficients, and the sign of the T3N-mixing block shown
+to be a convention that cancels in U (tests/test_oracle_recoupling.py); MT-III deuteron
 near -2.23 MeV at large N (CPP/Interactions/malfliet_tjon.cpp:11); V_WP of the
 MT potential against scipy.integrate.dblquad over one bin pair.  Breakup cells: every
 listed primary cell brackets the energy and every bracketing cell is listed, in loop

@@ -340,7 +340,22 @@ def test_row_apply_dense_and_dense_solve_residual():
     P = so.p_matrix(*pm.to_csc(ent, dim), dim)
     K = so.KernelA(pw, swp, Vu, Vc, P, False, False)
     A = K.dense()
-    # explicit A = C^T (2P) VC assembled element by element for one entry
+    # elements of A = C^T (2P) VC from the per-alpha-pair blocks and P itself, by explicit sums
+    # (the sparse assembly of the q-diagonal block operators is not used)
+    Pd = P.toarray()
+    Na = pw.Nalpha
+    for (ar, qr, pr), (ac, qc, pc) in [((0, 1, 0), (1, 2, 3)), ((1, 4, 2), (0, 0, 5)), ((1, 3, 1), (1, 3, 1))]:
+        ref = 0.0
+        for ai in range(Na):
+            for pi in range(N):
+                cpart = K.Cb[ai, pi, ar, pr]
+                if cpart == 0.0:
+                    continue
+                for aj in range(Na):
+                    for pj in range(N):
+                        ref += cpart * 2.0 * Pd[ai * N * N + qr * N + pi, aj * N * N + qc * N + pj] \
+                            * K.VCb[aj, pj, ac, pc]
+        assert abs(A[ar * N * N + qr * N + pr, ac * N * N + qc * N + pc] - ref) <= 1e-13 * np.abs(A).max()
     X = rng.normal(size=(2, dim)) + 1j * rng.normal(size=(2, dim))
     assert np.allclose(K.row_apply(X), X @ A, atol=1e-13 * np.abs(X @ A).max())
     assert np.allclose(K.col_apply(X.T), A @ X.T, atol=1e-13 * np.abs(A @ X.T).max())
