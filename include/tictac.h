@@ -392,6 +392,23 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
                         const tt_pw_states *chn, const double *G, int nE, const int *g_index,
                         int r, const double *X, double *Y, int flags);
 
+/* No counterpart in the reference.  tt_apply_rows for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the FP64 row form Y = (X .* G_sel) A (flags 0: cell blocks, or the CSC gather above 64 partial
+ * waves; flags 2: the CSC gather) with every device buffer beyond the P123 object (operator
+ * tables, work items, staged X, G and Y, the product's partial panels) in a static scratch array
+ * of 128 MiB that each GPU the library runs on holds until the process ends; it calls no
+ * hipMalloc or hipFree.  Requires a P123 that already has its cell-block form (tt_fill_P123_arrays
+ * builds it) unless flags = 2.  Sizes needing more than 128 MiB give TT_ERR_ARG (roughly
+ * 48 dim r + 8 items Nalpha 2r bytes); FP32 products or the column operator give
+ * TT_ERR_UNSUPPORTED.  All calls on one GPU share that GPU's array: calls that overlap on one
+ * GPU (several host threads or contexts) race and must be serialised by the caller.  The
+ * context's stream is synchronised before the call returns. */
+tt_status tt_apply_rows_noalloc(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123,
+                                const double *V_unco, const double *V_coup, const tt_swp_states *swp,
+                                const tt_pw_states *chn, const double *G, int nE, const int *g_index,
+                                int r, const double *X, double *Y, int flags);
+
 #ifdef __cplusplus
 }
 #endif

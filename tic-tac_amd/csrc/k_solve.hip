@@ -691,22 +691,38 @@ static CellView cells_of(const DeviceCSC &P, bool transposed) {
     return v;
 }
 
+// The device arrays of a set of work items (CellItems owns them; the allocation-free path carves
+// them from its scratch).
+struct ItemsRef {
+    int n = 0;
+    const int2 *items = nullptr;
+    const int *ket_item_ptr = nullptr;
+};
+
 // Work items of the cell-block product: runs of at most item_len (<= 64) cells of one group.
 struct CellItems {
     int n = 0;
     DevBuf<int2> items;
     DevBuf<int> ket_item_ptr;
-    void build(const CellView &P, hipStream_t s, int item_len = CELL_ITEM) {
+    ItemsRef ref() const { return ItemsRef{n, items.p, ket_item_ptr.p}; }
+    // runs of at most item_len cells of each group (host side): it[] and the per-group item
+    // pointers kip[]
+    static void runs(const CellView &P, hipStream_t s, int item_len, std::vector<int2> &it, std::vector<int> &kip) {
         TT_REQUIRE(item_len >= 1 && item_len <= 64, "work item length must be in [1, 64]");
         std::vector<int> kp(P.NqNp + 1);
         TT_HIP(hipMemcpyAsync(kp.data(), P.grp_ptr, sizeof(int) * (P.NqNp + 1), hipMemcpyDeviceToHost, s));
         TT_HIP(hipStreamSynchronize(s));
-        std::vector<int2> it;
-        std::vector<int> kip(P.NqNp + 1, 0);
+        it.clear();
+        kip.assign(P.NqNp + 1, 0);
         for (int k = 0; k < P.NqNp; ++k) {
             for (int c = kp[k]; c < kp[k + 1]; c += item_len) it.push_back(make_int2(c, std::min(c + item_len, kp[k + 1])));
             kip[k + 1] = (int)it.size();
         }
+    }
+    void build(const CellView &P, hipStream_t s, int item_len = CELL_ITEM) {
+        std::vector<int2> it;
+        std::vector<int> kip;
+        runs(P, s, item_len, it, kip);
         n = (int)it.size();
         items.upload(it.data(), std::max<size_t>(it.size(), 1), s);
         ket_item_ptr.upload(kip.data(), kip.size(), s);
@@ -745,16 +761,16 @@ struct CellItems {
 };
 
 template <typename T, int MT, int NT, int UN>
-static void launch_cellmfma_un(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
+static void launch_cellmfma_un(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
                                T *Upart, int R2) {
     const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
-    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ci.items.p, P->partner, blk, S, Upart, P->Na,
+    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ci.items, P->partner, blk, S, Upart, P->Na,
                        R2);
 }
 
 // un: batch depth (one of cell_un_candidates<T>(); 0 = the default)
 template <typename T, int MT, int NT>
-static void launch_cellmfma(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
+static void launch_cellmfma(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
                             T *Upart, int R2, int un) {
     if (un == 0) un = cell_un<T>();
     if constexpr (sizeof(T) == 8) {
@@ -785,7 +801,7 @@ static int cell_nt(int R2) {
 }
 
 template <typename T, int MT>
-static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S,
+static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
                                T *Upart, int R2, int un) {
     switch (cell_nt(R2)) {
         case 1:
@@ -806,7 +822,7 @@ static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const CellItems
 // Upart = the work items' partial panels (items * Nalpha * R2); when U is given also
 // U = scale * (sum of each ket's item panels), the full product in cell-major order.
 template <typename T>
-static void launch_cellmm(hipStream_t s, const CellView *P, const CellItems &ci, const T *blk, const T *S, T *U,
+static void launch_cellmm(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S, T *U,
                           T *Upart, int R2, T scale, int un = 0) {
     TT_REQUIRE(P->Na >= 1 && P->Na <= CELL_NA_MAX, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
     if (ci.n > 0) {
@@ -822,7 +838,7 @@ static void launch_cellmm(hipStream_t s, const CellView *P, const CellItems &ci,
     if (U) {
         const int64_t nout = (int64_t)P->NqNp * P->Na * R2;
         hipLaunchKernelGGL((k_cell_reduce<T>), dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s,
-                           ci.ket_item_ptr.p, Upart, U, P->NqNp, P->Na, R2, scale);
+                           ci.ket_item_ptr, Upart, U, P->NqNp, P->Na, R2, scale);
     }
 }
 
@@ -1527,14 +1543,14 @@ struct RowOperator {
                     // trades the product's parallelism against that sum)
                     auto launch = [&]() {
                         if (fp32) {
-                            launch_cellmm<float>(s, &cv, ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                            launch_cellmm<float>(s, &cv, ci.ref(), cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                                  reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
                             launch_block<float2, float2, float>(s, reinterpret_cast<const float2 *>(upart.p), U32.p,
                                                                 vc_ptr.p, vc_alpha.p, vc_blocks32.p, vc_npart_max,
                                                                 nullptr, gidx.p, Na, Nq, Np, r, nullptr, 1, 0,
                                                                 ci.ket_item_ptr.p, 2.0);
                         } else {
-                            launch_cellmm<double>(s, &cv, ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
+                            launch_cellmm<double>(s, &cv, ci.ref(), cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
                                                   upart.p, R2, 2.0, un);
                             launch_block<double2, double2>(s, reinterpret_cast<const double2 *>(upart.p), U.p, vc_ptr.p,
                                                            vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p, Na,
@@ -1657,7 +1673,7 @@ struct RowOperator {
                                                1.0, zb, zn);
                 if (timed) record(e1);
                 if (cells)
-                    launch_cellmm<double>(s, &cv, *ci, cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
+                    launch_cellmm<double>(s, &cv, ci->ref(), cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
                                           upart.p, R2, 2.0, un);
                 else
                     launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
@@ -1682,7 +1698,7 @@ struct RowOperator {
                                                 1.0, zb, zn);
             if (timed) record(e1);
             if (cells)
-                launch_cellmm<float>(s, &cv, *ci, cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
+                launch_cellmm<float>(s, &cv, ci->ref(), cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                      reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
             else
                 launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
@@ -1714,7 +1730,7 @@ struct RowOperator {
         prepare(r, false);
         launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max, nullptr, gidx.p, Na,
                                        Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);
-        launch_cellmm<double>(s, &cv, items_for(false, 2 * r), cv.blk, reinterpret_cast<const double *>(S.p),
+        launch_cellmm<double>(s, &cv, items_for(false, 2 * r).ref(), cv.blk, reinterpret_cast<const double *>(S.p),
                               reinterpret_cast<double *>(Ucell), upart.p, 2 * r, 2.0, choice(false, 2 * r).un);
     }
 
@@ -2785,6 +2801,52 @@ static DeviceCSC *dev_of(const tt_csc *m) {
 
 using namespace tt;
 
+namespace tt {
+// ------------------------------------------------------------------ static scratch
+// Device memory of tt_apply_rows_noalloc: every buffer its kernels use beyond the caller's
+// inputs and outputs (operator tables, work items, the staged rows, G and the product's partial
+// panels) is carved from this array, so that call allocates nothing.  Each GPU the library runs
+// on holds its own copy, at its own address, until the process ends.  In host code the array's
+// name is not a device address: kernels and copies take scratch_ptr(device) instead.
+constexpr size_t SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+static std::mutex scratch_mu;
+static std::vector<void *> scratch_addr;   // per device, from hipGetSymbolAddress
+
+// Called once per device with that device current (tt_ctx_create).
+static void scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if ((int)scratch_addr.size() <= device) scratch_addr.resize(device + 1, nullptr);
+    if (scratch_addr[device]) return;
+    void *ptr = nullptr;
+    TT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    TT_REQUIRE(ptr, "no device address for the static scratch array");
+    scratch_addr[device] = ptr;
+}
+
+static unsigned char *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    TT_REQUIRE(device >= 0 && device < (int)scratch_addr.size() && scratch_addr[device],
+               "static scratch not set up on this device (tt_ctx_create does it)");
+    return static_cast<unsigned char *>(scratch_addr[device]);
+}
+
+// Consecutive 256-byte aligned pieces of one region, sized first (base == nullptr) and then
+// placed.
+struct Carve {
+    unsigned char *base = nullptr;
+    size_t used = 0;
+    template <typename T>
+    T *take(size_t count) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        used = at + sizeof(T) * std::max<size_t>(count, 1);
+        return base ? reinterpret_cast<T *>(base + at) : nullptr;
+    }
+};
+
+}  // namespace tt
+
 extern "C" {
 
 tt_status tt_ctx_create(int device, tt_ctx **out) {
@@ -2794,6 +2856,7 @@ tt_status tt_ctx_create(int device, tt_ctx **out) {
     TT_HIP(hipGetDeviceCount(&n));
     TT_REQUIRE(device >= 0 && device < n, "no such HIP device");
     TT_HIP(hipSetDevice(device));
+    scratch_setup(device);   // the static scratch's address on this device (once per device)
     auto *c = new tt_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -3513,6 +3576,132 @@ tt_status tt_apply_rows(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123
             Y[2 * ((size_t)k * dim + i)] = yt[(size_t)i * r + k].x;
             Y[2 * ((size_t)k * dim + i) + 1] = yt[(size_t)i * r + k].y;
         }
+    TT_GUARD_END
+}
+
+// tt_apply_rows (FP64 row form) with every device buffer beyond the P123 object carved from
+// the static scratch array: the same block-op, cell-product (or CSC gather) and block-op kernels,
+// the work items at the default length, no allocation, no autotuning, no timing events.
+tt_status tt_apply_rows_noalloc(tt_ctx *ctx, const tt_run_params *rp, const tt_csc *P123, const double *V_unco,
+                                const double *V_coup, const tt_swp_states *swp, const tt_pw_states *chn,
+                                const double *G, int nE, const int *g_index, int r, const double *X, double *Y,
+                                int flags) {
+    TT_GUARD_BEGIN
+    TT_REQUIRE(ctx && rp && P123 && V_unco && swp && chn && X && Y && r > 0, "null argument");
+    if (flags & ~2)
+        fail(TT_ERR_UNSUPPORTED, "tt_apply_rows_noalloc takes the FP64 row form only (flags 0 or 2)");
+    TT_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const DeviceCSC *P = dev_of(P123);
+    const int Na = chn->Nalpha, Nq = swp->Nq_WP, Np = swp->Np_WP;
+    const int64_t dim = (int64_t)Na * Nq * Np;
+    TT_REQUIRE(P->dim == dim, "P123 dimension does not match Nalpha*Nq*Np");
+    const bool cells = (flags & 2) == 0 && Na <= CELL_NA_MAX;
+    if (cells && (P->Na != Na || P->NqNp != Nq * Np))
+        fail(TT_ERR_UNSUPPORTED, "this P123 has no cell-block form yet (tt_fill_P123_arrays builds it; a "
+                                 "solve builds it for an uploaded CSC): build it first or pass flags = 2 "
+                                 "(gather over the CSC)");
+    std::vector<int> g(r, -1);
+    bool useG = false;
+    if (g_index)
+        for (int k = 0; k < r; ++k) {
+            TT_REQUIRE(g_index[k] < nE, "g_index out of range");
+            g[k] = g_index[k];
+            if (g[k] >= 0) useG = true;
+        }
+    TT_REQUIRE(!useG || (G && nE > 0), "G required when g_index selects energies");
+    // operator tables (host): C_op and VC_op partner blocks, as RowOperator::set_blocks
+    VCProducts vc;
+    vc_products(*rp, V_unco, V_coup, *swp, vc);
+    std::vector<int> p1, a1, p2, a2;
+    std::vector<double> b1, b2;
+    embed_2N_blocks(*chn, Np, swp->C_SWP_unco, swp->C_SWP_coup, *rp, false, p1, a1, b1, true);
+    embed_2N_blocks(*chn, Np, vc.u.data(), vc.c.data(), *rp, true, p2, a2, b2, true);
+    int m1 = 1, m2 = 1;
+    for (int a = 0; a < Na; ++a) {
+        m1 = std::max(m1, p1[a + 1] - p1[a]);
+        m2 = std::max(m2, p2[a + 1] - p2[a]);
+    }
+    a1.push_back(0);
+    a2.push_back(0);
+    b1.push_back(0.0);
+    b2.push_back(0.0);
+    CellView cv;
+    std::vector<int2> it;
+    std::vector<int> kip;
+    if (cells) {
+        cv = cells_of(*P, false);
+        CellItems::runs(cv, s, CELL_ITEM, it, kip);
+    }
+    const int R2 = 2 * r;
+    const size_t n_items = std::max<size_t>(it.size(), 1);
+    // the pieces of the scratch: sized first, then placed
+    unsigned char *base = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        Carve cut;
+        cut.base = base;
+        int *d_p1 = cut.take<int>(p1.size()), *d_a1 = cut.take<int>(a1.size());
+        double *d_b1 = cut.take<double>(b1.size());
+        int *d_p2 = cut.take<int>(p2.size()), *d_a2 = cut.take<int>(a2.size());
+        double *d_b2 = cut.take<double>(b2.size());
+        int *d_g = cut.take<int>(r);
+        int2 *d_it = cut.take<int2>(n_items);
+        int *d_kip = cut.take<int>(kip.size());
+        double2 *d_G = cut.take<double2>(useG ? (size_t)nE * dim : 1);
+        double2 *d_X = cut.take<double2>((size_t)dim * r), *d_S = cut.take<double2>((size_t)dim * r);
+        double2 *d_Y = cut.take<double2>((size_t)dim * r);
+        double *d_U = cut.take<double>(cells ? n_items * Na * R2 : (size_t)dim * R2);
+        if (pass == 0) {
+            if (cut.used > SCRATCH_BYTES)
+                fail(TT_ERR_ARG, "tt_apply_rows_noalloc needs " + std::to_string(cut.used) +
+                                     " bytes of scratch for these sizes, above its maximum of " +
+                                     std::to_string(SCRATCH_BYTES) + " (use tt_apply_rows)");
+            base = scratch_ptr(ctx->device);
+            continue;
+        }
+        std::vector<double2> xt((size_t)dim * r), yt((size_t)dim * r);
+        for (int k = 0; k < r; ++k)
+            for (int64_t i = 0; i < dim; ++i)
+                xt[(size_t)i * r + k] = make_double2(X[2 * ((size_t)k * dim + i)], X[2 * ((size_t)k * dim + i) + 1]);
+        auto h2d = [&](void *d, const void *h, size_t bytes) {
+            if (bytes) TT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+        };
+        h2d(d_p1, p1.data(), sizeof(int) * p1.size());
+        h2d(d_a1, a1.data(), sizeof(int) * a1.size());
+        h2d(d_b1, b1.data(), sizeof(double) * b1.size());
+        h2d(d_p2, p2.data(), sizeof(int) * p2.size());
+        h2d(d_a2, a2.data(), sizeof(int) * a2.size());
+        h2d(d_b2, b2.data(), sizeof(double) * b2.size());
+        h2d(d_g, g.data(), sizeof(int) * r);
+        if (cells) {
+            h2d(d_it, it.data(), sizeof(int2) * it.size());
+            h2d(d_kip, kip.data(), sizeof(int) * kip.size());
+        }
+        if (useG) h2d(d_G, G, sizeof(double2) * (size_t)nE * dim);
+        h2d(d_X, xt.data(), sizeof(double2) * xt.size());
+        // S = C_op (X .* G), then U = 2 P123^T S, then Y = VC_op^T U (RowOperator::apply, FP64)
+        const int cm = cells ? 1 : 0;
+        launch_block<double2, double2>(s, d_X, d_S, d_p1, d_a1, d_b1, m1, useG ? d_G : nullptr, d_g, Na, Nq, Np, r,
+                                       nullptr, 0, cm, nullptr, 1.0);
+        if (cells) {
+            const ItemsRef ci{(int)it.size(), d_it, d_kip};
+            launch_cellmm<double>(s, &cv, ci, cv.blk, reinterpret_cast<const double *>(d_S), nullptr, d_U, R2, 2.0);
+            launch_block<double2, double2>(s, reinterpret_cast<const double2 *>(d_U), d_Y, d_p2, d_a2, d_b2, m2,
+                                           nullptr, d_g, Na, Nq, Np, r, nullptr, cm, 0, d_kip, 2.0);
+        } else {
+            launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(d_S), d_U, dim, R2, 2.0);
+            launch_block<double2, double2>(s, reinterpret_cast<const double2 *>(d_U), d_Y, d_p2, d_a2, d_b2, m2,
+                                           nullptr, d_g, Na, Nq, Np, r, nullptr, 0, 0, nullptr, 1.0);
+        }
+        TT_HIP(hipGetLastError());
+        TT_HIP(hipMemcpyAsync(yt.data(), d_Y, sizeof(double2) * yt.size(), hipMemcpyDeviceToHost, s));
+        TT_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < r; ++k)
+            for (int64_t i = 0; i < dim; ++i) {
+                Y[2 * ((size_t)k * dim + i)] = yt[(size_t)i * r + k].x;
+                Y[2 * ((size_t)k * dim + i) + 1] = yt[(size_t)i * r + k].y;
+            }
+    }
     TT_GUARD_END
 }
 

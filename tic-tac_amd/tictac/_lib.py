@@ -134,6 +134,9 @@ PROTOTYPES = {
                                     C.c_int, _ip, C.POINTER(SolveOpts), C.POINTER(GradStats)]),
     "tt_apply_rows": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(CSC), _dp, _dp, C.POINTER(SwpStates),
                                 C.POINTER(PwStates), _dp, C.c_int, _ip, C.c_int, _dp, _dp, C.c_int]),
+    "tt_apply_rows_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(CSC), _dp, _dp,
+                                        C.POINTER(SwpStates), C.POINTER(PwStates), _dp, C.c_int, _ip, C.c_int, _dp,
+                                        _dp, C.c_int]),
     "tt_cell_tuning": (C.c_int, [C.c_char_p, C.c_int]),
     "tt_parameter_walk": (C.c_int, [C.c_void_p, C.POINTER(RunParams), C.POINTER(FwpStates), C.POINTER(PwStates),
                                     C.c_void_p, _dp, C.c_int, C.c_int, _ip, C.POINTER(C.POINTER(CSC)), C.c_int, _dp,

@@ -400,8 +400,9 @@ class Channel:
             return U, st, a.view(np.complex128).reshape(self.nd, self.nd, self.nE, nterms)
         return U, st
 
-    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False, csc=False, column=False):
-        """Y = (X .* G_sel) A for rows X (r, dim) complex; with column=True Y_k = A (G_k .* X_k)."""
+    def apply_rows(self, ctx, P, X, G=None, g_index=None, fp32=False, csc=False, column=False, noalloc=False):
+        """Y = (X .* G_sel) A for rows X (r, dim) complex; with column=True Y_k = A (G_k .* X_k).
+        noalloc=True calls tt_apply_rows_noalloc (FP64 row form, no device allocation)."""
         X = np.ascontiguousarray(X, np.complex128)
         r = X.shape[0]
         Y = np.zeros_like(X)
@@ -410,11 +411,11 @@ class Channel:
         gi = None
         if g_index is not None:
             gi = np.ascontiguousarray(g_index, np.int32)
-        check(L.lib().tt_apply_rows(ctx.handle, C.byref(self.prob.rp), C.byref(P.csc), dptr(self.prob.V_unco),
-                                    dptr(self.prob.V_coup), C.byref(self.prob.swp), C.byref(self.view),
-                                    dptr(Gf), nE, iptr(gi) if gi is not None else None, r,
-                                    dptr(X.view(np.float64)), dptr(Y.view(np.float64)),
-                                    int(fp32) | (2 if csc else 0) | (4 if column else 0)))
+        fn = L.lib().tt_apply_rows_noalloc if noalloc else L.lib().tt_apply_rows
+        check(fn(ctx.handle, C.byref(self.prob.rp), C.byref(P.csc), dptr(self.prob.V_unco), dptr(self.prob.V_coup),
+                 C.byref(self.prob.swp), C.byref(self.view), dptr(Gf), nE, iptr(gi) if gi is not None else None, r,
+                 dptr(X.view(np.float64)), dptr(Y.view(np.float64)),
+                 int(fp32) | (2 if csc else 0) | (4 if column else 0)))
         return Y
 
     def __del__(self):
