@@ -307,10 +307,9 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=30.0):
     inputs as the GPU (its P123, V and SWP basis):
 
     * value: matrix-free FP64 GMRES (CSC gather, all rows of a channel sharing each application)
-      to rtol 1e-12, one call per channel with all 8 T_lab, on a sample of the workload: its
-      smallest channel (1/2+, N_alpha 27) and its largest (7/2+, N_alpha 63), 16 channel-solves;
-      with the difference of their U to the GPU's plain FP64 solves and an estimate for the whole
-      batch (each channel's time scaled by its nonzeros from the measured channel of its class).
+      to rtol 1e-12, one call per channel with all 8 T_lab, over the whole workload (18 channels,
+      144 channel-solves; about 15 s on 16 host threads), with the largest difference of its U to
+      the GPU's plain FP64 solves.
     * reference_semantics: the reference's own algorithm on the 1/2+ channel at the 8 T_lab: every
       Neumann term regenerates the columns of A (CPP/solve_faddeev.cpp:1019-1091), timed for up
       to budget_s / 2 and extrapolated to the Neumann terms the same Pade solve takes on the GPU.
@@ -324,39 +323,33 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=30.0):
     Vu, Vc = oracle_V(prob)
     swp = oracle_swp(prob)
     threads = cpu.lib().ttcpu_threads()
-    sample = [0, next(i for i, ch in enumerate(wl.chans) if ch.Nalpha == 63)]
-    secs, diff, done = {}, 0.0, 0
-    ops = {}
-    for i in sample:
-        ch, P = wl.chans[i], wl.Ps[i]
+    secs, diff = {}, 0.0
+    op0 = None
+    for i, (ch, P) in enumerate(zip(wl.chans, wl.Ps)):
         val, row, ptr = P.to_host()
-        ops[i] = cpu.Problem(oracle_pw(ch), swp, Vu, Vc, val, row, ptr, tf, ib)
+        op = cpu.Problem(oracle_pw(ch), swp, Vu, Vc, val, row, ptr, tf, ib)
+        del val, row, ptr
         G = ch.resolvent(ctx)
         t0 = time.perf_counter()
-        U, rc, its, res = ops[i].gmres_U(G, ch.deut, ch.q_idx, tol=SOLVE_TOL)
-        secs[i] = time.perf_counter() - t0
-        done += len(BATCH_T)
+        U, rc, its, res = op.gmres_U(G, ch.deut, ch.q_idx, tol=SOLVE_TOL)
+        secs[wl.channel_table()[i]["JP"]] = round(time.perf_counter() - t0, 3)
         if U_plain is not None:
             diff = max(diff, float(np.max(np.abs(U - U_plain[i])) / np.max(np.abs(U_plain[i]))))
-    value = done / sum(secs.values())
-    # whole batch: every channel at the seconds per nonzero of the measured channel of its size class
-    per_nnz = {i: secs[i] / wl.Ps[i].nnz for i in sample}
-    est = sum(wl.Ps[c].nnz * (per_nnz[sample[0]] if ch.Nalpha < 45 else per_nnz[sample[1]])
-              for c, ch in enumerate(wl.chans))
-    out = {"value": round(value, 3), "unit": "channel-solves/s", "cores": threads, "kind": "port",
-           "sample": "oracle/cpu_solve.cpp (C++/OpenMP, FP64) matrix-free GMRES to rtol 1e-12 on the workload's "
-                     "1/2+ (N_alpha 27) and 7/2+ (N_alpha 63) channels, one call per channel with the 8 T_lab: "
-                     "%d channel-solves in %.1f s on %d threads" % (done, sum(secs.values()), threads),
-           "seconds_per_channel": {wl.channel_table()[i]["JP"]: round(secs[i], 2) for i in sample},
+        if i == 0:
+            op0 = op
+    total = sum(secs.values())
+    out = {"value": round(wl.n_solves / total, 3), "unit": "channel-solves/s", "cores": threads, "kind": "port",
+           "sample": "the whole workload: oracle/cpu_solve.cpp (C++/OpenMP, FP64) matrix-free GMRES to rtol 1e-12, "
+                     "one call per channel with the 8 T_lab, 18 channels = %d channel-solves in %.1f s on %d threads"
+                     % (wl.n_solves, total, threads),
+           "seconds_per_channel": secs,
            "max_rel_vs_gpu_plain_fp64": diff,
-           "whole_batch_estimate_s": round(est, 1),
-           "whole_batch_estimate_channel_solves_per_s": round(wl.n_solves / est, 3),
            "host": host_info()}
     # the reference's own algorithm (columns of A regenerated per Neumann term), capped
     ch = wl.chans[0]
     G = ch.resolvent(ctx)
     n_need = ch.solve(ctx, wl.Ps[0], G, method=tt.SOLVE_PADE)[1].iterations
-    a, n_done, spt = ops[0].neumann_terms(G, ch.deut, ch.q_idx, n_need, cap_s=budget_s / 2)
+    a, n_done, spt = op0.neumann_terms(G, ch.deut, ch.q_idx, n_need, cap_s=budget_s / 2)
     out["reference_semantics"] = {
         "channel": "1/2+ (N_alpha 27, dim 24300), 8 T_lab -> 12 on-shell rows",
         "seconds_per_neumann_term": round(spt, 3), "terms_timed": n_done, "terms_needed": n_need,
@@ -370,7 +363,7 @@ def cpu_baseline(wl, ctx, U_plain, budget_s=30.0):
 
 def host_info():
     """The host the CPU baseline ran on: CPU model, the CPUs this process may use, and the
-    OpenMP / BLAS thread setting (scipy's sparse products use one thread)."""
+    OpenMP thread setting the restatement runs with."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
