@@ -235,6 +235,28 @@ def single_channel(ctx, steps, warmup):
                                               measured_traffic("k_cellmfma<double, 2, 1, 2>"))}
 
 
+def dense_leg(ctx):
+    """The dense path (solve_dense=true) on the single-channel workload: per on-shell energy, L = 1 -
+    A G_j (dim 24,300, complex FP64) formed and LU-factorised with partial pivoting on the device;
+    seconds per energy and the LU's rate at (8/3) dim^3 real flops per factorisation against the
+    FP64 MFMA peak, with the difference to the GMRES solve."""
+    prob = tt.Problem(wp_params(30, 1), tt.Potential("synthetic", tt.synthetic_params(7)))
+    ch = prob.channel(0, [10.0])
+    P = ch.build_p123(ctx)
+    G = ch.resolvent(ctx)
+    U64, _ = ch.solve(ctx, P, G, method=tt.SOLVE_GMRES, tol=SOLVE_TOL)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    Ud, _ = ch.solve(ctx, P, G, method=tt.SOLVE_DENSE)
+    dt = (time.perf_counter() - t0) / ch.nE
+    flops = 8.0 / 3.0 * float(ch.dim) ** 3
+    return {"workload": SINGLE_WORKLOAD + "; dense LU of 1 - A G_j per on-shell bin", "dim": ch.dim,
+            "seconds_per_energy": round(dt, 3), "lu_tflops": round(flops / dt / 1e12, 2),
+            "fp64_mfma_peak_tflops": MFMA_PEAK_TFLOPS["fp64"],
+            "frac_of_peak_incl_forming_L": round(flops / dt / 1e12 / MFMA_PEAK_TFLOPS["fp64"], 4),
+            "max_rel_vs_gmres": float(np.max(np.abs(Ud - U64)) / np.max(np.abs(U64)))}
+
+
 GRAD_WORKLOAD = ("same synthetic NN, channels 2J=1,3 both parities (N_alpha 27, 47; dim 24300, 42300), Np=Nq=30, "
                  "T_lab 5, 50, 150 MeV, 12 strengths theta (3 Yukawas, 6 contacts, 3 separables); FP64")
 
@@ -426,7 +448,7 @@ def copy_calibration(mib=256, reps=3):
     return {"bytes_each_way": n, "launches": reps}
 
 
-LEGS = ("plain", "walk", "single", "n40", "grad", "cpu", "copy")
+LEGS = ("plain", "walk", "single", "n40", "grad", "dense", "cpu", "copy")
 
 
 def main():
@@ -540,6 +562,9 @@ def main():
         log("gradient")
         out["gradient"] = gradient_ratio(ctx)
         out["grad_over_fwd"] = out["gradient"]["grad_over_fwd"]
+    if "dense" in legs:
+        log("dense LU")
+        out["dense"] = dense_leg(ctx)
     if "copy" in legs:
         log("copy calibration")
         out["copy_calibration"] = copy_calibration()

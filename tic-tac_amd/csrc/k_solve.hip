@@ -2695,98 +2695,94 @@ static void solve_on_lanes(tt_ctx *ctx, std::vector<KMember> &ms, const tt_solve
     solve_lockstep(ms, o);
 }
 
-// Dense path: explicit A on the device (dim unit rows through the operator), LU on the host.
-// With U_BU, the breakup states' columns of A join the right-hand sides (U = (1 - A G)^-1 A
-// column by column); the reference's dense path leaves U_BU unset.
+// M[c][r0 + k] (column-major, leading dim dim) = delta - A[r0 + k][c] G[c] from the rows
+// r0 .. r0 + rc - 1 of A, Y[c][k] (RHS-fastest: Y = E A for unit rows E)
+__global__ void k_dense_rows(const double2 *__restrict__ Y, int rc, int64_t r0, int64_t dim,
+                             const double2 *__restrict__ G, double2 *__restrict__ M) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dim * rc) return;
+    const int64_t c = t / rc, k = t - c * rc, r = r0 + k;
+    const double2 y = Y[t], g = G[c];
+    M[c * dim + r] = make_double2((r == c ? 1.0 : 0.0) - (y.x * g.x - y.y * g.y), -(y.x * g.y + y.y * g.x));
+}
+
+// B[q][r0 + k] = A[r0 + k][cols[q]] (the right-hand-side columns of A) from the same rows
+__global__ void k_dense_rhs(const double2 *__restrict__ Y, int rc, int64_t r0, int64_t dim,
+                            const int64_t *__restrict__ cols, int nrhs, double2 *__restrict__ B) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrhs * rc) return;
+    const int q = (int)(t / rc), k = (int)(t - (int64_t)q * rc);
+    B[(int64_t)q * dim + r0 + k] = Y[cols[q] * rc + k];
+}
+
+// Dense path (CPP/solve_faddeev.cpp:305-438): for every on-shell energy j, L = 1 - A diag(G_j)
+// is formed on the device from the rows of A (dim unit rows through the operator, in chunks),
+// with the on-shell columns of A (and, with U_BU, the breakup states' columns) as right-hand
+// sides, and solved by the device LU (dense_lu_solve).  The reference solves for all dim columns
+// of A; only the columns whose elements are read are solved here.
 static void solve_dense(RowOperator &op, const tt_channel_os &os, cdouble *U, tt_solve_stats &st,
                         cdouble *U_BU = nullptr) {
     const int64_t dim = op.dim;
-    TT_REQUIRE(dim <= 4096, "dense solve is a debug path for dim <= 4096");
-    std::vector<cdouble> A((size_t)dim * dim);
-    const int chunk = 64;
-    DevBuf<double2> E((size_t)dim * chunk), Y((size_t)dim * chunk);
-    DevBuf<int64_t> di;
-    std::vector<double2> h((size_t)dim * chunk);
-    op.set_gidx(std::vector<int>(chunk, -1));
-    for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
-        int rc = (int)std::min<int64_t>(chunk, dim - r0);
-        std::vector<int64_t> idx(rc);
-        for (int k = 0; k < rc; ++k) idx[k] = r0 + k;
-        TT_HIP(hipMemsetAsync(E.p, 0, sizeof(double2) * dim * rc, op.s));
-        di.upload(idx.data(), rc, op.s);
-        hipLaunchKernelGGL(k_unit_rows, dim3(nblk(rc, 64)), dim3(64), 0, op.s, E.p, di.p, rc);
-        op.apply(E.p, Y.p, rc, false, false);
-        TT_HIP(hipMemcpyAsync(h.data(), Y.p, sizeof(double2) * dim * rc, hipMemcpyDeviceToHost, op.s));
-        TT_HIP(hipStreamSynchronize(op.s));
-        for (int k = 0; k < rc; ++k)
-            for (int64_t c = 0; c < dim; ++c) A[(size_t)(r0 + k) * dim + c] = cdouble(h[c * rc + k].x, h[c * rc + k].y);
-    }
-    std::vector<double2> Gh((size_t)op.nE * dim);
-    TT_HIP(hipMemcpyAsync(Gh.data(), op.G.p, sizeof(double2) * Gh.size(), hipMemcpyDeviceToHost, op.s));
-    TT_HIP(hipStreamSynchronize(op.s));
     const int nd = os.num_deuteron_states, nE = os.num_T_lab;
     const bool bu = U_BU && os.num_BU_chns > 0;
+    {
+        size_t fr = 0, tot = 0;
+        TT_HIP(hipMemGetInfo(&fr, &tot));
+        const double need = 16.0 * (double)dim * dim + 16.0 * (double)dim * (2 * 64 + 130 + 3 * 64);
+        if (need > 0.95 * (double)fr)
+            fail(TT_ERR_ALLOC, "dense solve needs " + std::to_string((long long)(need / (1 << 20))) +
+                                   " MiB of device memory for dim " + std::to_string((long long)dim) + ", " +
+                                   std::to_string((long long)(fr >> 20)) + " MiB free");
+    }
+    const int chunk = 64;
+    DevBuf<double2> E((size_t)dim * chunk), Y((size_t)dim * chunk), M((size_t)dim * dim);
+    DevBuf<int64_t> di, dcols;
+    op.set_gidx(std::vector<int>(chunk, -1));
     if (bu) std::fill(U_BU, U_BU + (size_t)nd * os.num_BU_chns, cdouble(NAN, NAN));
     for (int j = 0; j < nE; ++j) {
-        // breakup cells of energy j inside the state space: extra right-hand-side columns
+        // right-hand sides: the on-shell columns of A, then the breakup cells of energy j inside the
+        // state space (CPP/solve_faddeev.cpp:372-388)
+        std::vector<int64_t> cols;
+        for (int dc = 0; dc < nd; ++dc) cols.push_back(((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np);
         std::vector<int> bsel;
-        std::vector<int64_t> bcol;
         if (bu)
             for (int b = os.q_com_BU_idx[j]; b < os.q_com_BU_idx[j + 1]; ++b) {
                 const int64_t flat = breakup_flat(os, b, op.Nq, op.Np);
                 if (flat < 0 || flat >= dim) continue;
                 bsel.push_back(b);
-                bcol.push_back(flat);
+                cols.push_back(flat);
             }
-        const int nb = (int)bsel.size();
-        const int nrhs = nd + nb;
-        auto bu_col = [&](int k) { return bcol[k]; };
-        // L = 1 - A G_j (columns scaled), RHS = on-shell columns of A (CPP/solve_faddeev.cpp:372-388)
-        std::vector<cdouble> M((size_t)dim * dim);
-        for (int64_t rr = 0; rr < dim; ++rr)
-            for (int64_t c = 0; c < dim; ++c) {
-                const double2 g = Gh[(size_t)j * dim + c];
-                M[(size_t)rr * dim + c] = (rr == c ? 1.0 : 0.0) - A[(size_t)rr * dim + c] * cdouble(g.x, g.y);
-            }
-        std::vector<cdouble> Bm((size_t)dim * nrhs);
-        for (int64_t rr = 0; rr < dim; ++rr) {
-            for (int dc = 0; dc < nd; ++dc) {
-                int64_t c = ((int64_t)os.deuteron_idx[dc] * op.Nq + os.q_com_idx[j]) * op.Np;
-                Bm[(size_t)rr * nrhs + dc] = A[(size_t)rr * dim + c];
-            }
-            for (int k = 0; k < nb; ++k) Bm[(size_t)rr * nrhs + nd + k] = A[(size_t)rr * dim + bu_col(k)];
+        const int nrhs = (int)cols.size();
+        dcols.upload(cols.data(), cols.size(), op.s);
+        DevBuf<double2> B((size_t)dim * nrhs);
+        const double2 *Gj = op.G.p + (size_t)j * dim;
+        for (int64_t r0 = 0; r0 < dim; r0 += chunk) {
+            const int rc = (int)std::min<int64_t>(chunk, dim - r0);
+            std::vector<int64_t> idx(rc);
+            for (int k = 0; k < rc; ++k) idx[k] = r0 + k;
+            TT_HIP(hipMemsetAsync(E.p, 0, sizeof(double2) * dim * rc, op.s));
+            di.upload(idx.data(), rc, op.s);
+            hipLaunchKernelGGL(k_unit_rows, dim3(nblk(rc, 64)), dim3(64), 0, op.s, E.p, di.p, rc);
+            op.apply(E.p, Y.p, rc, false, false, false);
+            hipLaunchKernelGGL(k_dense_rows, dim3(nblk(dim * rc, 256)), dim3(256), 0, op.s, Y.p, rc, r0, dim, Gj, M.p);
+            hipLaunchKernelGGL(k_dense_rhs, dim3(nblk((int64_t)nrhs * rc, 256)), dim3(256), 0, op.s, Y.p, rc, r0, dim,
+                               dcols.p, nrhs, B.p);
         }
-        // Gaussian elimination with partial pivoting
-        for (int64_t k = 0; k < dim; ++k) {
-            int64_t piv = k;
-            double best = std::abs(M[(size_t)k * dim + k]);
-            for (int64_t i = k + 1; i < dim; ++i)
-                if (std::abs(M[(size_t)i * dim + k]) > best) {
-                    best = std::abs(M[(size_t)i * dim + k]);
-                    piv = i;
-                }
-            if (piv != k) {
-                for (int64_t c = 0; c < dim; ++c) std::swap(M[(size_t)k * dim + c], M[(size_t)piv * dim + c]);
-                for (int c = 0; c < nrhs; ++c) std::swap(Bm[(size_t)k * nrhs + c], Bm[(size_t)piv * nrhs + c]);
-            }
-            cdouble d = M[(size_t)k * dim + k];
-            for (int64_t i = k + 1; i < dim; ++i) {
-                cdouble f = M[(size_t)i * dim + k] / d;
-                if (f == cdouble(0.0)) continue;
-                for (int64_t c = k + 1; c < dim; ++c) M[(size_t)i * dim + c] -= f * M[(size_t)k * dim + c];
-                for (int c = 0; c < nrhs; ++c) Bm[(size_t)i * nrhs + c] -= f * Bm[(size_t)k * nrhs + c];
-            }
-        }
-        for (int64_t i = dim - 1; i >= 0; --i)
-            for (int c = 0; c < nrhs; ++c) {
-                cdouble s = Bm[(size_t)i * nrhs + c];
-                for (int64_t l = i + 1; l < dim; ++l) s -= M[(size_t)i * dim + l] * Bm[(size_t)l * nrhs + c];
-                Bm[(size_t)i * nrhs + c] = s / M[(size_t)i * dim + i];
-            }
+        if (!dense_lu_solve(op.s, M.p, dim, B.p, nrhs))
+            fail(TT_ERR_PHYSICS, "dense solve: 1 - A G has a zero pivot column (singular)");
+        std::vector<double2> Bh((size_t)dim * nrhs);
+        TT_HIP(hipMemcpyAsync(Bh.data(), B.p, sizeof(double2) * Bh.size(), hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipStreamSynchronize(op.s));
         for (int dr = 0; dr < nd; ++dr) {
-            int64_t rr = ((int64_t)os.deuteron_idx[dr] * op.Nq + os.q_com_idx[j]) * op.Np;
-            for (int dc = 0; dc < nd; ++dc) U[(size_t)dr * nd * nE + (size_t)dc * nE + j] = Bm[(size_t)rr * nrhs + dc];
-            for (int k = 0; k < nb; ++k) U_BU[(size_t)dr * os.num_BU_chns + bsel[k]] = Bm[(size_t)rr * nrhs + nd + k];
+            const int64_t rr = ((int64_t)os.deuteron_idx[dr] * op.Nq + os.q_com_idx[j]) * op.Np;
+            for (int dc = 0; dc < nd; ++dc) {
+                const double2 v = Bh[(size_t)dc * dim + rr];
+                U[(size_t)dr * nd * nE + (size_t)dc * nE + j] = cdouble(v.x, v.y);
+            }
+            for (size_t k = 0; k < bsel.size(); ++k) {
+                const double2 v = Bh[(size_t)(nd + k) * dim + rr];
+                U_BU[(size_t)dr * os.num_BU_chns + bsel[k]] = cdouble(v.x, v.y);
+            }
         }
     }
     st.iterations = 0;

@@ -220,6 +220,13 @@ void free_plan_cache(PlanCache *c);
     }                                                 \
     return TT_OK;
 
+namespace tt {
+// LU with partial pivoting of the column-major n x n complex matrix M (device, leading dimension
+// n; overwritten by the factors) and B <- M^-1 B for the column-major n x nrhs B (device, leading
+// dimension n), on the stream.  Returns false when a pivot column is zero.  (k_dense.hip)
+bool dense_lu_solve(hipStream_t s, double2 *M, int64_t n, double2 *B, int nrhs);
+}  // namespace tt
+
 struct tt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
