@@ -354,6 +354,7 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
                                                int out_cell, const int *__restrict__ ket_item_ptr,
                                                double in_scale, double2 *__restrict__ zero_buf, int zero_n) {
     extern __shared__ double smem[];
+    __shared__ int gk[256];   // gidx of this chunk's columns (kc <= 256)
     if (zero_buf && blockIdx.x == 0 && blockIdx.y == 0)   // a buffer a later kernel accumulates into
         for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_buf[i] = make_double2(0.0, 0.0);
     const int64_t dim = (int64_t)Na * Nq * Np;
@@ -366,20 +367,30 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     CT2 *xs = reinterpret_cast<CT2 *>(Bs + ((ne_max * NpNp + 1) & ~1));       // ne * Np * nk
     const CT *Bg = blocks + (size_t)e0 * NpNp;
     for (int i = threadIdx.x; i < ne * NpNp; i += blockDim.x) Bs[i] = Bg[i];
+    if (G)
+        for (int i = threadIdx.x; i < nk; i += blockDim.x) gk[i] = gidx[k0 + i];
+    if (G) __syncthreads();
     const int tile = Np * nk;
     for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
-        const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, k = k0 + rem - (rem / nk) * nk;
+        const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
         const int ae = part_alpha[e0 + e];
         const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
         double xr, xi;
         if (ket_item_ptr) {
-            const int ket = q * Np + pp;
+            // the ket's item panels, four loads in flight at a time, added in item order
+            const int ket = q * Np + pp, i0 = ket_item_ptr[ket], i1 = ket_item_ptr[ket + 1];
             xr = 0.0;
             xi = 0.0;
-            for (int it = ket_item_ptr[ket]; it < ket_item_ptr[ket + 1]; ++it) {
-                const TI v = X[((int64_t)it * Na + ae) * r + k];
-                xr += (double)v.x;
-                xi += (double)v.y;
+            for (int it = i0; it < i1; it += 4) {
+                TI v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = X[((int64_t)min(it + u, i1 - 1) * Na + ae) * r + k];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (it + u < i1) {
+                        xr += (double)v[u].x;
+                        xi += (double)v[u].y;
+                    }
             }
             xr *= in_scale;
             xi *= in_scale;
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
             xi = (double)v.y;
         }
         if (G) {
-            const int g = gidx[k];
+            const int g = gk[kk];
             if (g >= 0) {
                 const GT gv0 = G[(int64_t)g * dim + src];
                 const double2 gv = make_double2((double)gv0.x, (double)gv0.y);
@@ -403,28 +414,44 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         xs[idx].y = (CT)xi;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < tile; o += blockDim.x) {
-        const int p = o / nk, kk = o - p * nk, k = k0 + kk;
-        CT sr = 0, si = 0;
-        for (int e = 0; e < ne; ++e) {
-            const CT *B = Bs + (e * Np + p) * Np;
-            const CT2 *xe = xs + e * tile + kk;
-#pragma unroll 6
-            for (int pp = 0; pp < Np; ++pp) {
-                const CT b = B[pp];
-                const CT2 x = xe[pp * nk];
-                sr += b * x.x;
-                si += b * x.y;
+    // Thread (pg, kk) forms the outputs (p, kk) for p = pg, pg + npg, ... (BP at a time): each
+    // staged input x[pp][kk] is read from LDS once per BP outputs, the block values B[p][pp] are
+    // the same for the nk threads of a p group (LDS broadcast).
+    constexpr int BP = 4;
+    const int npg = blockDim.x / nk, pg = threadIdx.x / nk, kk = threadIdx.x - pg * nk, k = k0 + kk;
+    if (pg < npg)
+        for (int p0 = pg; p0 < Np; p0 += BP * npg) {
+            CT sr[BP], si[BP];
+#pragma unroll
+            for (int i = 0; i < BP; ++i) sr[i] = si[i] = 0;
+            for (int e = 0; e < ne; ++e) {
+                const CT *B = Bs + e * NpNp;
+                const CT2 *xe = xs + e * tile + kk;
+                for (int pp = 0; pp < Np; ++pp) {
+                    const CT2 x = xe[pp * nk];
+#pragma unroll
+                    for (int i = 0; i < BP; ++i) {
+                        const int p = min(p0 + i * npg, Np - 1);   // rows past Np are formed, not stored
+                        const CT b = B[p * Np + pp];
+                        sr[i] += b * x.x;
+                        si[i] += b * x.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < BP; ++i) {
+                const int p = p0 + i * npg;
+                if (p >= Np) continue;
+                const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
+                CT vr = sr[i], vi = si[i];
+                if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
+                    vr = (CT)Zsub[dst].x - vr;
+                    vi = (CT)Zsub[dst].y - vi;
+                }
+                Y[dst].x = vr;
+                Y[dst].y = vi;
             }
         }
-        const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
-        if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
-            sr = (CT)Zsub[dst].x - sr;
-            si = (CT)Zsub[dst].y - si;
-        }
-        Y[dst].x = sr;
-        Y[dst].y = si;
-    }
 }
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
