@@ -229,10 +229,10 @@ def single_channel(ctx, steps, warmup):
             "p123_build_ms": round(P.stats.ms_total, 2), "mixed": round(v_mix, 2), "fp64": round(v_64, 2),
             "mixed_pcie_inclusive": round(v_pcie, 2),
             "mixed_vs_fp64_max_rel": float(np.max(np.abs(U_mix - U_64)) / np.max(np.abs(U_64))),
-            "roofline_fp32": product_roofline(st_mix, shape, "k_cellmfma<float, 2, 1, 4>", True,
-                                              measured_traffic("k_cellmfma<float, 2, 1, 4>")),
-            "roofline_fp64": product_roofline(st_64, shape, "k_cellmfma<double, 2, 1, 2>", False,
-                                              measured_traffic("k_cellmfma<double, 2, 1, 2>"))}
+            "roofline_fp32": product_roofline(st_mix, shape, "k_cellmfma<float, 2, 1, UN>", True,
+                                              measured_traffic("k_cellmfma<float, 2, 1, UN>")),
+            "roofline_fp64": product_roofline(st_64, shape, "k_cellmfma<double, 2, 1, UN>", False,
+                                              measured_traffic("k_cellmfma<double, 2, 1, UN>"))}
 
 
 def dense_leg(ctx):
