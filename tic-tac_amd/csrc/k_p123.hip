@@ -77,26 +77,31 @@ __global__ void k_mask(unsigned char *__restrict__ mask, const double *__restric
     mask[t] = hit ? 1 : 0;
 }
 
-constexpr int KCHUNK = 64;
-
 // F[b][t][k] for the bra bins rb0 + b, k = iphi*Nx + ix (points fastest, so a wave's stores are
-// one contiguous row).  Block: (chunk of KCHUNK points, b), 8 waves; the chunk's tables Y2, Y3 in
-// LDS point-fastest ([i][kk]: a wave's reads of one table entry are conflict-free).
-constexpr int ANG_THREADS = 512;
-__global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F, int rb0, const double *__restrict__ phi,
-                                                         const double *__restrict__ x, const double *__restrict__ Y1tab,
-                                                         int n1, int lmax2, int n2, int lmax3, int n3, int K, int Nx,
-                                                         int Nphi, int T, const int *__restrict__ term_ptr,
+// one contiguous row).  Block: (chunk of kc = 64 or 128 points, b), 16 waves.  In LDS, point-fastest
+// so that a wave's reads of one table entry are conflict-free: the chunk's tables Y2[i][kk],
+// Y3[i][kk], and Y1[i][ix] of all Nx x nodes.
+constexpr int ANG_THREADS = 1024;
+__global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F, int rb0, int kc,
+                                                         const double *__restrict__ phi, const double *__restrict__ x,
+                                                         const double *__restrict__ Y1tab, int n1, int lmax2, int n2,
+                                                         int lmax3, int n3, int K, int Nx, int Nphi, int T,
+                                                         const int *__restrict__ term_ptr,
                                                          const double *__restrict__ coef,
                                                          const int4 *__restrict__ tidx) {
     extern __shared__ double lds[];
-    double *Y2 = lds;                    // n2 * KCHUNK
-    double *Y3 = lds + n2 * KCHUNK;      // n3 * KCHUNK
+    double *Y2 = lds;                    // n2 * kc
+    double *Y3 = lds + n2 * kc;          // n3 * kc
+    double *Y1 = Y3 + n3 * kc;           // n1 * Nx
     const int b = blockIdx.y;
     const int rb = rb0 + b;
-    const int k0 = blockIdx.x * KCHUNK;
-    if (threadIdx.x < 2 * KCHUNK) {
-        const int kk = threadIdx.x % KCHUNK, which = threadIdx.x / KCHUNK;
+    const int k0 = blockIdx.x * kc;
+    for (int i = threadIdx.x; i < n1 * Nx; i += blockDim.x) {
+        const int i1 = i / Nx, ix = i - i1 * Nx;
+        Y1[i] = Y1tab[ix * n1 + i1];
+    }
+    if (threadIdx.x < 2 * kc) {
+        const int kk = threadIdx.x % kc, which = threadIdx.x / kc;
         const int k = min(k0 + kk, K - 1);   // points past K get tables too (their F is not stored)
         const int iphi = k / Nx, ix = k % Nx;
         double sp, cp;
@@ -105,24 +110,24 @@ __global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F,
         if (which == 0) {
             const double z1 = pi1t(sp, cp, xx);
             const double ct1 = fmin(1.0, fmax(-1.0, -(0.5 * sp + 0.75 * cp * xx) / z1));
-            spharm_table(lmax2, ct1, Y2 + kk, KCHUNK);
+            spharm_table(lmax2, ct1, Y2 + kk, kc);
         } else {
             const double z2 = pi2t(sp, cp, xx);
             const double ct2 = fmin(1.0, fmax(-1.0, (sp - 0.5 * cp * xx) / z2));
-            spharm_table(lmax3, ct2, Y3 + kk, KCHUNK);
+            spharm_table(lmax3, ct2, Y3 + kk, kc);
         }
     }
     __syncthreads();
-    // one tuple per wave at a time, its lanes on the points: every lane of a wave runs the same
-    // term list, whose data (coefficient, Y1 row index, Y2 / Y3 offsets pre-scaled by KCHUNK) are
-    // wave-uniform scalar loads; eight terms' table reads are issued before their products are
-    // added (in term order, as a plain loop would)
-    const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    // one tuple per wave at a time, its lanes on 64 of the chunk's points: every lane of a wave runs
+    // the same term list, whose data (coefficient, Y1 row offset pre-scaled by Nx, Y2 / Y3 offsets
+    // pre-scaled by kc) are wave-uniform scalar loads; eight terms' table reads are issued before
+    // their products are added (in term order, as a plain loop would)
+    const int lane = threadIdx.x & 63, nh = kc >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: term data by scalar loads
-    const int k = k0 + lane;
-    const double *y1 = Y1tab + (int64_t)(min(k, K - 1) % Nx) * n1;
-    const double *y2 = Y2 + lane, *y3 = Y3 + lane;
-    for (int t = wave; t < T; t += nwave) {
+    const int kk = (wave % nh) * 64 + lane, k = k0 + kk;
+    const double *y1 = Y1 + min(k, K - 1) % Nx;
+    const double *y2 = Y2 + kk, *y3 = Y3 + kk;
+    for (int t = wave / nh; t < T; t += (blockDim.x >> 6) / nh) {
         const int j0 = term_ptr[t], j1 = term_ptr[t + 1];
         double acc = 0.0;
         int j = j0;
@@ -455,10 +460,14 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
     std::vector<double> Y1((size_t)Nx * n1);
     for (int i = 0; i < Nx; ++i) spharm_table(ang.lmax1, x[i], &Y1[(size_t)i * n1]);
     DBuf<double> d_Y1(Y1.size()), d_coef(std::max<size_t>(ang.term_coef.size(), 1));
-    // term data of k_angular: Y1 row index, Y2 / Y3 offsets in the point-fastest LDS tables
+    // k_angular's chunk of points: 128 where its LDS tables fit, else 64
+    auto ang_lds = [&](int kc) { return (size_t)(kc * (n2 + n3) + n1 * Nx) * sizeof(double); };
+    const int kc = ang_lds(128) <= 160 * 1024 ? 128 : 64;
+    TT_REQUIRE(ang_lds(kc) <= 160 * 1024, "angular tables exceed LDS");
+    // term data of k_angular: offsets in the point-fastest LDS tables
     std::vector<int4> tidx(std::max<size_t>(ang.term_i1.size(), 1));
     for (size_t j = 0; j < ang.term_i1.size(); ++j)
-        tidx[j] = make_int4(ang.term_i1[j], ang.term_i2[j] * KCHUNK, ang.term_i3[j] * KCHUNK, 0);
+        tidx[j] = make_int4(ang.term_i1[j] * Nx, ang.term_i2[j] * kc, ang.term_i3[j] * kc, 0);
     DBuf<int> d_tptr(ang.term_ptr.size());
     DBuf<int4> d_tidx(tidx.size());
     h2d(d_Y1.p, Y1, s);
@@ -482,14 +491,12 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
         DBuf<QPoint> d_Q((size_t)NqNp * K);
         hipLaunchKernelGGL(k_qpoints, dim3(nb((int64_t)NqNp * K, 256)), dim3(256), 0, s, d_Q.p, NqNp, K, Nx, Nphi, Np,
                            d_pe.p, d_qe.p, d_x.p, d_wx.p, d_phi.p, d_wphi.p);
-        const size_t lds = (size_t)KCHUNK * (n2 + n3) * sizeof(double);
-        TT_REQUIRE(lds <= 160 * 1024, "angular tables exceed LDS");
         const int tpb = QUAD_THREADS / 4;   // tuples per k_quad block
         for (int rb0 = 0; rb0 < NqNp; rb0 += batch) {
             const int nrb = std::min(batch, NqNp - rb0);
-            hipLaunchKernelGGL(k_angular, dim3(nb(K, KCHUNK), nrb), dim3(ANG_THREADS), lds, s, d_F.p, rb0, d_phi.p,
-                               d_x.p, d_Y1.p, n1, ang.lmax2, n2, ang.lmax3, n3, K, Nx, Nphi, T, d_tptr.p, d_coef.p,
-                               d_tidx.p);
+            hipLaunchKernelGGL(k_angular, dim3(nb(K, kc), nrb), dim3(ANG_THREADS), ang_lds(kc), s, d_F.p, rb0, kc,
+                               d_phi.p, d_x.p, d_Y1.p, n1, ang.lmax2, n2, ang.lmax3, n3, K, Nx, Nphi, T, d_tptr.p,
+                               d_coef.p, d_tidx.p);
             hipLaunchKernelGGL(k_quad, dim3(nb(T, tpb), nrb), dim3(QUAD_THREADS), 0, s, d_H.p, d_F.p, rb0, T, K, Np,
                                d_rbptr.p, d_rbcells.p, d_cket.p, d_pe.p, d_qe.p, d_Q.p);
         }
