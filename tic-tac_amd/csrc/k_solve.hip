@@ -345,6 +345,50 @@ struct Cplx<float> {
     typedef float2 type;
 };
 
+// Rows p0 .. p0 + BP - 1 (those below Np) of the block op's output column kk, from the staged
+// partner blocks Bs and inputs xs.
+template <int BP, typename CT, typename CT2, typename TO>
+__device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 *__restrict__ xs, int ne, int Np,
+                                           int nk, int tile, int kk, int p0, TO *__restrict__ Y,
+                                           const TO *__restrict__ Zsub, int out_cell, int a, int q, int Na, int Nq,
+                                           int r, int k) {
+    const int NpNp = Np * Np;
+    CT sr[BP], si[BP];
+    int prow[BP];
+#pragma unroll
+    for (int i = 0; i < BP; ++i) {
+        sr[i] = si[i] = 0;
+        prow[i] = min(p0 + i, Np - 1) * Np;   // a remainder row past Np is formed, not stored
+    }
+    for (int e = 0; e < ne; ++e) {
+        const CT *B = Bs + e * NpNp;
+        const CT2 *xe = xs + e * tile + kk;
+#pragma unroll 4
+        for (int pp = 0; pp < Np; ++pp) {
+            const CT2 x = xe[pp * nk];
+#pragma unroll
+            for (int i = 0; i < BP; ++i) {
+                const CT b = B[prow[i] + pp];
+                sr[i] += b * x.x;
+                si[i] += b * x.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BP; ++i) {
+        const int p = p0 + i;
+        if (p >= Np) continue;
+        const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
+        CT vr = sr[i], vi = si[i];
+        if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
+            vr = (CT)Zsub[dst].x - vr;
+            vi = (CT)Zsub[dst].y - vi;
+        }
+        Y[dst].x = vr;
+        Y[dst].y = vi;
+    }
+}
+
 template <typename TI, typename TO, typename CT = double, typename GT = double2>
 __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
                                                const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
@@ -365,20 +409,27 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     typedef typename Cplx<CT>::type CT2;
     CT *Bs = reinterpret_cast<CT *>(smem);                                    // ne * Np * Np
     CT2 *xs = reinterpret_cast<CT2 *>(Bs + ((ne_max * NpNp + 1) & ~1));       // ne * Np * nk
+    // the partners' alphas and the item ranges of this q's kets, read once per workgroup so that
+    // every staged element below costs a single round trip to memory
+    int *s_alpha = reinterpret_cast<int *>(xs + ne_max * Np * kc);            // ne
+    int *s_items = s_alpha + ne_max;                                          // Np + 1
     const CT *Bg = blocks + (size_t)e0 * NpNp;
     for (int i = threadIdx.x; i < ne * NpNp; i += blockDim.x) Bs[i] = Bg[i];
     if (G)
         for (int i = threadIdx.x; i < nk; i += blockDim.x) gk[i] = gidx[k0 + i];
-    if (G) __syncthreads();
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) s_alpha[i] = part_alpha[e0 + i];
+    if (ket_item_ptr)
+        for (int i = threadIdx.x; i <= Np; i += blockDim.x) s_items[i] = ket_item_ptr[q * Np + i];
+    __syncthreads();
     const int tile = Np * nk;
     for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
         const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
-        const int ae = part_alpha[e0 + e];
+        const int ae = s_alpha[e];
         const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
         double xr, xi;
         if (ket_item_ptr) {
             // the ket's item panels, four loads in flight at a time, added in item order
-            const int ket = q * Np + pp, i0 = ket_item_ptr[ket], i1 = ket_item_ptr[ket + 1];
+            const int i0 = s_items[pp], i1 = s_items[pp + 1];
             xr = 0.0;
             xi = 0.0;
             for (int it = i0; it < i1; it += 4) {
@@ -414,44 +465,21 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         xs[idx].y = (CT)xi;
     }
     __syncthreads();
-    // Thread (pg, kk) forms the outputs (p, kk) for p = pg, pg + npg, ... (BP at a time): each
-    // staged input x[pp][kk] is read from LDS once per BP outputs, the block values B[p][pp] are
-    // the same for the nk threads of a p group (LDS broadcast).
-    constexpr int BP = 4;
-    const int npg = blockDim.x / nk, pg = threadIdx.x / nk, kk = threadIdx.x - pg * nk, k = k0 + kk;
-    if (pg < npg)
-        for (int p0 = pg; p0 < Np; p0 += BP * npg) {
-            CT sr[BP], si[BP];
-#pragma unroll
-            for (int i = 0; i < BP; ++i) sr[i] = si[i] = 0;
-            for (int e = 0; e < ne; ++e) {
-                const CT *B = Bs + e * NpNp;
-                const CT2 *xe = xs + e * tile + kk;
-                for (int pp = 0; pp < Np; ++pp) {
-                    const CT2 x = xe[pp * nk];
-#pragma unroll
-                    for (int i = 0; i < BP; ++i) {
-                        const int p = min(p0 + i * npg, Np - 1);   // rows past Np are formed, not stored
-                        const CT b = B[p * Np + pp];
-                        sr[i] += b * x.x;
-                        si[i] += b * x.y;
-                    }
-                }
+    // Thread (g, kk) forms the outputs (p, kk) of the bp consecutive rows p = g bp .. g bp + bp - 1,
+    // with bp the fewest rows per thread that cover the Np rows with the block's threads (so no row
+    // is formed twice and none past Np, but for the last group's remainder): each staged input
+    // x[pp][kk] is read from LDS once per bp outputs, the block values B[p][pp] are the same for the
+    // nk threads of a group (LDS broadcast).
+    // (with more than 4 npg rows, groups take further runs of 4 rows, npg runs apart)
+    const int npg = blockDim.x / nk, bp = min(4, (Np + npg - 1) / npg);
+    const int g = threadIdx.x / nk, kk = threadIdx.x - g * nk;
+    if (g < npg)
+        for (int p0 = g * bp; p0 < Np; p0 += npg * bp) switch (bp) {
+                case 1: block_rows<1>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
+                case 2: block_rows<2>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
+                case 3: block_rows<3>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
+                default: block_rows<4>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
             }
-#pragma unroll
-            for (int i = 0; i < BP; ++i) {
-                const int p = p0 + i * npg;
-                if (p >= Np) continue;
-                const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
-                CT vr = sr[i], vi = si[i];
-                if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
-                    vr = (CT)Zsub[dst].x - vr;
-                    vi = (CT)Zsub[dst].y - vi;
-                }
-                Y[dst].x = vr;
-                Y[dst].y = vi;
-            }
-        }
 }
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
@@ -470,7 +498,8 @@ static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr,
     const int nch = (r + 23) / 24;
     const int kc = (r + nch - 1) / nch;
     const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(CT) +
-                       (size_t)ne_max * Np * kc * sizeof(typename Cplx<CT>::type);
+                       (size_t)ne_max * Np * kc * sizeof(typename Cplx<CT>::type) +
+                       (size_t)(ne_max + Np + 1) * sizeof(int);
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
     hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
