@@ -66,7 +66,7 @@ class BatchWorkload:
         self.plain = [[self.prob.channel(c, [T]) for T in BATCH_T] for c in range(self.prob.n_channels)]
         self.n_solves = len(self.chans) * len(BATCH_T)
 
-    graphs = False
+    graphs = None   # the library's choice
 
     def solve(self, ctx, method):
         return self.prob.solve_batch(ctx, self.chans, self.Ps, method=method, tol=SOLVE_TOL, graphs=self.graphs)
@@ -458,8 +458,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--legs", default=",".join(x for x in LEGS if x != "copy"),
                     help="comma-separated extra lines beside the headline batch: " + ", ".join(LEGS) + " (or none)")
-    ap.add_argument("--graphs", action="store_true", help="replay captured HIP graphs of the Arnoldi steps in the "
-                    "batched solves (default: plain stream launches)")
+    ap.add_argument("--graphs", choices=("auto", "on", "off"), default="auto",
+                    help="replayed HIP graphs of the Arnoldi steps in the batched solves: the library's choice "
+                    "(auto: graphs for groups of 4 or more channels), or forced on / off")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each channel-solve is "
@@ -473,7 +474,7 @@ def main():
     ctx = tt.Context(0)
     log("building P123 of the 18 channels")
     wl = BatchWorkload(ctx)
-    wl.graphs = args.graphs
+    wl.graphs = {"auto": None, "on": True, "off": False}[args.graphs]
     log("P123 built in %.1f s; timing the batched mixed solve" % wl.build_s)
     t_mix, U_mix, st_mix = time_batches(ctx, wl, tt.SOLVE_MIXED, args.steps, args.warmup)
     log("batched FP64")
@@ -500,6 +501,7 @@ def main():
         "config": {"workload": BATCH_WORKLOAD, "channel_solves_per_step": wl.n_solves,
                    "nnz_P123_total": int(sum(P.nnz for P in wl.Ps)),
                    "p123_build_s_all_channels": round(wl.build_s, 2), "solve_tol": SOLVE_TOL,
+                   "step_graphs": args.graphs,
                    "channels": table},
         "batched_fp64": round(fp64, 2),
         "mixed_over_batched_fp64": round(value / fp64, 3),

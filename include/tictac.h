@@ -246,12 +246,14 @@ typedef struct {
     double *neumann_terms;
     int p123_format;            /* tt_p123_format                           */
     int use_graphs;             /* 1: each Arnoldi step is captured once as a HIP graph and
-                                 * replayed; 0: plain stream launches (default: on MI355X a
-                                 * replay of the step's half-dozen kernels costs more device
-                                 * time than launching them).  The context
-                                 * keeps the operator and Krylov workspace of recent channels
-                                 * (keyed by the P123 object and a hash of V, C and the channel),
-                                 * so repeated solves of a channel only upload G. */
+                                 * replayed; 0: plain stream launches; -1 (default): graphs for
+                                 * lockstep groups of 4 or more channels (tt_solve_batch),
+                                 * where they save host launch time, plain launches otherwise
+                                 * (for one channel a graph launch costs more than its
+                                 * kernels' launches).  The context keeps the operator and
+                                 * Krylov workspace of recent channels (keyed by the P123
+                                 * object, the channel and the exact contents of V and C), so
+                                 * repeated solves of a channel only upload G. */
     int resolvent_on_device;    /* 1: ignore G (may be NULL) and compute the resolvent of the
                                  * on-shell energies E_com + E_bound on the device, straight
                                  * into the solver's buffer (no host round trip); 0: use G. */

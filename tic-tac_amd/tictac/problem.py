@@ -130,14 +130,15 @@ class Problem:
         return Channel(self, c, T_lab, breakup)
 
     def solve_batch(self, ctx, channels, P123s, method=L.SOLVE_MIXED, tol=1e-12, max_iter=2000, restart=40,
-                    graphs=False):
+                    graphs=None):
         """tt_solve_batch over channels (each with all its on-shell bins), G on the device.
+        graphs: True / False force replayed step graphs on / off, None leaves the library's choice.
         Returns [(U[d_row, d_col, j], stats, status)] per channel; raises on the first failure
         after every channel was attempted."""
         o = L.SolveOpts()
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart = method, tol, max_iter, restart
-        o.use_graphs = int(bool(graphs))
+        o.use_graphs = -1 if graphs is None else int(bool(graphs))
         n = len(channels)
         items = (L.BatchItem * max(n, 1))()
         Us, stats, Ubus = [], [], []
@@ -358,8 +359,9 @@ class Channel:
         return G.view(np.complex128).reshape(len(E), self.dim)
 
     def solve(self, ctx, P, G=None, method=L.SOLVE_PADE, tol=1e-12, max_iter=2000, restart=40, NM_max=14,
-              neumann=False, p123_format=L.P123_CELLS, graphs=False, device_resolvent=False):
+              neumann=False, p123_format=L.P123_CELLS, graphs=None, device_resolvent=False):
         """On-shell U[d_row, d_col, j] (CPP/solve_faddeev.cpp:424 layout) and solve statistics.
+        graphs: True / False force replayed step graphs on / off, None leaves the library's choice.
 
         With ``neumann=True`` (Pade) also returns the Neumann terms a[d_row, d_col, j, n].
         With ``device_resolvent=True`` G is computed on the device inside the call (G ignored).
@@ -376,7 +378,7 @@ class Channel:
         L.lib().tt_solve_opts_default(C.byref(o))
         o.method, o.tol, o.max_iter, o.restart, o.NM_max = method, tol, max_iter, restart, NM_max
         o.p123_format = p123_format
-        o.use_graphs = int(bool(graphs))
+        o.use_graphs = -1 if graphs is None else int(bool(graphs))
         o.resolvent_on_device = int(bool(device_resolvent))
         nterms = 2 * NM_max + 1
         a = np.zeros(self.nd * self.nd * self.nE * nterms * 2)
