@@ -235,6 +235,24 @@ def single_channel(ctx, steps, warmup):
                                               measured_traffic("k_cellmfma<double, 2, 1, UN>"))}
 
 
+def dominant_alone(ctx, wl, reps=4):
+    """The batch's dominant product shape measured without the other lanes: the first N_alpha = 63
+    channel (7/2+, dim 56,700) with all 8 T_lab (r = 18 systems, R2 = 36) solved alone on one
+    stream, mixed and FP64, reps times after a warm-up; the product's roofline over those
+    launches (the same instances the batch runs, k_cellmfma<*, 4, 3, UN>)."""
+    i = next(k for k, ch in enumerate(wl.chans) if ch.Nalpha == 63)
+    ch, P = wl.chans[i], wl.Ps[i]
+    row = wl.channel_table()[i]
+    shape = [(row["nnz"], row["dim"], row["r"])]
+    res = {"channel": row["JP"], "dim": row["dim"], "r": row["r"]}
+    for method, fp32, key in ((tt.SOLVE_MIXED, True, "roofline_fp32"), (tt.SOLVE_GMRES, False, "roofline_fp64")):
+        ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)
+        st = [ch.solve(ctx, P, None, method=method, tol=SOLVE_TOL, device_resolvent=True)[1] for _ in range(reps)]
+        k = "k_cellmfma<%s, 4, 3, UN>" % ("float" if fp32 else "double")
+        res[key] = product_roofline(st, shape, k, fp32, measured_traffic(k))
+    return res
+
+
 def dense_leg(ctx):
     """The dense path (solve_dense=true) on the single-channel workload: per on-shell energy, L = 1 -
     A G_j (dim 24,300, complex FP64) formed and LU-factorised with partial pivoting on the device;
@@ -515,6 +533,8 @@ def main():
                                      "fp64": iteration_summary(st_64, len(wl.chans))},
         "cell_autotune": tt.cell_tuning(),
     }
+    log("dominant product shape alone")
+    out["roofline_alone"] = dominant_alone(ctx, wl)
     U_plain = U_plain_ch = None
     if "plain" in legs or "cpu" in legs:
         log("plain FP64: one call per channel and T_lab")
