@@ -182,39 +182,47 @@ __global__ void k_laswp_all(double2 *__restrict__ B, int64_t ld, int64_t n, int 
 __global__ __launch_bounds__(NB) void k_trinv(const double2 *__restrict__ M, int64_t ld, int64_t k0, int w, int lower,
                                               double2 *__restrict__ W) {
     __shared__ double2 T[NB * NB];   // T[r * NB + t] = M[k0 + r][k0 + t]
+    __shared__ double2 X[NB * NB];   // X[t * NB + j] = row t of the inverse's column j
     for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
         const int r = e % w, t = e / w;
         T[r * NB + t] = M[(k0 + t) * ld + k0 + r];
     }
     __syncthreads();
+    // thread j forms column j by substitution, its partial results in LDS (the threads of a row t
+    // touch consecutive words; the T values they read are the same word: a broadcast)
     const int j = threadIdx.x;
-    if (j >= w) return;
-    double2 *x = W + (int64_t)j * NB;
-    if (lower) {
-        for (int r = 0; r < w; ++r) {
-            double2 s = make_double2(r == j ? 1.0 : 0.0, 0.0);
-            if (r > j)
-                for (int t = j; t < r; ++t) {
-                    const double2 a = T[r * NB + t], b = x[t];
+    if (j < w) {
+        if (lower) {
+            for (int r = 0; r < w; ++r) {
+                double2 s = make_double2(r == j ? 1.0 : 0.0, 0.0);
+                if (r > j)
+                    for (int t = j; t < r; ++t) {
+                        const double2 a = T[r * NB + t], b = X[t * NB + j];
+                        s.x -= a.x * b.x - a.y * b.y;
+                        s.y -= a.x * b.y + a.y * b.x;
+                    }
+                X[r * NB + j] = r < j ? make_double2(0.0, 0.0) : s;
+            }
+        } else {
+            for (int r = w - 1; r >= 0; --r) {
+                if (r > j) {
+                    X[r * NB + j] = make_double2(0.0, 0.0);
+                    continue;
+                }
+                double2 s = make_double2(r == j ? 1.0 : 0.0, 0.0);
+                for (int t = r + 1; t <= j; ++t) {
+                    const double2 a = T[r * NB + t], b = X[t * NB + j];
                     s.x -= a.x * b.x - a.y * b.y;
                     s.y -= a.x * b.y + a.y * b.x;
                 }
-            x[r] = r < j ? make_double2(0.0, 0.0) : s;
-        }
-    } else {
-        for (int r = w - 1; r >= 0; --r) {
-            if (r > j) {
-                x[r] = make_double2(0.0, 0.0);
-                continue;
+                X[r * NB + j] = cmul(s, crecip(T[r * NB + r]));
             }
-            double2 s = make_double2(r == j ? 1.0 : 0.0, 0.0);
-            for (int t = r + 1; t <= j; ++t) {
-                const double2 a = T[r * NB + t], b = x[t];
-                s.x -= a.x * b.x - a.y * b.y;
-                s.y -= a.x * b.y + a.y * b.x;
-            }
-            x[r] = cmul(s, crecip(T[r * NB + r]));
         }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {   // W[j * NB + r] = column j, row r
+        const int r = e % w, jj = e / w;
+        W[(int64_t)jj * NB + r] = X[r * NB + jj];
     }
 }
 
