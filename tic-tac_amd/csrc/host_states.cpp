@@ -222,6 +222,41 @@ void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, cons
     }
 }
 
+void embed_2N_layout(const tt_pw_states &chn, int Np, int n_unco, const tt_run_params &rp, bool transpose,
+                     std::vector<int> &ptr, std::vector<int> &alpha, std::vector<int64_t> &src,
+                     std::vector<int> &ld) {
+    const int Na = chn.Nalpha;
+    ptr.assign(Na + 1, 0);
+    alpha.clear();
+    src.clear();
+    ld.clear();
+    for (int a = 0; a < Na; ++a) {
+        for (int b = 0; b < Na; ++b) {
+            const int ar = transpose ? b : a, ac = transpose ? a : b;
+            int Lr = chn.L_2N[ar], Sr = chn.S_2N[ar], Jr = chn.J_2N[ar], Tr = chn.T_2N[ar];
+            int Lc = chn.L_2N[ac];
+            bool checkL = (rp.tensor_force && std::abs(Lr - Lc) <= 2) || Lr == Lc;
+            if (!(Tr == chn.T_2N[ac] && Jr == chn.J_2N[ac] && Sr == chn.S_2N[ac] && checkL &&
+                  chn.L_1N[ar] == chn.L_1N[ac] && chn.two_J_1N[ar] == chn.two_J_1N[ac]))
+                continue;
+            bool via_L, via_T;
+            coupling_kind(Lr, Sr, Jr, Lc, rp, via_L, via_T);
+            if (via_L || via_T) {
+                const int idx = unique_2N_idx(Lr, Sr, Jr, Tr, true, rp);
+                const int hr = half_of(chn, ar, via_L), hc = half_of(chn, ac, via_L);
+                src.push_back((int64_t)n_unco * Np * Np + (int64_t)idx * 4 * Np * Np + (int64_t)hr * Np * 2 * Np +
+                              (int64_t)hc * Np);
+                ld.push_back(2 * Np);
+            } else {
+                src.push_back((int64_t)unique_2N_idx(Lr, Sr, Jr, Tr, false, rp) * Np * Np);
+                ld.push_back(Np);
+            }
+            alpha.push_back(b);
+        }
+        ptr[a + 1] = (int)alpha.size();
+    }
+}
+
 void swp_edge_offsets(const tt_pw_states &chn, const tt_swp_states &swp, const tt_run_params &rp,
                       std::vector<int> &which, std::vector<int> &offset) {
     const int Np = swp.Np_WP;

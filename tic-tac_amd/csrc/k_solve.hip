@@ -1282,7 +1282,36 @@ struct VCProducts {
     ContentRef content;     // the V and C they were made from
     bool valid = false;
     std::vector<double> u, c;
+    // on the device, for forming every channel's partner blocks there: [C_unco | C_coup] and
+    // [VC_unco | VC_coup] (uploaded once per V and C, shared by all channel plans)
+    DevBuf<double> dC, dVC;
+    bool dev_valid = false;
+    void upload(const tt_swp_states &swp, int nu, int nc, hipStream_t s) {
+        const size_t Np = (size_t)swp.Np_WP, nu2 = (size_t)nu * Np * Np, nc2 = (size_t)nc * 4 * Np * Np;
+        std::vector<double> t(std::max<size_t>(nu2 + nc2, 1), 0.0);
+        std::copy(swp.C_SWP_unco, swp.C_SWP_unco + nu2, t.begin());
+        if (nc) std::copy(swp.C_SWP_coup, swp.C_SWP_coup + nc2, t.begin() + nu2);
+        dC.upload(t.data(), t.size(), s);
+        std::copy(u.begin(), u.begin() + nu2, t.begin());
+        if (nc) std::copy(c.begin(), c.begin() + nc2, t.begin() + nu2);
+        dVC.upload(t.data(), t.size(), s);
+        dev_valid = true;
+    }
 };
+
+// blocks[e][p][pp] = T[src[e] + pr * ld[e] + pc] with (pr, pc) = (pp, p) transposed, else (p, pp);
+// its FP32 copy beside (the partner blocks of a channel from the shared 2N tables)
+__global__ void k_embed(double *__restrict__ blocks, float *__restrict__ blocks32, const double *__restrict__ T,
+                        const int64_t *__restrict__ src, const int *__restrict__ ld, int nb, int Np, int transpose) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t NpNp = (int64_t)Np * Np;
+    if (t >= nb * NpNp) return;
+    const int e = (int)(t / NpNp), i = (int)(t - e * NpNp), p = i / Np, pp = i - p * Np;
+    const int pr = transpose ? pp : p, pc = transpose ? p : pp;
+    const double v = T[src[e] + (int64_t)pr * ld[e] + pc];
+    blocks[t] = v;
+    blocks32[t] = (float)v;
+}
 static void vc_products(const tt_run_params &rp, const double *V_unco, const double *V_coup,
                         const tt_swp_states &swp, VCProducts &out) {
     int nu, nc;
@@ -1316,6 +1345,9 @@ struct RowOperator {
     int64_t dim;
     const DeviceCSC *P;
     DevBuf<int> c_ptr, c_alpha, vc_ptr, vc_alpha;
+    DevBuf<int64_t> c_src, vc_src;   // where each partner block sits in the 2N tables
+    DevBuf<int> c_ld, vc_ld;
+    int c_nb = 0, vc_nb = 0;         // partner blocks (0: layout not made yet)
     DevBuf<double> c_blocks, vc_blocks;
     DevBuf<float> c_blocks32, vc_blocks32;   // FP32 copies for the block ops on FP32 vectors
     DevBuf<double2> G;
@@ -1390,7 +1422,7 @@ struct RowOperator {
 
     void setup(hipStream_t stream, const tt_run_params &rp, const DeviceCSC *csc, const double *V_unco,
                const double *V_coup, const tt_swp_states &swp, const tt_pw_states &chn,
-               int p123_format = TT_P123_CELLS, bool col_ = false, const VCProducts *vc = nullptr) {
+               int p123_format = TT_P123_CELLS, bool col_ = false, VCProducts *vc = nullptr) {
         s = stream;
         cells = p123_format == TT_P123_CELLS;
         col = col_;
@@ -1414,52 +1446,67 @@ struct RowOperator {
 
     // C_op / VC_op partner blocks from V and the SWP eigenvectors.  The partner lists are
     // structural (zero blocks kept), so new V or C (another theta) only rewrite the block
-    // contents in the same buffers and captured graphs stay valid.  vc: the V.C products of
-    // these V and C when the caller has them (they are the same for every 3N channel).
+    // contents in the same buffers and captured graphs stay valid.  The layout (which 2N block
+    // each partner block reads) is made once; the blocks are then formed on the device from the
+    // 2N tables of C and V.C (k_embed), which the caller's vc holds on the device for every
+    // channel of these V and C (else they are made and uploaded here).
     void set_blocks(const tt_run_params &rp, const double *V_unco, const double *V_coup, const tt_swp_states &swp,
-                    const tt_pw_states &chn, const VCProducts *vc = nullptr) {
-        VCProducts own;
+                    const tt_pw_states &chn, VCProducts *vc = nullptr) {
+        int nu, nc;
+        num_2N_states(rp, nu, nc);
+        std::unique_ptr<VCProducts> own;
         if (!vc) {
-            vc_products(rp, V_unco, V_coup, swp, own);
-            vc = &own;
+            own.reset(new VCProducts);
+            vc_products(rp, V_unco, V_coup, swp, *own);
+            vc = own.get();
         }
-        const std::vector<double> &VCu = vc->u, &VCc = vc->c;
-        std::vector<int> p1, a1, p2, a2;
-        std::vector<double> b1, b2;
-        const double *Cu = swp.C_SWP_unco, *Cc = swp.C_SWP_coup;
-        if (!col) {
-            embed_2N_blocks(chn, Np, Cu, Cc, rp, false, p1, a1, b1, true);                 // S = C_op (X G)
-            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, true, p2, a2, b2, true);   // Y = VC_op^T U
-        } else {
-            embed_2N_blocks(chn, Np, VCu.data(), VCc.data(), rp, false, p1, a1, b1, true);  // S = VC_op (X G)
-            embed_2N_blocks(chn, Np, Cu, Cc, rp, true, p2, a2, b2, true);                   // Y = C_op^T U
+        if (!vc->dev_valid) vc->upload(swp, nu, nc, s);
+        if (!c_nb) {
+            std::vector<int> p1, a1, p2, a2, l1, l2;
+            std::vector<int64_t> s1, s2;
+            // row form: S = C_op (X G), Y = VC_op^T U; column form: S = VC_op (X G), Y = C_op^T U
+            embed_2N_layout(chn, Np, nu, rp, false, p1, a1, s1, l1);
+            embed_2N_layout(chn, Np, nu, rp, true, p2, a2, s2, l2);
+            int m1 = 1, m2 = 1;
+            for (int a = 0; a < Na; ++a) {
+                m1 = std::max(m1, p1[a + 1] - p1[a]);
+                m2 = std::max(m2, p2[a + 1] - p2[a]);
+            }
+            ++layout_gen;   // new buffers and launch parameters of the graphs
+            c_npart_max = m1;
+            vc_npart_max = m2;
+            c_nb = (int)a1.size();
+            vc_nb = (int)a2.size();
+            a1.push_back(0);
+            a2.push_back(0);
+            s1.push_back(0);
+            s2.push_back(0);
+            l1.push_back(0);
+            l2.push_back(0);
+            c_ptr.upload(p1.data(), p1.size(), s);
+            c_alpha.upload(a1.data(), a1.size(), s);
+            c_src.upload(s1.data(), s1.size(), s);
+            c_ld.upload(l1.data(), l1.size(), s);
+            vc_ptr.upload(p2.data(), p2.size(), s);
+            vc_alpha.upload(a2.data(), a2.size(), s);
+            vc_src.upload(s2.data(), s2.size(), s);
+            vc_ld.upload(l2.data(), l2.size(), s);
+            const size_t n1 = (size_t)c_nb * Np * Np + 1, n2 = (size_t)vc_nb * Np * Np + 1;
+            c_blocks.alloc(n1);
+            c_blocks32.alloc(n1);
+            vc_blocks.alloc(n2);
+            vc_blocks32.alloc(n2);
         }
-        int m1 = 1, m2 = 1;
-        for (int a = 0; a < Na; ++a) {
-            m1 = std::max(m1, p1[a + 1] - p1[a]);
-            m2 = std::max(m2, p2[a + 1] - p2[a]);
-        }
-        auto up = [&](auto &buf, const auto &v) {
-            const size_t n = std::max<size_t>(v.size(), 1);
-            if (n > buf.n) ++layout_gen;
-            buf.upload(v.data(), n, s);
-        };
-        if (m1 != c_npart_max || m2 != vc_npart_max) ++layout_gen;   // a launch parameter of the graphs
-        c_npart_max = m1;
-        vc_npart_max = m2;
-        a1.push_back(0);
-        a2.push_back(0);
-        b1.push_back(0.0);
-        b2.push_back(0.0);
-        up(c_ptr, p1);
-        up(c_alpha, a1);
-        up(c_blocks, b1);
-        up(vc_ptr, p2);
-        up(vc_alpha, a2);
-        up(vc_blocks, b2);
-        std::vector<float> f1(b1.begin(), b1.end()), f2(b2.begin(), b2.end());
-        up(c_blocks32, f1);
-        up(vc_blocks32, f2);
+        const int64_t nn1 = (int64_t)c_nb * Np * Np, nn2 = (int64_t)vc_nb * Np * Np;
+        const double *T1 = col ? vc->dVC.p : vc->dC.p, *T2 = col ? vc->dC.p : vc->dVC.p;
+        if (nn1)
+            hipLaunchKernelGGL(k_embed, dim3(nblk(nn1, 256)), dim3(256), 0, s, c_blocks.p, c_blocks32.p, T1, c_src.p,
+                               c_ld.p, c_nb, Np, 0);
+        if (nn2)
+            hipLaunchKernelGGL(k_embed, dim3(nblk(nn2, 256)), dim3(256), 0, s, vc_blocks.p, vc_blocks32.p, T2,
+                               vc_src.p, vc_ld.p, vc_nb, Np, 1);
+        TT_HIP(hipGetLastError());
+        if (own) TT_HIP(hipStreamSynchronize(s));   // own tables are freed on return
     }
 
     // Device buffer for the resolvent of nE energies (contents written by the caller).
@@ -2569,9 +2616,10 @@ static SolvePlan &get_plan(tt_ctx *ctx, const tt_run_params &rp, const DeviceCSC
     PlanCache &c = *ctx->plans;
     const std::vector<int> key = plan_struct_key(rp, swp, chn, format);
     const ContentRef content = content_key ? *content_key : plan_content(rp, V_unco, V_coup, swp);
-    auto products = [&]() -> const VCProducts & {
+    auto products = [&]() -> VCProducts & {
         if (!c.vc.valid || !same_content(c.vc.content, content)) {
             c.vc.valid = false;
+            c.vc.dev_valid = false;
             vc_products(rp, V_unco, V_coup, swp, c.vc);
             c.vc.content = content;
         }

@@ -115,6 +115,13 @@ void coupling_kind(int Lr, int Sr, int Jr, int Lc, const tt_run_params &rp, bool
 // CPP/solve_faddeev.cpp:1588-1927).  Dense (Nalpha*Np)^2 layout, row-major.
 std::vector<double> embed_2N(const tt_pw_states &chn, int Np, const double *M_unco,
                              const double *M_coup, const tt_run_params &rp);
+// The same blocks (all of them, zero or not) as positions in the table [M_unco | M_coup]
+// (unco: n_unco Np x Np matrices, then the 2Np x 2Np coupled ones): block e of alpha a is
+// element (p, pp) = table[src[e] + pr * ld[e] + pc], (pr, pc) = (pp, p) when transposed, else
+// (p, pp).  For forming the blocks on the device from the 2N tables.
+void embed_2N_layout(const tt_pw_states &chn, int Np, int n_unco, const tt_run_params &rp, bool transpose,
+                     std::vector<int> &ptr, std::vector<int> &alpha, std::vector<int64_t> &src,
+                     std::vector<int> &ld);
 void embed_2N_blocks(const tt_pw_states &chn, int Np, const double *M_unco, const double *M_coup,
                      const tt_run_params &rp, bool transpose, std::vector<int> &ptr, std::vector<int> &alpha,
                      std::vector<double> &blocks, bool keep_zero = false);
