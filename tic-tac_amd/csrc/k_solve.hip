@@ -1731,38 +1731,23 @@ struct RowOperator {
 
     // Y = (X .* G_{gidx}) A ; X, Y device, dim*r complex; use_G selects whether G multiplies X.
     // Y = (X .* G_{gidx}) A, or Y = Zsub - (X .* G_{gidx}) A when Zsub is given.
-    // With ext (four events), the timing events are the caller's (a captured graph records
-    // them on every replay and its owner reads them with add_timing); otherwise timed
-    // applications take four events of the pool, read back in collect().
+    // Timed applications take four events of the pool, read back in collect().
     // VT: element type of X, Y and Zsub (double2, or float2 for the FP32 vectors of the inner
     // solves of iterative refinement, which take the FP32 product).
     template <typename VT>
-    void apply(const VT *X, VT *Y, int r, bool use_G, bool fp32, bool timed = true, const VT *Zsub = nullptr,
-               const hipEvent_t *ext = nullptr) {
+    void apply(const VT *X, VT *Y, int r, bool use_G, bool fp32, bool timed = true, const VT *Zsub = nullptr) {
         constexpr bool v64 = std::is_same<VT, double2>::value;
         TT_REQUIRE(v64 || fp32, "FP32 vectors take the FP32 product");
         prepare(r, fp32);
-        if (!ext && timed && ev_used + 4 > 4096) collect();
+        if (timed && ev_used + 4 > 4096) collect();
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-        if (ext) {
-            e0 = ext[0];
-            e1 = ext[1];
-            e2 = ext[2];
-            e3 = ext[3];
-            timed = true;
-        } else if (timed) {
+        if (timed) {
             e0 = next_event();
             e1 = next_event();
             e2 = next_event();
             e3 = next_event();
         }
-        // caller events are recorded as external event nodes when the stream is being captured
-        auto record = [&](hipEvent_t e) {
-            if (ext)
-                TT_HIP(hipEventRecordWithFlags(e, s, hipEventRecordExternal));
-            else
-                TT_HIP(hipEventRecord(e, s));
-        };
+        auto record = [&](hipEvent_t e) { TT_HIP(hipEventRecord(e, s)); };
         if (timed) record(e0);
         const int R2 = 2 * r;
         const int cm = cells ? 1 : 0;
@@ -1831,10 +1816,8 @@ struct RowOperator {
         }
         if (timed) {
             record(e3);
-            if (!ext) {
-                spmm_bytes_of.push_back(product_bytes(r, fp32));
-                spmm_fp32_of.push_back(fp32 ? 1 : 0);
-            }
+            spmm_bytes_of.push_back(product_bytes(r, fp32));
+            spmm_fp32_of.push_back(fp32 ? 1 : 0);
         }
     }
 
@@ -1847,19 +1830,6 @@ struct RowOperator {
                                        Nq, Np, r, nullptr, 0, 1, nullptr, 1.0);
         launch_cellmm<double>(s, &cv, items_for(false, 2 * r).ref(), cv.blk, reinterpret_cast<const double *>(S.p),
                               reinterpret_cast<double *>(Ucell), upart.p, 2 * r, 2.0, choice(false, 2 * r).un);
-    }
-
-    // Adds the timing of one application recorded into caller events (after they completed).
-    // Returns false, adding nothing, where the runtime keeps no timestamps for them.
-    bool add_timing(const hipEvent_t *ev, double bytes, bool fp32) {
-        float a = 0.f, b = 0.f;
-        if (hipEventElapsedTime(&a, ev[0], ev[3]) != hipSuccess || hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess) {
-            (void)hipGetLastError();
-            return false;
-        }
-        ms_apply += a;
-        add_product_time(b, bytes, fp32);
-        return true;
     }
 
     void add_product_time(double ms, double bytes, bool fp32) {
@@ -1959,7 +1929,6 @@ struct Krylov : KrylovBase {
     double2 *n2h = nullptr;             // pinned copy of n2d (squared norms)
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
-        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         uint64_t gen = 0;
     };
     std::vector<StepGraph> graphs;
@@ -1972,7 +1941,6 @@ struct Krylov : KrylovBase {
     int launched = 0;
     bool hold = false, conv = false;
     double est = 0.0;                   // largest relative residual estimate at the end of the cycle
-    bool graph_timing = true;
 
     Krylov(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
         TT_REQUIRE(r <= 256, "at most 256 simultaneous systems");
@@ -1999,8 +1967,6 @@ struct Krylov : KrylovBase {
     ~Krylov() {
         for (auto &gr : graphs) {
             if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
-            for (auto e : gr.ev)
-                if (e) (void)hipEventDestroy(e);
         }
         for (auto e : step_ev) (void)hipEventDestroy(e);
         if (Hh) (void)hipHostFree(Hh);
@@ -2039,13 +2005,13 @@ struct Krylov : KrylovBase {
     // The work of Arnoldi step j: v_{j+1} and Hessenberg column j (slot j of Hd, mirrored into Hh).
     // The product's launch time is sampled in the first step of a cycle only: timing events
     // are not free (a few microseconds of gap each).
-    void enqueue_step(int j, const hipEvent_t *ev, bool capturing = false) {
+    void enqueue_step(int j, bool capturing = false) {
         VT *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
         double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
         op.zero_next = h1;   // h1, h2 and n2 accumulate by atomics: the step's first kernel zeroes them
         op.zero_next_n = slot_len;
         // the new vector w = M v_j is formed in place, in column j + 1 of the basis
-        op.apply(vj, vj1, r, true, fp32, ev != nullptr || (!capturing && j == 0), vj, ev);
+        op.apply(vj, vj1, r, true, fp32, !capturing && j == 0, vj);
         // FP64: two classical Gram-Schmidt passes, the norm of the second folded in (one pass with
         // the Pythagorean norm ||w||^2 - ||h||^2 loses the normalisation of the basis)
         dots_dev(V.p, j + 1, vj1, h1);
@@ -2069,9 +2035,11 @@ struct Krylov : KrylovBase {
     }
 
     // Launch Arnoldi step j (a replay of its graph, captured on first use) and record step_ev[j].
+    // The first step of a cycle always runs as plain launches: the product's launch time is
+    // sampled there, by events on the stream (event nodes inside a graph add their own gaps).
     void step(int j) {
-        if (!use_graphs) {
-            enqueue_step(j, nullptr);
+        if (!use_graphs || j == 0) {
+            enqueue_step(j);
             TT_HIP(hipEventRecord(step_ev[j], op.s));
             return;
         }
@@ -2080,15 +2048,11 @@ struct Krylov : KrylovBase {
         if (!gr.exec || gr.gen != op.layout_gen) {
             if (gr.exec) TT_HIP(hipGraphExecDestroy(gr.exec));
             gr.exec = nullptr;
-            for (auto &e : gr.ev)
-                if (!e) TT_HIP(hipEventCreate(&e));
             const int na = op.n_apply, na32 = op.n_apply32;  // capture runs nothing: count launches below
             TT_HIP(hipStreamBeginCapture(op.s, hipStreamCaptureModeThreadLocal));
             hipGraph_t graph = nullptr;
             try {
-                // timing events (external event nodes, a few microseconds of gap each) only in
-                // the first step of a cycle: the product's launch time is sampled there
-                enqueue_step(j, j == 0 ? gr.ev : nullptr, true);
+                enqueue_step(j, true);
             } catch (...) {
                 (void)hipStreamEndCapture(op.s, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -2109,14 +2073,6 @@ struct Krylov : KrylovBase {
         TT_HIP(hipEventRecord(step_ev[j], op.s));
         ++op.n_apply;
         if (fp32) ++op.n_apply32;
-    }
-
-    // Timing of a completed step (graph mode records the application of step 0 into the
-    // graph's events as external event nodes); stops asking once the runtime reports no
-    // timestamps for them.
-    void read_step_timing(int j) {
-        if (use_graphs && graph_timing && j == 0)
-            graph_timing = op.add_timing(graphs[j].ev, op.product_bytes(r, fp32), fp32);
     }
 
     // Starts a cycle on the residual R (FP64, device) whose column norms beta are in n2d
@@ -2149,7 +2105,6 @@ struct Krylov : KrylovBase {
 
     // Hessenberg column j (the step has completed): Givens rotations and residual estimates.
     void take_column(int j) override {
-        read_step_timing(j);
         const double2 *hj = Hh + (size_t)j * slot_len;
         bool conv_all = true, hold_next = true;
         est = 0.0;
