@@ -762,7 +762,9 @@ struct CellItems {
     DevBuf<int> ket_item_ptr;
     ItemsRef ref() const { return ItemsRef{n, items.p, ket_item_ptr.p}; }
     // runs of at most item_len cells of each group (host side): it[] and the per-group item
-    // pointers kip[]
+    // pointers kip[].  A group of n cells takes ceil(n / item_len) runs of as equal lengths as
+    // possible (lengths differ by at most one), so that no short remainder run stands beside
+    // full ones: the launch's workgroups do more equal work and its tail is shorter.
     static void runs(const CellView &P, hipStream_t s, int item_len, std::vector<int2> &it, std::vector<int> &kip) {
         TT_REQUIRE(item_len >= 1 && item_len <= 64, "work item length must be in [1, 64]");
         std::vector<int> kp(P.NqNp + 1);
@@ -771,7 +773,12 @@ struct CellItems {
         it.clear();
         kip.assign(P.NqNp + 1, 0);
         for (int k = 0; k < P.NqNp; ++k) {
-            for (int c = kp[k]; c < kp[k + 1]; c += item_len) it.push_back(make_int2(c, std::min(c + item_len, kp[k + 1])));
+            const int n = kp[k + 1] - kp[k], m = (n + item_len - 1) / item_len;
+            for (int i = 0, c = kp[k]; i < m; ++i) {
+                const int len = n / m + (i < n % m ? 1 : 0);
+                it.push_back(make_int2(c, c + len));
+                c += len;
+            }
             kip[k + 1] = (int)it.size();
         }
     }
