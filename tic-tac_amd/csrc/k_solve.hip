@@ -414,7 +414,23 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     int *s_alpha = reinterpret_cast<int *>(xs + ne_max * Np * kc);            // ne
     int *s_items = s_alpha + ne_max;                                          // Np + 1
     const CT *Bg = blocks + (size_t)e0 * NpNp;
-    for (int i = threadIdx.x; i < ne * NpNp; i += blockDim.x) Bs[i] = Bg[i];
+    {   // the partner blocks: 16-byte loads, four in flight per thread before their LDS stores
+        typedef typename std::conditional<sizeof(CT) == 8, double2, float4>::type V16;
+        constexpr int per = 16 / (int)sizeof(CT);
+        const int n = ne * NpNp;
+        const int n16 = ((uintptr_t)Bg % 16 == 0) ? n / per : 0;
+        const V16 *Bg16 = reinterpret_cast<const V16 *>(Bg);
+        V16 *Bs16 = reinterpret_cast<V16 *>(Bs);
+        for (int i0 = threadIdx.x; i0 < n16; i0 += 4 * blockDim.x) {
+            V16 t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) t[u] = Bg16[min(i0 + u * (int)blockDim.x, n16 - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * (int)blockDim.x < n16) Bs16[i0 + u * blockDim.x] = t[u];
+        }
+        for (int i = n16 * per + threadIdx.x; i < n; i += blockDim.x) Bs[i] = Bg[i];
+    }
     if (G)
         for (int i = threadIdx.x; i < nk; i += blockDim.x) gk[i] = gidx[k0 + i];
     for (int i = threadIdx.x; i < ne; i += blockDim.x) s_alpha[i] = part_alpha[e0 + i];
