@@ -437,48 +437,69 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     if (ket_item_ptr)
         for (int i = threadIdx.x; i <= Np; i += blockDim.x) s_items[i] = ket_item_ptr[q * Np + i];
     __syncthreads();
-    const int tile = Np * nk;
-    for (int idx = threadIdx.x; idx < ne * tile; idx += blockDim.x) {
-        const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
-        const int ae = s_alpha[e];
-        const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
-        double xr, xi;
-        if (ket_item_ptr) {
-            // the ket's item panels, four loads in flight at a time, added in item order
-            const int i0 = s_items[pp], i1 = s_items[pp + 1];
-            xr = 0.0;
-            xi = 0.0;
-            for (int it = i0; it < i1; it += 4) {
-                TI v[4];
+    const int tile = Np * nk, total = ne * tile;
+    // Two elements per thread and pass, their loads issued before either is used (the ket's
+    // first two item panels, or the input element, and G): one wait for memory per two
+    // elements; a ket's further item panels are added after, in item order.
+    for (int base = threadIdx.x; base < total; base += 2 * blockDim.x) {
+        TI v[2][2];
+        GT gv[2];
+        int i0s[2], i1s[2], aes[2], kks[2];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = X[((int64_t)min(it + u, i1 - 1) * Na + ae) * r + k];
+        for (int u = 0; u < 2; ++u) {
+            const int idx = min(base + u * (int)blockDim.x, total - 1);   // past total: loaded, not stored
+            const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
+            const int ae = s_alpha[e];
+            const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
+            aes[u] = ae;
+            kks[u] = kk;
+            if (ket_item_ptr) {
+                const int i0 = s_items[pp], i1 = s_items[pp + 1];
+                i0s[u] = i0;
+                i1s[u] = i1;
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (it + u < i1) {
-                        xr += (double)v[u].x;
-                        xi += (double)v[u].y;
-                    }
+                for (int w = 0; w < 2; ++w) v[u][w] = X[((int64_t)max(0, min(i0 + w, i1 - 1)) * Na + ae) * r + k];
+            } else {
+                const int64_t srcm = in_cell ? ((int64_t)q * Np + pp) * Na + ae : src;  // memory index of X
+                v[u][0] = X[srcm * r + k];
             }
-            xr *= in_scale;
-            xi *= in_scale;
-        } else {
-            const int64_t srcm = in_cell ? ((int64_t)q * Np + pp) * Na + ae : src;  // memory index of X
-            const TI v = X[srcm * r + k];
-            xr = (double)v.x;
-            xi = (double)v.y;
+            if (G) gv[u] = G[(int64_t)max(0, gk[kk]) * dim + src];
         }
-        if (G) {
-            const int g = gk[kk];
-            if (g >= 0) {
-                const GT gv0 = G[(int64_t)g * dim + src];
-                const double2 gv = make_double2((double)gv0.x, (double)gv0.y);
-                const double tr = xr * gv.x - xi * gv.y;
-                xi = xr * gv.y + xi * gv.x;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int idx = base + u * (int)blockDim.x;
+            if (idx >= total) continue;
+            double xr, xi;
+            if (ket_item_ptr) {
+                const int i0 = i0s[u], i1 = i1s[u];
+                xr = 0.0;
+                xi = 0.0;
+#pragma unroll
+                for (int w = 0; w < 2; ++w)
+                    if (i0 + w < i1) {
+                        xr += (double)v[u][w].x;
+                        xi += (double)v[u][w].y;
+                    }
+                for (int it = i0 + 2; it < i1; ++it) {
+                    const TI x = X[((int64_t)it * Na + aes[u]) * r + k0 + kks[u]];
+                    xr += (double)x.x;
+                    xi += (double)x.y;
+                }
+                xr *= in_scale;
+                xi *= in_scale;
+            } else {
+                xr = (double)v[u][0].x;
+                xi = (double)v[u][0].y;
+            }
+            if (G && gk[kks[u]] >= 0) {
+                const double2 g = make_double2((double)gv[u].x, (double)gv[u].y);
+                const double tr = xr * g.x - xi * g.y;
+                xi = xr * g.y + xi * g.x;
                 xr = tr;
             }
+            xs[idx].x = (CT)xr;
+            xs[idx].y = (CT)xi;
         }
-        xs[idx].x = (CT)xr;
-        xs[idx].y = (CT)xi;
     }
     __syncthreads();
     // Thread (g, kk) forms the outputs (p, kk) of the bp consecutive rows p = g bp .. g bp + bp - 1,
