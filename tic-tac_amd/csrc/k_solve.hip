@@ -438,15 +438,17 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         for (int i = threadIdx.x; i <= Np; i += blockDim.x) s_items[i] = ket_item_ptr[q * Np + i];
     __syncthreads();
     const int tile = Np * nk, total = ne * tile;
-    // Two elements per thread and pass, their loads issued before either is used (the ket's
-    // first two item panels, or the input element, and G): one wait for memory per two
-    // elements; a ket's further item panels are added after, in item order.
-    for (int base = threadIdx.x; base < total; base += 2 * blockDim.x) {
-        TI v[2][2];
-        GT gv[2];
-        int i0s[2], i1s[2], aes[2], kks[2];
+    // EU elements per thread and pass, their loads issued before any is used (the ket's first
+    // two item panels, or the input element, and G): one wait for memory per EU elements; a
+    // ket's further item panels are added after, in item order.  (EU = 4 was slower: 2858-2933
+    // against 2953-2985 channel-solves/s on the batch, its registers cutting the occupancy.)
+    constexpr int EU = 2;
+    for (int base = threadIdx.x; base < total; base += EU * blockDim.x) {
+        TI v[EU][2];
+        GT gv[EU];
+        int i0s[EU], i1s[EU], aes[EU], kks[EU];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < EU; ++u) {
             const int idx = min(base + u * (int)blockDim.x, total - 1);   // past total: loaded, not stored
             const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
             const int ae = s_alpha[e];
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
             if (G) gv[u] = G[(int64_t)max(0, gk[kk]) * dim + src];
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < EU; ++u) {
             const int idx = base + u * (int)blockDim.x;
             if (idx >= total) continue;
             double xr, xi;
