@@ -31,9 +31,9 @@ import tictac as tt  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # dense MFMA peaks of MI355X
-# streams a batched lockstep group is spread over: the library reads TT_LANES (1..16), else 8
+# streams a batched lockstep group is spread over: the library reads TT_LANES (1..16), else 6
 LANES = int(os.environ["TT_LANES"]) if os.environ.get("TT_LANES", "").isdigit() and \
-    1 <= int(os.environ["TT_LANES"]) <= 16 else 8
+    1 <= int(os.environ["TT_LANES"]) <= 16 else 6
 METRIC = "U-matrix channel-solves/s batched-mixed vs plain FP64 @|ΔU/U|≤1e-8; grad/fwd time"
 SOLVE_TOL = 1e-12
 BATCH_T = [float(t) for t in np.linspace(1.0, 200.0, 8)]

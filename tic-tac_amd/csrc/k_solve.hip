@@ -2787,14 +2787,15 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
 // independent channels' kernels overlap on the device.  Every lane first waits for the work
 // queued on the context stream (G, the set-up); afterwards the lanes are drained and the plans
 // go back to the context stream, also when the solve throws.
-// Lanes: TT_LANES (1..16) if set, else 8.  With HIP's default of four hardware queues per
-// process, eight streams share them; measured on the 18-channel batch, 8 lanes gave 2.7% more
-// than 4 (16 the same as 8, and 8 lanes on 8 queues less than 4 on 4).
+// Lanes: TT_LANES (1..16) if set, else 6.  With HIP's default of four hardware queues per
+// process, the streams share them; measured on the 18-channel batch with plain launches, 8 lanes
+// gave 2.7% more than 4 (16 the same as 8, and 8 lanes on 8 queues less than 4 on 4); with the
+// replayed step graphs, 6 lanes gave 1.5-3% more than 5, 7 or 8 (alternating runs on one box).
 static int lanes_wanted() {
     static const int n = [] {
         const char *e = std::getenv("TT_LANES");
         const int v = e ? std::atoi(e) : 0;
-        return v >= 1 && v <= 16 ? v : 8;
+        return v >= 1 && v <= 16 ? v : 6;
     }();
     return n;
 }
