@@ -126,3 +126,51 @@ def test_cli_parameter_walk_matches_the_binding(ctx, tmp_path):
         got = np.array([u for _, _, _, u in rows]).T.reshape(ch.nd, ch.nd, ch.nE)
         assert np.abs(got - U).max() <= 1e-10 * np.abs(U).max()
     assert len(os.listdir(out)) == 2   # parallel_run: channel 0 only, two parameter sets
+
+
+@pytest.mark.gpu
+def test_cli_p123_files_store_and_read(ctx, tmp_path):
+    """calculate_and_store_P123: a precalculation run (solve_faddeev=false) leaves the channels' P123
+    files in P123_folder; a run with calculate_and_store_P123=false reads them and writes the same U
+    as a run that builds P123 itself (CPP/make_permutation_matrix.cpp:1297-1403); a file of other
+    bin edges is refused with the reference's message."""
+    from tictac import h5
+    N = 10
+    (tmp_path / "e.txt").write_text("13.0\n")
+    base = dict(potential_model="malfliet_tjon", Np_WP=N, Nq_WP=N, J_2N_max=1, tensor_force="false",
+                isospin_breaking_1S0="false", energy_input_file=tmp_path / "e.txt", solve_method="gmres")
+    print("tictac_run MT N=10: P123 precalculation", flush=True)
+    r = _run([_input(tmp_path, solve_faddeev="false", P123_folder=tmp_path / "p123", **base)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    prob = tt.Problem(_mt_rp(N), tt.Potential("malfliet_tjon"))
+    files = sorted(os.listdir(tmp_path / "p123"))
+    assert files == sorted(os.path.basename(h5.p123_filename(".", prob.channel(c, [13.0])))
+                           for c in range(prob.n_channels))
+    assert not os.path.exists(tmp_path / "Output")   # nothing solved, nothing written
+    print("tictac_run MT N=10: read P123 files", flush=True)
+    r = _run([_input(tmp_path, calculate_and_store_P123="false", P123_folder=tmp_path / "p123",
+                     output_folder=tmp_path / "read", **base)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    print("tictac_run MT N=10: build P123", flush=True)
+    r = _run([_input(tmp_path, P123_folder=tmp_path / "p123b", output_folder=tmp_path / "built", **base)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    names = sorted(os.listdir(tmp_path / "built"))
+    assert names == sorted(os.listdir(tmp_path / "read")) and len(names) == prob.n_channels
+    for name in names:
+        _, rows_b = _read_elastic(tmp_path / "built" / name)
+        _, rows_r = _read_elastic(tmp_path / "read" / name)
+        ub = np.array([u for _, _, _, u in rows_b])
+        ur = np.array([u for _, _, _, u in rows_r])
+        assert np.abs(ub - ur).max() <= 1e-10 * np.abs(ub).max()
+    # the stored files equal the P123 the binding builds
+    for c in range(prob.n_channels):
+        ch = prob.channel(c, [13.0])
+        val, row, ptr = ch.build_p123(ctx).to_host()
+        v, rr, cc = h5.read_p123(h5.p123_filename(str(tmp_path / "p123"), ch), ch)
+        for a, b in zip(h5.csc_to_coo(ch.dim, val, row, ptr), (v, rr, cc)):
+            np.testing.assert_array_equal(a, b)
+    # other Chebyshev scale, same file name: the bin edges differ
+    print("tictac_run MT N=10: mismatched file", flush=True)
+    r = _run([_input(tmp_path, calculate_and_store_P123="false", P123_folder=tmp_path / "p123",
+                     chebyshev_s=120.0, output_folder=tmp_path / "bad", **base)], tmp_path)
+    assert r.returncode == 1 and "File-read P123 p-momentum boundaries mismatch." in r.stderr

@@ -6,12 +6,16 @@
 //   U_PW_elements*.txt               CPP/disk_io_routines.cpp:759-913 (store_U_matrix_elements_txt)
 //   U_PW_breakup_elements*.txt       CPP/disk_io_routines.cpp:341-565,566-706
 //
+//   P123 files in P123_folder        CPP/make_permutation_matrix.cpp:1297-1403 (libtictac_h5)
+//
 // The same key=value options (command line or <file>.txt, the last value read wins), the same
-// loops and the same output files.  P123 is built on the GPU for every channel, G is computed
-// on the device inside the solve.  Differences: errors exit with status 1 (the reference exits
-// with 0); the potential models are those of the library (malfliet_tjon, synthetic, the latter
-// with its 15 parameters from parameter_file); the P123 HDF5 cache options are accepted and
-// ignored; one option is added, solve_method=pade|dense|gmres|mixed (solve_tol for the last two).
+// loops and the same output files.  With calculate_and_store_P123=true P123 is built on the GPU
+// (or, with P123_recovery=true, merged from the thread spill files of P123_omp_num_threads
+// threads) and stored in the reference's HDF5 file; with false it is read from that file.  G is
+// computed on the device inside the solve.  Differences: errors exit with status 1 (the
+// reference exits with 0); the potential models are those of the library (malfliet_tjon,
+// synthetic, the latter with its 15 parameters from parameter_file); one option is added,
+// solve_method=pade|dense|gmres|mixed (solve_tol for the last two).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -28,9 +32,11 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tictac.h"
+#include "tictac_h5.h"
 
 namespace {
 
@@ -45,11 +51,23 @@ void check(tt_status s) {
     if (s != TT_OK) die(tt_last_error());
 }
 
+void check_h5(tt_status s) {
+    if (s != TT_OK) die(tt_h5_last_error());
+}
+
+// omp_get_max_threads() of the reference's default P123_omp_num_threads: OMP_NUM_THREADS when
+// set, else the host's hardware threads
+int max_threads() {
+    const char *env = std::getenv("OMP_NUM_THREADS");
+    const int n = env ? std::atoi(env) : (int)std::thread::hardware_concurrency();
+    return n > 0 ? n : 1;
+}
+
 // run_params of CPP/type_defs.h:87-123 with the defaults of set_run_parameters.cpp:577-615
 struct Options {
     int two_J_3N_max = 1, Np_WP = 50, Nq_WP = 50, J_2N_max = 3, Nphi = 48, Nx = 48;
     double chebyshev_t = 1, chebyshev_s = 100;
-    int Np_per_WP = 8, Nq_per_WP = 8, channel_idx = -1, P123_omp_num_threads = 1;
+    int Np_per_WP = 8, Nq_per_WP = 8, channel_idx = -1, P123_omp_num_threads = max_threads();
     int PSI_start = -1, PSI_end = -1;
     bool parallel_run = false, P123_recovery = false, tensor_force = true, isospin_breaking_1S0 = true;
     bool midpoint_approx = false, calculate_and_store_P123 = true, include_breakup_channels = false;
@@ -146,8 +164,8 @@ void show_usage() {
                  "  tensor_force isospin_breaking_1S0 midpoint_approx production_run include_breakup_channels\n"
                  "  solve_faddeev solve_dense parallel_run channel_idx potential_model (malfliet_tjon, synthetic)\n"
                  "  parameter_walk parameter_file PSI_start PSI_end energy_input_file output_folder\n"
-                 "  accepted and ignored: P123_omp_num_threads P123_recovery calculate_and_store_P123 P123_folder\n"
-                 "                        subfolder; p_grid_type/q_grid_type must be chebyshev\n"
+                 "  calculate_and_store_P123 P123_folder P123_recovery P123_omp_num_threads (the HDF5 P123 files)\n"
+                 "  accepted and ignored: subfolder; p_grid_type/q_grid_type must be chebyshev\n"
                  "Added: solve_method=pade|dense|gmres|mixed (default pade, as the reference), solve_tol=1e-12\n"
               << std::endl;
 }
@@ -502,9 +520,13 @@ int main(int argc, char **argv) {
         const int two_J = view.two_J_3N[0], P3 = view.P_3N[0];
         std::printf("Working on 3N-channel J_3N=%d/2, PAR=%d (channel %d of %d) with %d partial-wave states \n", two_J,
                     P3, chn + 1, pw.N_chn_3N, view.Nalpha);
+        // fill_P123_arrays with its file cache (make_permutation_matrix.cpp:1297-1403)
         tt_csc P123;
-        check(tt_fill_P123_arrays(ctx, &rp, &fwp, &view, o.production_run, &P123, nullptr));
-        if (!o.solve_faddeev) {   // P123 is not stored: no HDF5 cache in this build
+        if (o.calculate_and_store_P123) make_dir(o.P123_folder);
+        check_h5(tt_fill_P123_arrays_h5(ctx, &rp, &fwp, &view, o.production_run, o.P123_folder.c_str(),
+                                        o.calculate_and_store_P123, o.P123_recovery, o.P123_omp_num_threads, &P123,
+                                        nullptr));
+        if (!o.solve_faddeev) {   // the P123 precalculation run: the file is all it leaves
             tt_csc_free(&P123);
             continue;
         }
