@@ -10,6 +10,8 @@ Tolerances (floating point; the reference itself is FP64):
   Pade                 Neumann terms 1e-10; U within 1e-5 of the converged solve, the accuracy of
                        the reference's own stopping rule (1e-6 relative / 1e-7 absolute)
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse.linalg as sla
@@ -671,3 +673,44 @@ def test_channel_above_64_alphas_takes_the_csc_gather(ctx):
     with pytest.raises(tt.TictacError) as e:
         prob.gradient(ctx, c, P, tt.synthetic_params()[:12], T, basis=prob.basis_matrices(12))
     assert e.value.code == tt._lib.TT_ERR_UNSUPPORTED
+
+
+@pytest.fixture
+def split_half_product():
+    """Forces the split-half FP16 product for the FP32 applications of a fresh context (the
+    autotuner reads TT_CELL_H when it first meets a shape)."""
+    old = os.environ.get("TT_CELL_H")
+    os.environ["TT_CELL_H"] = "1"
+    ctx2 = tt.Context(0)
+    yield ctx2
+    del ctx2
+    if old is None:
+        del os.environ["TT_CELL_H"]
+    else:
+        os.environ["TT_CELL_H"] = old
+
+
+def test_split_half_product_against_oracle(ctx, c2, c3, split_half_product):
+    """The FP32 product on the FP16 MFMA (every operand as a high and a low half) at the one-,
+    two-, three- and four-M-tile shapes: N_alpha 27 (C2), 47, and 63 (a 7/2+ channel at
+    Np = Nq = 8): rows against the oracle's row application to 1e-5 like the FP32 MFMA product,
+    and the mixed solve against FP64 GMRES to 1e-10."""
+    ctx2 = split_half_product
+    rp, pot, T = synthetic_config(N=8, two_J_3N_max=7, T_lab=(10.0, 50.0))
+    prob = tt.Problem(rp, pot)
+    big = max(range(prob.n_channels), key=lambda i: prob.channel(i, T).Nalpha)
+    c4 = _setup(ctx, rp, pot, T, channel=big)
+    assert c4.ch.Nalpha > 48
+    rng = np.random.default_rng(9)
+    for c in (c2, c3, c4):
+        r = 18
+        X = rng.normal(size=(r, c.ch.dim)) + 1j * rng.normal(size=(r, c.ch.dim))
+        g = [k % c.ch.nE for k in range(r)]
+        Yo = c.K.row_apply(X * np.stack([c.G[k] for k in g]))
+        Y32 = c.ch.apply_rows(ctx2, c.P, X, G=c.G, g_index=g, fp32=True)
+        assert np.abs(Y32 - Yo).max() <= 1e-5 * np.abs(Yo).max(), c.ch.Nalpha
+        U64, _ = c.ch.solve(ctx2, c.P, c.G, method=tt.SOLVE_GMRES)
+        Umx, st = c.ch.solve(ctx2, c.P, c.G, method=tt.SOLVE_MIXED)
+        assert st.n_apply_fp32 > 0
+        assert np.abs(Umx - U64).max() <= 1e-10 * np.abs(U64).max(), c.ch.Nalpha
+    assert any("fp16-split" in ln for ln in tt.cell_tuning())

@@ -73,6 +73,8 @@ DeviceCSC::~DeviceCSC() {
     if (t_partner) (void)hipFree(t_partner);
     if (t_blk) (void)hipFree(t_blk);
     if (t_blk32) (void)hipFree(t_blk32);
+    if (blkh) (void)hipFree(blkh);
+    if (t_blkh) (void)hipFree(t_blkh);
 }
 
 // tblk[c'][i][j] = blk[src[c']][j][i]
@@ -334,6 +336,9 @@ struct Mfma16<float> {
 // solves, whose blocks are then FP32 copies: half the LDS, so more workgroups per CU hide the
 // op's load latency).  GT: the type of G (double2; float2, an FP32 copy of G, for the FP32
 // vectors, whose tile is rounded to FP32 anyway: half the bytes of the op's largest input).
+// words over which a block op spreads its max |output| atomics (the split-half product's scale)
+constexpr int AMAX_SLOTS = 32;
+
 template <typename CT>
 struct Cplx;
 template <>
@@ -351,7 +356,7 @@ template <int BP, typename CT, typename CT2, typename TO>
 __device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 *__restrict__ xs, int ne, int Np,
                                            int nk, int tile, int kk, int p0, TO *__restrict__ Y,
                                            const TO *__restrict__ Zsub, int out_cell, int a, int q, int Na, int Nq,
-                                           int r, int k) {
+                                           int r, int k, float &vmax) {
     const int NpNp = Np * Np;
     CT sr[BP], si[BP];
     int prow[BP];
@@ -386,6 +391,7 @@ __device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 
         }
         Y[dst].x = vr;
         Y[dst].y = vi;
+        vmax = fmaxf(vmax, fmaxf(fabsf((float)vr), fabsf((float)vi)));
     }
 }
 
@@ -396,7 +402,8 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
                                                const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
                                                int ne_max, const TO *__restrict__ Zsub, int in_cell,
                                                int out_cell, const int *__restrict__ ket_item_ptr,
-                                               double in_scale, double2 *__restrict__ zero_buf, int zero_n) {
+                                               double in_scale, double2 *__restrict__ zero_buf, int zero_n,
+                                               unsigned *__restrict__ amax) {
     extern __shared__ double smem[];
     __shared__ int gk[256];   // gidx of this chunk's columns (kc <= 256)
     if (zero_buf && blockIdx.x == 0 && blockIdx.y == 0)   // a buffer a later kernel accumulates into
@@ -512,13 +519,25 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
     // (with more than 4 npg rows, groups take further runs of 4 rows, npg runs apart)
     const int npg = blockDim.x / nk, bp = min(4, (Np + npg - 1) / npg);
     const int g = threadIdx.x / nk, kk = threadIdx.x - g * nk;
+    float vmax = 0.0f;   // largest magnitude this thread writes (NaNs are not counted)
     if (g < npg)
         for (int p0 = g * bp; p0 < Np; p0 += npg * bp) switch (bp) {
-                case 1: block_rows<1>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
-                case 2: block_rows<2>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
-                case 3: block_rows<3>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
-                default: block_rows<4>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk); break;
+                case 1: block_rows<1>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk, vmax); break;
+                case 2: block_rows<2>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk, vmax); break;
+                case 3: block_rows<3>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk, vmax); break;
+                default: block_rows<4>(Bs, xs, ne, Np, nk, tile, kk, p0, Y, Zsub, out_cell, a, q, Na, Nq, r, k0 + kk, vmax); break;
             }
+    if (amax) {   // the output's largest magnitude, for the FP16 split of the product that follows
+        __shared__ float wmax[4];
+        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;
+        __syncthreads();
+        // one atomic per workgroup, spread over AMAX_SLOTS words (a single word serialises the
+        // launch's atomics); non-negative floats order as their bit patterns do
+        if (threadIdx.x == 0)
+            atomicMax(amax + (blockIdx.x + blockIdx.y) % AMAX_SLOTS,
+                      __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+    }
 }
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
@@ -533,7 +552,7 @@ static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr,
                          const CT *blocks, int ne_max, const typename NoDeduce<GT>::type *G, const int *gidx, int Na,
                          int Nq, int Np,
                          int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
-                         double in_scale, double2 *zero_buf = nullptr, int zero_n = 0) {
+                         double in_scale, double2 *zero_buf = nullptr, int zero_n = 0, unsigned *amax = nullptr) {
     const int nch = (r + 23) / 24;
     const int kc = (r + nch - 1) / nch;
     const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(CT) +
@@ -542,7 +561,8 @@ static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr,
     TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
     const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
     hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
-                       Nq, Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n);
+                       Nq, Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n,
+                       amax);
 }
 
 // ------------------------------------------------------------------ gather SpMM
@@ -649,7 +669,7 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
                                                   const T *__restrict__ B, const T *__restrict__ S,
                                                   T *__restrict__ Upart, int Na, int R2) {
     typedef typename Mfma16<T>::acc_t acc_t;
-    __shared__ acc_t red[3][64];
+    __shared__ acc_t red[3][MT * NT][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so workgroup b runs on
@@ -728,24 +748,27 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
         mma_batch(base + stride, a1, b1);
         base += 2 * stride;
     }
-    // waves 1-3 hand their accumulators to wave 0 one tile at a time (fixed order
-    // ((w0 + w1) + w2) + w3), so the buffer does not grow with MT*NT
+    // waves 1-3 leave all their tiles in LDS at once (one barrier), wave 0 adds them in a fixed
+    // order ((w0 + w1) + w2) + w3 and stores
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            if (wave > 0) red[wave - 1][m * NT + n][lane] = acc[m][n];
+    __syncthreads();
+    if (wave != 0) return;
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
-            if (wave > 0) red[wave - 1][lane] = acc[m][n];
-            __syncthreads();
-            if (wave == 0) {
-                const acc_t t = ((acc[m][n] + red[0][lane]) + red[1][lane]) + red[2][lane];
-                const int v = v0 + 16 * n + il;
+            const int tt = m * NT + n;
+            const acc_t t = ((acc[m][n] + red[0][tt][lane]) + red[1][tt][lane]) + red[2][tt][lane];
+            const int v = v0 + 16 * n + il;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int a = MT * Mfma16<T>::row(lane, r) + m;
-                    if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = t[r];
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int a = MT * Mfma16<T>::row(lane, r) + m;
+                if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = t[r];
             }
-            __syncthreads();
         }
 }
 
@@ -761,6 +784,232 @@ __global__ void k_cell_reduce(const int *__restrict__ ket_item_ptr, const T *__r
     T acc = 0;
     for (int i = ket_item_ptr[ket]; i < ket_item_ptr[ket + 1]; ++i) acc += Upart[(int64_t)i * per + o];
     U[t] = scale * acc;
+}
+
+// ------------------------------------------------------------------ split-half cell product
+// The FP32 product on the FP16 MFMA.  Every operand x (a block value, or an element of S) is
+// held as two halves, hi = fp16(x) and lo = fp16(x - hi), after scaling by a power of two that
+// puts the operand's largest magnitude at or below 2^14 (so neither half overflows and the low
+// halves stay normal for all but tiny elements).  hi + lo carries 22 significant bits, and
+//   A B ~ Ahi Bhi + Alo Bhi + Ahi Blo
+// (the dropped Alo Blo is below 2^-22 of |A B|) with the products exact and the sums in FP32,
+// so the result matches the FP32 product to its own accumulation error.  The 16x16x32 FP16
+// MFMA takes 16 cycles for 32 K entries where the FP32 16x16x4 takes 32 cycles for 4: three of
+// them per tile cost a fifth of the FP32 product's matrix time, so the product is left bound
+// by the block bytes it streams (4 bytes per value, as in FP32).
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+struct __align__(16) HalfPair {   // the split halves of 8 consecutive K entries (32 bytes)
+    half8_t hi, lo;
+};
+constexpr int SPLIT_HEADROOM_LOG2 = 14;   // largest scaled magnitude: 2^14 (fp16 max: 65504)
+
+__device__ __forceinline__ void split8(const float (&x)[8], HalfPair &hp) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const _Float16 h = (_Float16)x[j];
+        hp.hi[j] = h;
+        hp.lo[j] = (_Float16)(x[j] - (float)h);   // x - h is exact in FP32
+    }
+}
+
+// 2^e with m * 2^e <= 2^SPLIT_HEADROOM_LOG2 (1 for m = 0 or not finite)
+__host__ __device__ inline float split_scale(float m) {
+    if (!(m > 0.0f) || !(m <= 3.0e38f)) return 1.0f;
+    int e = 0;
+    (void)frexpf(m, &e);   // m = f 2^e, 0.5 <= f < 1
+    return ldexpf(1.0f, SPLIT_HEADROOM_LOG2 - e);
+}
+
+__global__ void k_fill_u32(unsigned *__restrict__ p, int n, unsigned v) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// out = bits of max |x| over n values (atomicMax of non-negative floats' bit patterns)
+__global__ void k_absmax_f64(const double *__restrict__ x, int64_t n, unsigned *__restrict__ out) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, (float)fabs(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+// Split-half blocks: out[(c * Na + a) * Kp8 + k8] = halves of scale * blk[c][8 k8 + j][a], j < 8
+// (zero past Na).  blk rows are input alphas, columns output alphas.
+__global__ void k_blk_split(const double *__restrict__ blk, HalfPair *__restrict__ out, int64_t ncell, int Na,
+                            int Kp8, double scale) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * Na * Kp8) return;
+    const int k8 = (int)(t % Kp8);
+    const int64_t ca = t / Kp8;
+    const int a = (int)(ca % Na);
+    const int64_t c = ca / Na;
+    HalfPair hp;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int ap = 8 * k8 + j;
+        const double x = ap < Na ? blk[(c * Na + ap) * Na + a] * scale : 0.0;
+        const _Float16 h = (_Float16)(float)x;
+        hp.hi[j] = h;
+        hp.lo[j] = (_Float16)(float)(x - (double)h);
+    }
+    out[t] = hp;
+}
+
+// Split-half panels of S: St[(bra * R2 + v) * Kp8 + k8] = halves of sc * S[bra][8 k8 + j][v]
+// (zero past Na), sc = split_scale(max |S|) from the block op that wrote S; sc is also left in
+// *s_scale for the product.  One workgroup per bra bin: its Na x R2 panel goes through LDS
+// (rows padded by one word against bank conflicts) and leaves transposed.
+__global__ __launch_bounds__(256) void k_split_s(const float *__restrict__ S, HalfPair *__restrict__ St,
+                                                 const unsigned *__restrict__ amax, float *__restrict__ s_scale,
+                                                 int Na, int R2, int Kp8) {
+    extern __shared__ float panel[];   // Na * (R2 + 1)
+    const int bra = blockIdx.x, ld = R2 + 1;
+    unsigned mbits = 0;
+    for (int i = 0; i < AMAX_SLOTS; ++i) mbits = max(mbits, amax[i]);
+    const float sc = split_scale(__uint_as_float(mbits));
+    if (bra == 0 && threadIdx.x == 0) *s_scale = sc;
+    const float *src = S + (int64_t)bra * Na * R2;
+    for (int t = threadIdx.x; t < Na * R2; t += blockDim.x) {
+        const int ap = t / R2;
+        panel[ap * ld + (t - ap * R2)] = src[t];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R2 * Kp8; t += blockDim.x) {
+        const int v = t / Kp8, k8 = t - v * Kp8;
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int ap = 8 * k8 + j;
+            x[j] = ap < Na ? panel[ap * ld + v] * sc : 0.0f;
+        }
+        HalfPair hp;
+        split8(x, hp);
+        St[((int64_t)bra * R2 + v) * Kp8 + k8] = hp;
+    }
+}
+
+// U[ket][a][v] partial panels of the work items, as k_cellmfma, from the split halves:
+// Upart = (sum_K A B) / (p_scale * s_scale).  M = a in MT tiles of 16 (a = 16 m + row), N = v in
+// NT tiles of 16, K = the item's cells' input alphas, 8-entry chunks (Kp8 per cell) of which
+// lane group g = lane / 16 takes chunk 4 s + g of step s (32 entries): its A operand is the 8
+// entries of block column a = 16 m + lane % 16, its B operand those of column v of the bra's
+// panel, both one 32-byte load (hi, lo) from a layout made for it.  Steps are dealt to the four
+// waves in batches of UN; two register buffers alternate, loads of the next batch in flight
+// during the MFMAs of the current one, as in k_cellmfma.  Chunks past the item's end are
+// loaded from the clamped last chunk and zeroed in the B operand.  The first workgroup clears
+// the AMAX_SLOTS words at amax_reset, the block op's maximum that k_split_s has consumed, for
+// the next application.
+template <int MT, int NT, int UN>
+__global__ __launch_bounds__(256) void k_cellh(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
+                                               const HalfPair *__restrict__ A, const HalfPair *__restrict__ St,
+                                               const float *__restrict__ s_scale, float p_scale,
+                                               unsigned *__restrict__ amax_reset, float *__restrict__ Upart, int Na,
+                                               int R2, int Kp8) {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __shared__ acc_t red[3][MT * NT][64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, il = lane & 15;
+    if (amax_reset && blockIdx.x == 0 && blockIdx.y == 0 && tid < AMAX_SLOTS) amax_reset[tid] = 0u;
+    // XCD-aware item order, as k_cellmfma
+    const int nit = gridDim.x, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+    const int full = nit / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    const int item = (int)blockIdx.x < full ? ((w / XCD_CHUNK) * 8 + xcd) * XCD_CHUNK + w % XCD_CHUNK : (int)blockIdx.x;
+    const int v0 = blockIdx.y * 16 * NT;
+    const int c_beg = items[item].x, ncell = items[item].y - c_beg;
+    const int nch = ncell * Kp8;                       // 8-entry chunks of K
+    const int nb = ((nch + 3) / 4 + UN - 1) / UN;      // batches of UN steps of 4 chunks
+    const int my_bra = lane < ncell ? cell_bra[c_beg + lane] : 0;
+    const float rKp8 = 1.0f / (float)Kp8;
+    acc_t acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = acc_t{0, 0, 0, 0};
+    int arow[MT], bcol[NT];   // clamped block columns and panel columns of this lane
+#pragma unroll
+    for (int m = 0; m < MT; ++m) arow[m] = min(16 * m + il, Na - 1) * Kp8;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bcol[n] = min(v0 + 16 * n + il, R2 - 1) * Kp8;
+
+    HalfPair a0[UN][MT], b0[UN][NT], a1[UN][MT], b1[UN][NT];
+    auto load_batch = [&](int b, HalfPair (&a)[UN][MT], HalfPair (&bb)[UN][NT]) {
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const int ch = min((b * UN + u) * 4 + g, nch - 1);
+            const int c = (int)(((float)ch + 0.5f) * rKp8);   // exact: ch < 4096, Kp8 <= 8
+            const int k8 = ch - c * Kp8;
+            const int bra = __shfl(my_bra, c);
+            const HalfPair *ab = A + (int64_t)(c_beg + c) * Na * Kp8 + k8;
+            const HalfPair *sb = St + (int64_t)bra * R2 * Kp8 + k8;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) a[u][m] = ab[arow[m]];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) bb[u][n] = sb[bcol[n]];
+        }
+    };
+    auto mma_batch = [&](int b, const HalfPair (&a)[UN][MT], const HalfPair (&bb)[UN][NT]) {
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const bool ok = (b * UN + u) * 4 + g < nch;
+            half8_t bh[NT], bl[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                bh[n] = ok ? bb[u][n].hi : half8_t{};
+                bl[n] = ok ? bb[u][n].lo : half8_t{};
+            }
+            // the three terms in turn over all tiles: no MFMA waits on the one just issued
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[u][m].hi, bh[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[u][m].lo, bh[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[u][m].hi, bl[n], acc[m][n], 0, 0, 0);
+        }
+    };
+    int b = wave;
+    const int nbw = b < nb ? (nb - b + 3) / 4 : 0;
+    load_batch(b, a0, b0);
+    for (int i = 0; i < nbw; i += 2) {
+        load_batch(b + 4, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_batch(b, a0, b0);
+        load_batch(b + 8, a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_batch(b + 4, a1, b1);
+        b += 8;
+    }
+    // waves 1-3 leave all their tiles in LDS at once (one barrier), wave 0 adds them in a fixed
+    // order ((w0 + w1) + w2) + w3 and stores
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            if (wave > 0) red[wave - 1][m * NT + n][lane] = acc[m][n];
+    __syncthreads();
+    if (wave != 0) return;
+    const float inv = 1.0f / (p_scale * *s_scale);   // powers of two: exact
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int t = m * NT + n;
+            const acc_t sum = ((acc[m][n] + red[0][t][lane]) + red[1][t][lane]) + red[2][t][lane];
+            const int v = v0 + 16 * n + il;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = 16 * m + 4 * g + r;
+                if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = sum[r] * inv;
+            }
+        }
 }
 
 // One orientation of the cell-block form: cells grouped by output bin (kets for the product
@@ -942,6 +1191,56 @@ static void launch_cellmm(hipStream_t s, const CellView *P, const ItemsRef &ci, 
         hipLaunchKernelGGL((k_cell_reduce<T>), dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s,
                            ci.ket_item_ptr, Upart, U, P->NqNp, P->Na, R2, scale);
     }
+}
+
+// The split-half product over the split blocks Ah and panels St (k_split_s), un = 1 or 2 steps
+// per batch; Upart as launch_cellmm's.
+template <int MT, int NT>
+static void launch_cellh_un(hipStream_t s, const CellView *P, const ItemsRef &ci, const HalfPair *Ah,
+                            const HalfPair *St, const float *s_scale, float p_scale, unsigned *amax_reset,
+                            float *Upart, int R2, int un) {
+    const int Kp8 = (P->Na + 7) / 8;
+    const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
+    if (un == 2)
+        hipLaunchKernelGGL((k_cellh<MT, NT, 2>), grid, block, 0, s, ci.items, P->partner, Ah, St, s_scale, p_scale,
+                           amax_reset, Upart, P->Na, R2, Kp8);
+    else
+        hipLaunchKernelGGL((k_cellh<MT, NT, 1>), grid, block, 0, s, ci.items, P->partner, Ah, St, s_scale, p_scale,
+                           amax_reset, Upart, P->Na, R2, Kp8);
+}
+template <int MT>
+static void launch_cellh_nt(hipStream_t s, const CellView *P, const ItemsRef &ci, const HalfPair *Ah,
+                            const HalfPair *St, const float *s_scale, float p_scale, unsigned *amax_reset,
+                            float *Upart, int R2, int un) {
+    switch (cell_nt(R2)) {
+        case 1: launch_cellh_un<MT, 1>(s, P, ci, Ah, St, s_scale, p_scale, amax_reset, Upart, R2, un); break;
+        case 2: launch_cellh_un<MT, 2>(s, P, ci, Ah, St, s_scale, p_scale, amax_reset, Upart, R2, un); break;
+        case 3: launch_cellh_un<MT, 3>(s, P, ci, Ah, St, s_scale, p_scale, amax_reset, Upart, R2, un); break;
+        default: launch_cellh_un<MT, 4>(s, P, ci, Ah, St, s_scale, p_scale, amax_reset, Upart, R2, un); break;
+    }
+}
+// S (FP32, cell-major [bra][alpha][v], written by a block op that left max |S| in *amax) ->
+// split panels St -> the work items' partial panels Upart of 2 P^T S / 2 (the factor 2 is the
+// V.C op's, as for launch_cellmm).
+static void launch_cellh(hipStream_t s, const CellView *P, const ItemsRef &ci, const HalfPair *Ah, float p_scale,
+                         const float *S, HalfPair *St, float *s_scale, unsigned *amax, float *Upart, int R2, int un) {
+    TT_REQUIRE(P->Na >= 1 && P->Na <= CELL_NA_MAX, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
+    const int Kp8 = (P->Na + 7) / 8;
+    const size_t lds = sizeof(float) * (size_t)P->Na * (R2 + 1);
+    TT_REQUIRE(lds <= 64 * 1024, "split-half panel exceeds LDS");
+    hipLaunchKernelGGL(k_split_s, dim3((unsigned)P->NqNp), dim3(256), lds, s, S, St, amax, s_scale, P->Na, R2, Kp8);
+    if (ci.n <= 0) {   // nothing reads the maximum: clear it for the next application here
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, amax, AMAX_SLOTS, 0u);
+        return;
+    }
+    if (P->Na <= 16)
+        launch_cellh_nt<1>(s, P, ci, Ah, St, s_scale, p_scale, amax, Upart, R2, un);
+    else if (P->Na <= 32)
+        launch_cellh_nt<2>(s, P, ci, Ah, St, s_scale, p_scale, amax, Upart, R2, un);
+    else if (P->Na <= 48)
+        launch_cellh_nt<3>(s, P, ci, Ah, St, s_scale, p_scale, amax, Upart, R2, un);
+    else
+        launch_cellh_nt<4>(s, P, ci, Ah, St, s_scale, p_scale, amax, Upart, R2, un);
 }
 
 // Launch shape of the gather SpMM for R2 real components: NG value groups (a power of two)
@@ -1256,14 +1555,28 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 struct TuneKey {
     int dev, col, fp32, Na, R2;
     int64_t ncell;
+    int hmode;
     bool operator<(const TuneKey &o) const {
-        return std::tie(dev, col, fp32, Na, R2, ncell) < std::tie(o.dev, o.col, o.fp32, o.Na, o.R2, o.ncell);
+        return std::tie(dev, col, fp32, Na, R2, ncell, hmode) <
+               std::tie(o.dev, o.col, o.fp32, o.Na, o.R2, o.ncell, o.hmode);
     }
 };
 struct TuneEntry {
-    int len, un;
+    int len, un, h;
     float ms;
 };
+// Which FP32 cell product the autotuner may pick: the FP32 MFMA one (default, TT_CELL_H unset
+// or 0), only the split-half FP16 one (TT_CELL_H=1), or whichever times faster
+// (TT_CELL_H=auto).  On the 18-channel batch the two measure the same (2992-3020 against
+// 3023-3025 channel-solves/s, alternating runs on one box), so the default keeps the product
+// that needs no split kernel and no split copy of the blocks.
+static int cell_h_mode() {
+    const char *e = std::getenv("TT_CELL_H");
+    if (!e) return 0;
+    if (std::strcmp(e, "1") == 0) return 1;
+    if (std::strcmp(e, "auto") == 0) return -1;
+    return 0;
+}
 struct TuneTable {
     std::mutex mu;
     std::map<TuneKey, TuneEntry> map;
@@ -1423,10 +1736,17 @@ struct RowOperator {
     // (precision, R2), and every application of that shape uses its choice
     std::map<int, std::unique_ptr<CellItems>> items_by_len;
     struct CellTune {
-        int len = CELL_ITEM, un = 0;
+        int len = CELL_ITEM, un = 0, h = 0;   // h: the split-half FP16 product (FP32 only)
     };
     std::map<int, CellTune> tuned;   // key tune_key(fp32, R2)
     DevBuf<double> upart;  // partial panels of the work items (FP64 and FP32 share it)
+    // the split-half FP16 product: its split panels of S, [0] the S scale (float bits, written by
+    // k_split_s) and [1 .. AMAX_SLOTS] the block op's max |S| (float bits, cleared by the
+    // product), the split blocks and their scale
+    DevBuf<HalfPair> Sth;
+    DevBuf<unsigned> hsc;
+    const HalfPair *Ah = nullptr;
+    float Ah_scale = 1.0f;
     // bumped whenever a buffer the kernels address is (re)allocated: captured graphs hold
     // these addresses and are rebuilt when it changes
     uint64_t layout_gen = 0;
@@ -1613,6 +1933,7 @@ struct RowOperator {
         }
         if (cells) {
             tune(r, fp32);
+            if (fp32 && choice(true, 2 * r).h) prepare_h(r);
             const size_t need = (size_t)std::max(items_for(fp32, 2 * r).n, 1) * Na * 2 * r;
             if (upart.n < need) {
                 upart.alloc(need);
@@ -1621,7 +1942,7 @@ struct RowOperator {
         }
     }
 
-    static int tune_key(bool fp32, int R2) { return (fp32 ? 1 << 20 : 0) + R2; }
+    static int tune_key(bool fp32, int R2) { return (fp32 ? (1 << 20) + ((cell_h_mode() + 1) << 16) : 0) + R2; }
 
     CellItems &items_of_len(int len) {
         auto it = items_by_len.find(len);
@@ -1671,12 +1992,14 @@ struct RowOperator {
         int dev = 0;
         TT_HIP(hipGetDevice(&dev));
         TuneTable &tt_ = tune_table();
-        const TuneKey gk{dev, col ? 1 : 0, fp32 ? 1 : 0, Na, R2, cv.ncell};
+        const int hmode = fp32 ? cell_h_mode() : 0;
+        const TuneKey gk{dev, col ? 1 : 0, fp32 ? 1 : 0, Na, R2, cv.ncell, hmode};
         {
             std::lock_guard<std::mutex> lock(tt_.mu);
             auto it = tt_.map.find(gk);
             if (it != tt_.map.end()) {
-                tuned[key] = {it->second.len, it->second.un};
+                tuned[key] = {it->second.len, it->second.un, it->second.h};
+                if (it->second.h) prepare_h(r);
                 ++layout_gen;
                 return;
             }
@@ -1695,15 +2018,31 @@ struct RowOperator {
         TT_HIP(hipEventCreate(&e1));
         CellTune best;
         float best_ms = 1e30f;
+        if (hmode != 0) prepare_h(r);
         try {
             for (int len : lens) {
                 const CellItems &ci = items_of_len(len);
-                const std::vector<int> &uns = fp32 ? cell_un_candidates<float>() : cell_un_candidates<double>();
-                for (int un : uns) {
+                // candidates: (un, h); the split-half product takes 1 or 2 steps per batch
+                std::vector<std::pair<int, int>> cands;
+                if (hmode != 1)
+                    for (int un : fp32 ? cell_un_candidates<float>() : cell_un_candidates<double>())
+                        cands.emplace_back(un, 0);
+                if (hmode != 0)
+                    for (int un : {1, 2}) cands.emplace_back(un, 1);
+                for (const auto &cand : cands) {
+                    const int un = cand.first, h = cand.second;
                     // the product and the block op that sums its item panels (the item length
                     // trades the product's parallelism against that sum)
                     auto launch = [&]() {
-                        if (fp32) {
+                        if (h) {
+                            launch_cellh(s, &cv, ci.ref(), Ah, Ah_scale, reinterpret_cast<const float *>(S32.p), Sth.p,
+                                         reinterpret_cast<float *>(hsc.p), hsc.p + 1,
+                                         reinterpret_cast<float *>(upart.p), R2, un);
+                            launch_block<float2, float2, float>(s, reinterpret_cast<const float2 *>(upart.p), U32.p,
+                                                                vc_ptr.p, vc_alpha.p, vc_blocks32.p, vc_npart_max,
+                                                                nullptr, gidx.p, Na, Nq, Np, r, nullptr, 1, 0,
+                                                                ci.ket_item_ptr.p, 2.0);
+                        } else if (fp32) {
                             launch_cellmm<float>(s, &cv, ci.ref(), cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                                  reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
                             launch_block<float2, float2, float>(s, reinterpret_cast<const float2 *>(upart.p), U32.p,
@@ -1729,6 +2068,7 @@ struct RowOperator {
                         best_ms = ms;
                         best.len = len;
                         best.un = un;
+                        best.h = h;
                     }
                 }
             }
@@ -1742,7 +2082,7 @@ struct RowOperator {
         tuned[key] = best;
         ++layout_gen;
         std::lock_guard<std::mutex> lock(tt_.mu);
-        tt_.map[gk] = {best.len, best.un, best_ms / 3.0f};
+        tt_.map[gk] = {best.len, best.un, best.h, best_ms / 3.0f};
     }
 
     void set_gidx(const std::vector<int> &g) {
@@ -1764,6 +2104,47 @@ struct RowOperator {
         } else if (!m->val32) {
             TT_HIP(hipMalloc(&m->val32, sizeof(float) * std::max<int64_t>(m->nnz, 1)));
             if (m->nnz) hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz, 256)), dim3(256), 0, s, m->val, m->val32, m->nnz);
+        }
+    }
+
+    // Split-half blocks of this orientation (made once per P123 from the FP64 blocks) and the
+    // buffers of the split-half product for r right-hand sides.  Not inside a graph capture.
+    void prepare_h(int r) {
+        DeviceCSC *m = const_cast<DeviceCSC *>(P);
+        uint16_t *&bh = col ? m->t_blkh : m->blkh;
+        double &sc = col ? m->t_blkh_scale : m->blkh_scale;
+        const int Kp8 = (Na + 7) / 8;
+        if (!bh) {
+            const double *src = col ? m->t_blk : m->blk;
+            DevBuf<unsigned> mx(1);
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, mx.p, 1, 0u);
+            const int64_t n = m->ncell * Na * Na;
+            hipLaunchKernelGGL(k_absmax_f64, dim3(std::min<unsigned>(nblk(n, 256), 2048)), dim3(256), 0, s, src, n, mx.p);
+            unsigned bits = 0;
+            TT_HIP(hipMemcpyAsync(&bits, mx.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            TT_HIP(hipStreamSynchronize(s));
+            float mf;
+            std::memcpy(&mf, &bits, sizeof(mf));
+            sc = split_scale(mf);
+            const int64_t nh = m->ncell * Na * Kp8;
+            TT_HIP(hipMalloc(&bh, sizeof(HalfPair) * std::max<int64_t>(nh, 1)));
+            if (nh)
+                hipLaunchKernelGGL(k_blk_split, dim3(nblk(nh, 256)), dim3(256), 0, s, src,
+                                   reinterpret_cast<HalfPair *>(bh), m->ncell, Na, Kp8, sc);
+            TT_HIP(hipGetLastError());
+        }
+        Ah = reinterpret_cast<const HalfPair *>(bh);
+        Ah_scale = (float)sc;
+        reserve(r, true);
+        const size_t need = (size_t)m->NqNp * 2 * r_cap * Kp8;
+        if (Sth.n < need) {
+            Sth.alloc(need);
+            ++layout_gen;
+        }
+        if (!hsc.p) {
+            hsc.alloc(1 + AMAX_SLOTS);
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, hsc.p, 1 + AMAX_SLOTS, 0u);
+            ++layout_gen;
         }
     }
 
@@ -1834,16 +2215,22 @@ struct RowOperator {
             ensure_f32();
             // FP32 vectors stage and sum the block ops in FP32 too (FP32 block copies); FP64
             // vectors keep FP64 block ops around the FP32 product
+            // the split-half product needs the block op's max |S|
+            const bool h = cells && choice(true, R2).h;
+            unsigned *amax = h ? hsc.p + 1 : nullptr;
             if constexpr (v64)
                 launch_block<VT, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
                                          use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr, 1.0,
-                                         zb, zn);
+                                         zb, zn, amax);
             else
                 launch_block<VT, float2, float, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks32.p, c_npart_max,
                                                 use_G ? G32.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
-                                                1.0, zb, zn);
+                                                1.0, zb, zn, amax);
             if (timed) record(e1);
-            if (cells)
+            if (h)
+                launch_cellh(s, &cv, ci->ref(), Ah, Ah_scale, reinterpret_cast<const float *>(S32.p), Sth.p,
+                             reinterpret_cast<float *>(hsc.p), amax, reinterpret_cast<float *>(upart.p), R2, un);
+            else if (cells)
                 launch_cellmm<float>(s, &cv, ci->ref(), cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
                                      reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
             else
@@ -3652,11 +4039,11 @@ int tt_cell_tuning(char *buf, int len) {
     {
         std::lock_guard<std::mutex> lock(t.mu);
         for (const auto &kv : t.map) {
-            char line[160];
-            std::snprintf(line, sizeof(line), "dev %d %s %s Nalpha %d R2 %d ncell %lld -> item_len %d un %d (%.1f us)\n",
+            char line[192];
+            std::snprintf(line, sizeof(line), "dev %d %s %s Nalpha %d R2 %d ncell %lld -> item_len %d un %d%s (%.1f us)\n",
                           kv.first.dev, kv.first.col ? "column" : "row", kv.first.fp32 ? "fp32" : "fp64",
                           kv.first.Na, kv.first.R2, (long long)kv.first.ncell, kv.second.len, kv.second.un,
-                          1e3 * kv.second.ms);
+                          kv.second.h ? " fp16-split" : "", 1e3 * kv.second.ms);
             out += line;
         }
     }

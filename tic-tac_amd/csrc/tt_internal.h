@@ -187,6 +187,12 @@ struct DeviceCSC {
     int *t_partner = nullptr;        // ncell
     double *t_blk = nullptr;         // ncell * Na * Na + BLK_SLACK
     float *t_blk32 = nullptr;
+    // split-half copies of the blocks for the FP16 product (made on first use, per orientation):
+    // each block column alpha as chunks of 8 consecutive input alphas (padded with zeros to
+    // Kp = 8 * ceil(Na / 8)), every chunk its 8 high halves then its 8 low halves, of the block
+    // values times blkh_scale (a power of two putting the largest magnitude at or below 2^14)
+    uint16_t *blkh = nullptr, *t_blkh = nullptr;   // ncell * Na * (Kp / 8) * 16
+    double blkh_scale = 0.0, t_blkh_scale = 0.0;
     ~DeviceCSC();
     static uint64_t next_gen();
 };
