@@ -56,8 +56,10 @@ def extract_potential_element(L, Lp, J, S, coupled, v):
     return 0.0
 
 
-def potential_matrices(fwp, pw_full, pot, tensor_force, isospin_breaking_1S0, J_2N_max):
-    """CPP/make_potential_matrix.cpp:51-430 (quadrature branch, midpoint_approx=false).
+def potential_matrices(fwp, pw_full, pot, tensor_force, isospin_breaking_1S0, J_2N_max, midpoint_approx=False):
+    """CPP/make_potential_matrix.cpp:51-430: the quadrature branch (330-396) or, with
+    midpoint_approx, the mid-point branch (271-329): V_WP = p_in p_out f_in f_out d_r d_c V / (N_r N_c)
+    at the bin midpoints p (CPP/make_wp_states.cpp:150-160), f = 1 for momentum wave packets.
 
     Returns (V_unco[nu, Np, Np], V_coup[nc, 2Np, 2Np]); blocks never touched stay NaN.
     """
@@ -100,37 +102,46 @@ def potential_matrices(fwp, pw_full, pot, tensor_force, isospin_breaking_1S0, J_
                 if idx in done_u:
                     continue
                 done_u.add(idx)
+            def isoscalar(p_in, p_out):
+                """The six V slots of the isoscalar combination (lines 283-316 / 351-385)."""
+                if not via_T and iso == 0:
+                    return np.array(pot.V(p_in, p_out, coupled_model, Sr, Jr, Tr, 0))
+                if not via_T and iso == 1:
+                    vnn = np.array(pot.V(p_in, p_out, coupled_model, Sr, Jr, Tr, -1))
+                    vnp = np.array(pot.V(p_in, p_out, coupled_model, Sr, Jr, Tr, 0))
+                    return (2. / 3) * vnn + (1. / 3) * vnp
+                if via_T:
+                    vnn = pot.V(p_in, p_out, coupled_model, Sr, Jr, Tr, -1)
+                    vnp = pot.V(p_in, p_out, coupled_model, Sr, Jr, Tr, 0)
+                    enn = extract_potential_element(Lr, Lc, Jr, Sr, False, vnn)
+                    enp = extract_potential_element(Lr, Lc, Jr, Sr, False, vnp)
+                    vis = np.zeros(6)
+                    vis[2] = (2. / 3) * enn + (1. / 3) * enp
+                    vis[3] = (math.sqrt(2) / 3.) * (enn - enp)
+                    vis[4] = (math.sqrt(2) / 3.) * (enn - enp)
+                    vis[5] = (1. / 3) * enn + (2. / 3) * enp
+                    return vis
+                raise ValueError("unknown isoscalar potential")
+
             for br in range(Np):
                 Nr = fwp.norm_p[br]
                 for bc in range(Np):
                     Nc = fwp.norm_p[bc]
                     acc = np.zeros(6)
-                    for ir in range(npw):
-                        pr = pts[br * npw + ir]
-                        fr = wts[br * npw + ir] * pr
-                        for ic in range(npw):
-                            pc = pts[bc * npw + ic]
-                            fc = wts[bc * npw + ic] * pc
-                            fac = fr * fc
-                            if not via_T and iso == 0:
-                                vis = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0))
-                            elif not via_T and iso == 1:
-                                vnn = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, -1))
-                                vnp = np.array(pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0))
-                                vis = (2. / 3) * vnn + (1. / 3) * vnp
-                            elif via_T:
-                                vnn = pot.V(pc, pr, coupled_model, Sr, Jr, Tr, -1)
-                                vnp = pot.V(pc, pr, coupled_model, Sr, Jr, Tr, 0)
-                                enn = extract_potential_element(Lr, Lc, Jr, Sr, False, vnn)
-                                enp = extract_potential_element(Lr, Lc, Jr, Sr, False, vnp)
-                                vis = np.zeros(6)
-                                vis[2] = (2. / 3) * enn + (1. / 3) * enp
-                                vis[3] = (math.sqrt(2) / 3.) * (enn - enp)
-                                vis[4] = (math.sqrt(2) / 3.) * (enn - enp)
-                                vis[5] = (1. / 3) * enn + (2. / 3) * enp
-                            else:
-                                raise ValueError("unknown isoscalar potential")
-                            acc += fac * vis / (Nr * Nc)
+                    if midpoint_approx:
+                        p_in = 0.5 * (fwp.p_edges[bc] + fwp.p_edges[bc + 1])
+                        p_out = 0.5 * (fwp.p_edges[br] + fwp.p_edges[br + 1])
+                        d_c = fwp.p_edges[bc + 1] - fwp.p_edges[bc]
+                        d_r = fwp.p_edges[br + 1] - fwp.p_edges[br]
+                        acc += p_in * p_out * d_r * d_c * isoscalar(p_in, p_out) / (Nr * Nc)
+                    else:
+                        for ir in range(npw):
+                            pr = pts[br * npw + ir]
+                            fr = wts[br * npw + ir] * pr
+                            for ic in range(npw):
+                                pc = pts[bc * npw + ic]
+                                fc = wts[bc * npw + ic] * pc
+                                acc += fr * fc * isoscalar(pc, pr) / (Nr * Nc)
                     if coupled_matrix:
                         if via_L:
                             Vc[idx, br, bc] = extract_potential_element(Jr - 1, Jr - 1, Jr, Sr, True, acc)

@@ -180,3 +180,35 @@ def test_breakup_cells_match_oracle():
         assert np.array_equal(ch.bu_ptr, ptr)
         assert np.array_equal(ch.bu_cells, cells)
         assert ptr[1] == 0   # the lowest bin lies below the threshold
+
+
+def test_midpoint_potential_matrices_match_oracle():
+    """The library's mid-point V_WP (run option midpoint_approx, CPP/make_potential_matrix.cpp:271-329)
+    against the oracle's restatement, on the synthetic potential with tensor force and isospin
+    breaking; the builders are called directly (a mid-point V_WP of this potential need not bind the
+    deuteron alone, which the SWP step checks)."""
+    N = 8
+    rp = tt.run_params(two_J_3N_max=1, J_2N_max=3, Np_WP=N, Nq_WP=N, chebyshev_s=tt.chebyshev_scale(N),
+                       midpoint_approx=1)
+    pot = tt.Potential("synthetic", tt.synthetic_params(7))
+    lib = L.lib()
+    pw, fwp = L.PwStates(), L.FwpStates()
+    L.check(lib.tt_construct_pw_states(C.byref(rp), C.byref(pw)))
+    L.check(lib.tt_make_fwp_statespace(C.byref(rp), C.byref(fwp)))
+    nu, nc = C.c_int(), C.c_int()
+    lib.tt_num_2N_states(C.byref(rp), C.byref(nu), C.byref(nc))
+    Vu = np.full(nu.value * N * N, np.nan)
+    Vc = np.full(nc.value * 4 * N * N, np.nan)
+    L.check(lib.tt_calculate_potential_matrices(C.byref(rp), C.byref(fwp), C.byref(pw), pot._h, L.dptr(Vu),
+                                                L.dptr(Vc)))
+    lib.tt_pw_states_free(C.byref(pw))
+    lib.tt_fwp_states_free(C.byref(fwp))
+    pw_o = ost.construct_pw_states(3, 1, True)
+    fwp_o = ost.make_fwp(N, N, rp.chebyshev_s, rp.chebyshev_t, rp.Np_per_WP)
+    theta, beta = opot.synthetic_theta(7)
+    Vuo, Vco = osw.potential_matrices(fwp_o, pw_o, opot.Synthetic(theta, beta), True, True, 3,
+                                      midpoint_approx=True)
+    for a, b in ((Vu.reshape(Vuo.shape), Vuo), (Vc.reshape(Vco.shape), Vco)):
+        used = ~np.isnan(b)
+        assert np.array_equal(used, ~np.isnan(a)) and used.any()
+        assert np.abs(a[used] - b[used]).max() <= 1e-12 * np.abs(b[used]).max()

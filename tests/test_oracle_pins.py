@@ -454,3 +454,29 @@ def test_breakup_dense_values_against_explicit_inverse_and_pade():
     Up = np.array([[so.pade_select(a_bu[d, b])[0] if ok[d, b] else np.nan for b in range(len(cells))]
                    for d in range(len(deut))])
     assert np.abs(Up[ok] - Ubu[ok]).max() <= 1e-6 * np.abs(Ubu[ok]).max()
+
+
+def test_midpoint_branch_is_one_point_quadrature():
+    """The mid-point branch of V_WP (CPP/make_potential_matrix.cpp:271-329: p_in p_out d_r d_c V at the
+    bin midpoints) equals the quadrature branch (330-396, pinned against scipy dblquad) with one
+    Gauss-Legendre point per bin, whose node is the bin midpoint and whose weight the bin width; on
+    the synthetic potential with tensor force and isospin breaking (coupled and T-mixing blocks)."""
+    N = 6
+    pw = st.construct_pw_states(3, 1, True)
+    s = 100.0
+    fwp1 = st.make_fwp(N, N, s, 1.0, Np_per_WP=1)
+    fwp8 = st.make_fwp(N, N, s, 1.0, Np_per_WP=8)
+    assert np.allclose(fwp1.p_pts, 0.5 * (fwp1.p_edges[1:] + fwp1.p_edges[:-1]), rtol=1e-15)
+    assert np.allclose(fwp1.p_wts, np.diff(fwp1.p_edges), rtol=1e-14)
+    theta, beta = pot.synthetic_theta(7)
+    orc = pot.Synthetic(theta, beta)
+    Vu_q, Vc_q = sw.potential_matrices(fwp1, pw, orc, True, True, 3)
+    Vu_m, Vc_m = sw.potential_matrices(fwp8, pw, orc, True, True, 3, midpoint_approx=True)
+    for a, b in ((Vu_m, Vu_q), (Vc_m, Vc_q)):
+        used = ~np.isnan(b)
+        assert np.array_equal(used, ~np.isnan(a)) and used.any()
+        assert np.abs(a[used] - b[used]).max() <= 1e-13 * np.abs(b[used]).max()
+    # and it is not the 8-point quadrature (the branch is taken)
+    Vu_8, _ = sw.potential_matrices(fwp8, pw, orc, True, True, 3)
+    used = ~np.isnan(Vu_8)
+    assert np.abs(Vu_m[used] - Vu_8[used]).max() > 1e-6 * np.abs(Vu_8[used]).max()
