@@ -691,10 +691,10 @@ def split_half_product():
 
 
 def test_split_half_product_against_oracle(ctx, c2, c3, split_half_product):
-    """The FP32 product on the FP16 MFMA (every operand as a high and a low half) at the one-,
-    two-, three- and four-M-tile shapes: N_alpha 27 (C2), 47, and 63 (a 7/2+ channel at
-    Np = Nq = 8): rows against the oracle's row application to 1e-5 like the FP32 MFMA product,
-    and the mixed solve against FP64 GMRES to 1e-10."""
+    """The FP32 product on the FP16 MFMA (every operand as a high and a low half) at the
+    two-, three- and four-M-tile shapes, N_alpha 27 (C2), 47, and 63 (a 7/2+ channel at
+    Np = Nq = 8): rows and columns against the oracle's applications to 1e-5 like the FP32 MFMA
+    product, and the mixed solve against FP64 GMRES to 1e-10."""
     ctx2 = split_half_product
     rp, pot, T = synthetic_config(N=8, two_J_3N_max=7, T_lab=(10.0, 50.0))
     prob = tt.Problem(rp, pot)
@@ -713,4 +713,10 @@ def test_split_half_product_against_oracle(ctx, c2, c3, split_half_product):
         Umx, st = c.ch.solve(ctx2, c.P, c.G, method=tt.SOLVE_MIXED)
         assert st.n_apply_fp32 > 0
         assert np.abs(Umx - U64).max() <= 1e-10 * np.abs(U64).max(), c.ch.Nalpha
+        # the column form (transposed cells, products with P123 itself) of an odd number of columns
+        Xc = X[:3]
+        Yco = c.K.col_apply((Xc * np.stack([c.G[k] for k in g[:3]])).T).T
+        Yc32 = c.ch.apply_rows(ctx2, c.P, Xc, G=c.G, g_index=g[:3], column=True, fp32=True)
+        assert np.abs(Yc32 - Yco).max() <= 1e-5 * np.abs(Yco).max(), c.ch.Nalpha
     assert any("fp16-split" in ln for ln in tt.cell_tuning())
+    assert any("column fp32" in ln and "fp16-split" in ln for ln in tt.cell_tuning())
