@@ -125,7 +125,31 @@ def s8d_bytes(nnz, dim, r, fp32):
     return nnz * (sv + 4.0) + 4.0 * (dim + 1) + 2.0 * r * dim * sc
 
 
-def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=None, lanes=1):
+TRACE_FILE = os.path.join(ROOT, "profiles", "kernels_batch_synthetic_np30.md")
+
+
+def traced_product_us(fp32):
+    """Mean kernel duration of the cell products of one precision in the batch's rocprofv3 kernel
+    trace summarised in profiles/ (the same bench.py workload; the mixed batch's table for FP32,
+    the FP64 batch's for FP64), launch-weighted over the instances, or None."""
+    head = "Per mixed batch" if fp32 else "Per FP64 batch"
+    prefix = "| `k_cellmfma<%s" % ("float" if fp32 else "double")
+    n = tot = 0.0
+    try:
+        lines = open(TRACE_FILE).read().split(head, 1)[1].split("\n\n", 2)[1].splitlines()
+    except (OSError, IndexError):
+        return None
+    for line in lines:
+        if not line.startswith("|"):
+            break
+        if line.startswith(prefix):
+            cells = [c.strip() for c in line.strip("|").split("|")]
+            n += float(cells[1])
+            tot += float(cells[2]) * 1e3
+    return tot / n if n else None
+
+
+def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=None, lanes=1, traced_us=None):
     """HBM roofline of the product over P123 (the dominant kernel of the A-application).
 
     achieved = the algorithmic bytes per launch by s8d_bytes (per-channel nonzeros, dim and r)
@@ -137,7 +161,10 @@ def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=Non
     against the MFMA roof of their precision, at 4 real flops per nonzero and complex
     right-hand side (a real value times a complex element: two fused multiply-adds).
     With lanes > 1 the channels' streams run concurrently, so a launch shares the device with
-    the other lanes' kernels and its event time is its duration under that sharing.  With
+    the other lanes' kernels and its event time is its duration under that sharing, plus the time
+    it waits behind another lane's kernels where two lanes share one of HIP's hardware queues;
+    traced_us (traced_product_us) is the kernels' own mean duration in the rocprofv3 trace of
+    the same workload, and frac_kernel_trace the fraction with it.  With
     wall_s, aggregate_achieved = the s8d bytes of every product launch of the timed steps
     (sampled or not) over their wall time: a lower bound on the products' HBM rate.
     stats[i] belongs to the channel of shapes[i % len(shapes)] (nnz, dim, r)."""
@@ -170,6 +197,9 @@ def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=Non
                     "peak_tflops": peak_tf,
                     "frac": round(fl / sec / 1e12 / peak_tf, 4) if ms > 0 else 0.0},
            "concurrent_streams": lanes}
+    if traced_us:
+        out["avg_us_kernel_trace"] = round(traced_us, 2)
+        out["frac_kernel_trace"] = round(b8 / max(n, 1) / (traced_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
     if wall_s:
         out["aggregate_achieved"] = round(every / wall_s / 1e9, 1)
     return out
@@ -526,9 +556,11 @@ def main():
         "plain_fp64": None,
         "mixed_over_plain_fp64": None,
         "grad_over_fwd": None,
-        "roofline": product_roofline(st_mix, shapes, k32, True, measured_traffic(k32), t_mix, LANES),
+        "roofline": product_roofline(st_mix, shapes, k32, True, measured_traffic(k32), t_mix, LANES,
+                                     traced_product_us(True)),
         "roofline_fp64": product_roofline(st_64, shapes, "k_cellmfma<double, MT, NT, UN>", False,
-                                          measured_traffic("k_cellmfma<double, MT, NT, UN>"), t_64, LANES),
+                                          measured_traffic("k_cellmfma<double, MT, NT, UN>"), t_64, LANES,
+                                          traced_product_us(False)),
         "krylov_steps_per_channel": {"mixed": iteration_summary(st_mix, len(wl.chans)),
                                      "fp64": iteration_summary(st_64, len(wl.chans))},
         "cell_autotune": tt.cell_tuning(),
