@@ -676,6 +676,55 @@ def test_channel_above_64_alphas_takes_the_csc_gather(ctx):
 
 
 @pytest.fixture
+def grouped_launches():
+    """Restores TT_GROUPED and TT_LANES (read by every batch call) after the test."""
+    old = {k: os.environ.get(k) for k in ("TT_GROUPED", "TT_LANES")}
+    yield
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+def test_grouped_batch_matches_lane_batch(ctx, grouped_launches):
+    """TT_GROUPED=1: each stage of the lockstep steps (the block ops, the cell product per
+    kernel instance, the Gram-Schmidt kernels) and of the FP64 residual applications is one
+    launch for all channels of a lane: all channels of the batch with TT_LANES=1, two or one
+    per lane with three lanes.  The channels 2J = 1..7 of both parities at Np = Nq = 8
+    (N_alpha 27 to 63: the two-, three- and four-M-tile product instances) give the six-lane
+    batch's U to the solve tolerance (1e-10 of max|U|: the dot products sum by device
+    atomics), FP64 and mixed, twice in a row with single solves in between; the FP64 result of
+    the first channel equals the oracle's dense solve."""
+    rp, pot, _ = synthetic_config(N=8, two_J_3N_max=7)
+    T = [5.0, 50.0]
+    prob = tt.Problem(rp, pot)
+    chans = [ch for ch in (prob.channel(c, T) for c in range(prob.n_channels)) if ch.nd > 0]
+    assert {27, 47, 63} <= {ch.Nalpha for ch in chans}
+    Ps = [ch.build_p123(ctx) for ch in chans]
+    for method in (tt.SOLVE_GMRES, tt.SOLVE_MIXED):
+        os.environ["TT_GROUPED"] = "0"
+        os.environ.pop("TT_LANES", None)
+        ref = [U for U, _, _ in prob.solve_batch(ctx, chans, Ps, method=method)]
+        os.environ["TT_GROUPED"] = "1"
+        for lanes in ("1", "3"):
+            os.environ["TT_LANES"] = lanes
+            for rep in range(2):
+                out = prob.solve_batch(ctx, chans, Ps, method=method)
+                for (U, st, status), Ur, ch in zip(out, ref, chans):
+                    assert status == 0 and st.residual <= 1e-12, (method, lanes, rep, ch.Nalpha)
+                    assert np.abs(U - Ur).max() <= 1e-10 * np.abs(Ur).max(), (method, lanes, rep, ch.Nalpha)
+                chans[rep].solve(ctx, Ps[rep], None, method=method, device_resolvent=True)
+                ctx.synchronize()
+    os.environ["TT_LANES"] = "1"
+    U0 = prob.solve_batch(ctx, chans, Ps, method=tt.SOLVE_GMRES)[0][0]
+    c = _setup(ctx, rp, pot, T, channel=0)
+    assert c.ch.Nalpha == chans[0].Nalpha
+    Uo = so.dense_solve(c.K.dense(), c.G, c.ch.deut, c.ch.q_idx, rp.Nq_WP, rp.Np_WP)
+    assert np.abs(U0 - Uo).max() <= 1e-10 * np.abs(Uo).max()
+
+
+@pytest.fixture
 def split_half_product():
     """Forces the split-half FP16 product for the FP32 applications of a fresh context (the
     autotuner reads TT_CELL_H when it first meets a shape)."""

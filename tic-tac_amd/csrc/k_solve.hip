@@ -395,22 +395,24 @@ __device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 
     }
 }
 
-template <typename TI, typename TO, typename CT = double, typename GT = double2>
-__global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
-                                               const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
-                                               const CT *__restrict__ blocks, const GT *__restrict__ G,
-                                               const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
-                                               int ne_max, const TO *__restrict__ Zsub, int in_cell,
-                                               int out_cell, const int *__restrict__ ket_item_ptr,
-                                               double in_scale, double2 *__restrict__ zero_buf, int zero_n,
-                                               unsigned *__restrict__ amax) {
+// The tile (bx, by) of the op: bx = a * Nq + q, by = the k-chunk (k_block's grid; a grouped
+// launch passes each member's own tile indices).
+template <typename TI, typename TO, typename CT, typename GT>
+__device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restrict__ Y,
+                                           const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
+                                           const CT *__restrict__ blocks, const GT *__restrict__ G,
+                                           const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
+                                           int ne_max, const TO *__restrict__ Zsub, int in_cell, int out_cell,
+                                           const int *__restrict__ ket_item_ptr, double in_scale,
+                                           double2 *__restrict__ zero_buf, int zero_n, unsigned *__restrict__ amax,
+                                           int bx, int by) {
     extern __shared__ double smem[];
     __shared__ int gk[256];   // gidx of this chunk's columns (kc <= 256)
-    if (zero_buf && blockIdx.x == 0 && blockIdx.y == 0)   // a buffer a later kernel accumulates into
+    if (zero_buf && bx == 0 && by == 0)   // a buffer a later kernel accumulates into
         for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_buf[i] = make_double2(0.0, 0.0);
     const int64_t dim = (int64_t)Na * Nq * Np;
-    const int a = blockIdx.x / Nq, q = blockIdx.x % Nq;
-    const int k0 = blockIdx.y * kc, nk = min(kc, r - k0);
+    const int a = bx / Nq, q = bx % Nq;
+    const int k0 = by * kc, nk = min(kc, r - k0);
     const int e0 = part_ptr[a], ne = part_ptr[a + 1] - e0;
     const int NpNp = Np * Np;
     typedef typename Cplx<CT>::type CT2;
@@ -535,9 +537,22 @@ __global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__r
         // one atomic per workgroup, spread over AMAX_SLOTS words (a single word serialises the
         // launch's atomics); non-negative floats order as their bit patterns do
         if (threadIdx.x == 0)
-            atomicMax(amax + (blockIdx.x + blockIdx.y) % AMAX_SLOTS,
+            atomicMax(amax + (bx + by) % AMAX_SLOTS,
                       __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
     }
+}
+
+template <typename TI, typename TO, typename CT = double, typename GT = double2>
+__global__ __launch_bounds__(256) void k_block(const TI *__restrict__ X, TO *__restrict__ Y,
+                                               const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
+                                               const CT *__restrict__ blocks, const GT *__restrict__ G,
+                                               const int *__restrict__ gidx, int Na, int Nq, int Np, int r, int kc,
+                                               int ne_max, const TO *__restrict__ Zsub, int in_cell,
+                                               int out_cell, const int *__restrict__ ket_item_ptr,
+                                               double in_scale, double2 *__restrict__ zero_buf, int zero_n,
+                                               unsigned *__restrict__ amax) {
+    block_tile<TI, TO, CT, GT>(X, Y, part_ptr, part_alpha, blocks, G, gidx, Na, Nq, Np, r, kc, ne_max, Zsub, in_cell,
+                               out_cell, ket_item_ptr, in_scale, zero_buf, zero_n, amax, blockIdx.x, blockIdx.y);
 }
 
 // Launch of k_block over all (a, q) and k-chunks of at most 24 columns, of balanced widths
@@ -547,21 +562,33 @@ template <typename T>
 struct NoDeduce {
     typedef T type;
 };
+// k-chunks of at most 24 columns, of balanced widths: kc columns each, gy chunks; the dynamic
+// LDS of a workgroup (partner blocks, staged inputs, partner alphas and item ranges)
+struct BlockGeom {
+    int kc, gy;
+    size_t lds;
+};
+template <typename CT>
+static BlockGeom block_geom(int r, int ne_max, int Np) {
+    BlockGeom g;
+    const int nch = (r + 23) / 24;
+    g.kc = (r + nch - 1) / nch;
+    g.gy = (r + g.kc - 1) / g.kc;
+    g.lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(CT) +
+            (size_t)ne_max * Np * g.kc * sizeof(typename Cplx<CT>::type) + (size_t)(ne_max + Np + 1) * sizeof(int);
+    TT_REQUIRE(g.lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
+    return g;
+}
 template <typename TI, typename TO, typename CT = double, typename GT = double2>
 static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr, const int *part_alpha,
                          const CT *blocks, int ne_max, const typename NoDeduce<GT>::type *G, const int *gidx, int Na,
                          int Nq, int Np,
                          int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
                          double in_scale, double2 *zero_buf = nullptr, int zero_n = 0, unsigned *amax = nullptr) {
-    const int nch = (r + 23) / 24;
-    const int kc = (r + nch - 1) / nch;
-    const size_t lds = (size_t)((ne_max * Np * Np + 1) & ~1) * sizeof(CT) +
-                       (size_t)ne_max * Np * kc * sizeof(typename Cplx<CT>::type) +
-                       (size_t)(ne_max + Np + 1) * sizeof(int);
-    TT_REQUIRE(lds <= 160 * 1024, "block-op tile exceeds LDS (partner blocks too large)");
-    const dim3 grid((unsigned)(Na * Nq), (unsigned)((r + kc - 1) / kc)), block(256);
-    hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx, Na,
-                       Nq, Np, r, kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n,
+    const BlockGeom bg = block_geom<CT>(r, ne_max, Np);
+    const dim3 grid((unsigned)(Na * Nq), (unsigned)bg.gy), block(256);
+    hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, bg.lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx,
+                       Na, Nq, Np, r, bg.kc, ne_max, Zsub, in_cell, out_cell, ket_item_ptr, in_scale, zero_buf, zero_n,
                        amax);
 }
 
@@ -664,23 +691,29 @@ struct Run {
 };
 
 constexpr int XCD_CHUNK = 32;   // items per chunk of the XCD-aware order
+// XCD-aware order of n workgroups: workgroups are dealt to the 8 XCDs round-robin, so
+// workgroup b runs on XCD b % 8.  The items are cut into chunks of XCD_CHUNK consecutive items
+// (neighbouring kets, which read largely the same bra rows of S), dealt to the XCDs in turn:
+// each chunk is read through one XCD's L2, and every XCD gets work from the whole item range,
+// so the XCDs finish together.  The last partial round keeps the launch order.  A bijection
+// of [0, n).
+__device__ __forceinline__ int xcd_order(int b, int n) {
+    const int xcd = b % 8, w = b / 8;
+    const int full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    return b < full ? ((w / XCD_CHUNK) * 8 + xcd) * XCD_CHUNK + w % XCD_CHUNK : b;
+}
+
+// Work item `item`, v-tile vt (columns 16 NT vt ...) of the product (k_cellmfma's grid; a
+// grouped launch passes each member's own item and tile).
 template <typename T, int MT, int NT, int UN>
-__global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
-                                                  const T *__restrict__ B, const T *__restrict__ S,
-                                                  T *__restrict__ Upart, int Na, int R2) {
+__device__ __forceinline__ void cell_item(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
+                                          const T *__restrict__ B, const T *__restrict__ S, T *__restrict__ Upart,
+                                          int Na, int R2, int item, int vt) {
     typedef typename Mfma16<T>::acc_t acc_t;
     __shared__ acc_t red[3][MT * NT][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
-    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so workgroup b runs on
-    // XCD b % 8.  The items are cut into chunks of XCD_CHUNK consecutive items (neighbouring
-    // kets, which read largely the same bra rows of S), dealt to the XCDs in turn: each chunk
-    // is read through one XCD's L2, and every XCD gets work from the whole item range, so the
-    // XCDs finish together.  The last partial round keeps the launch order.
-    const int nit = gridDim.x, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
-    const int full = nit / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
-    const int item = (int)blockIdx.x < full ? ((w / XCD_CHUNK) * 8 + xcd) * XCD_CHUNK + w % XCD_CHUNK : (int)blockIdx.x;
-    const int v0 = blockIdx.y * 16 * NT;
+    const int v0 = vt * 16 * NT;
     const int c_beg = items[item].x, ncell = items[item].y - c_beg;
     const int K = ncell * Na;
     const T *Bk = B + (int64_t)c_beg * Na * Na;
@@ -770,6 +803,14 @@ __global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items
                 if (a < Na && v < R2) Upart[((int64_t)item * Na + a) * R2 + v] = t[r];
             }
         }
+}
+
+template <typename T, int MT, int NT, int UN>
+__global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
+                                                  const T *__restrict__ B, const T *__restrict__ S,
+                                                  T *__restrict__ Upart, int Na, int R2) {
+    cell_item<T, MT, NT, UN>(items, cell_bra, B, S, Upart, Na, R2, xcd_order((int)blockIdx.x, (int)gridDim.x),
+                             (int)blockIdx.y);
 }
 
 // U[ket][a][v] = scale * sum of the partial panels of the ket's work items, in item order
@@ -910,10 +951,7 @@ __global__ __launch_bounds__(256) void k_cellh(const int2 *__restrict__ items, c
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, il = lane & 15;
     if (amax_reset && blockIdx.x == 0 && blockIdx.y == 0 && tid < AMAX_SLOTS) amax_reset[tid] = 0u;
-    // XCD-aware item order, as k_cellmfma
-    const int nit = gridDim.x, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
-    const int full = nit / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
-    const int item = (int)blockIdx.x < full ? ((w / XCD_CHUNK) * 8 + xcd) * XCD_CHUNK + w % XCD_CHUNK : (int)blockIdx.x;
+    const int item = xcd_order((int)blockIdx.x, (int)gridDim.x);   // as k_cellmfma
     const int v0 = blockIdx.y * 16 * NT;
     const int c_beg = items[item].x, ncell = items[item].y - c_beg;
     const int nch = ncell * Kp8;                       // 8-entry chunks of K
@@ -1296,12 +1334,12 @@ __global__ void k_to_f32(const double *__restrict__ a, float *__restrict__ b, in
 // several times the dot products themselves; atomics need none.
 constexpr int DOT_UN = 8;
 
+// Slice b of nb of vector i (k_dots' grid; a grouped launch passes each member's own).  Threads
+// past the last whole column group (blockDim.x % r of them) only take part in the reduction.
 template <typename VT>
-__global__ void k_dots(const VT *__restrict__ V, int64_t vstride, const VT *__restrict__ W,
-                       int64_t n_elem, int r, double2 *__restrict__ accum) {
+__device__ __forceinline__ void dots_slice(const VT *__restrict__ V, int64_t vstride, const VT *__restrict__ W,
+                                           int64_t n_elem, int r, double2 *__restrict__ accum, int b, int nb, int i) {
     extern __shared__ double2 sh[];
-    const int i = blockIdx.y;
-    const int b = blockIdx.x, nb = gridDim.x;
     const VT *Vi = V + (int64_t)i * vstride;
     const int tpr = blockDim.x / r;  // threads per column k
     const int k = threadIdx.x % r;
@@ -1340,6 +1378,12 @@ __global__ void k_dots(const VT *__restrict__ V, int64_t vstride, const VT *__re
         atomicAdd(&accum[(int64_t)i * r + threadIdx.x].x, ar);
         atomicAdd(&accum[(int64_t)i * r + threadIdx.x].y, ai);
     }
+}
+
+template <typename VT>
+__global__ void k_dots(const VT *__restrict__ V, int64_t vstride, const VT *__restrict__ W,
+                       int64_t n_elem, int r, double2 *__restrict__ accum) {
+    dots_slice<VT>(V, vstride, W, n_elem, r, accum, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y);
 }
 
 // out[i][k] = sum_b partial[i][b][k]; one workgroup per vector i.  G = 256 / r threads (a power
@@ -1413,15 +1457,20 @@ __device__ __forceinline__ double2 combo_element(double2 w, const TV *__restrict
 }
 
 template <typename TW, typename TV>
-__global__ void k_combo(TW *__restrict__ W, const TV *__restrict__ V, int64_t vstride, int nvec,
-                        const double2 *__restrict__ h, int64_t n_elem, int r, double sign) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void combo_at(TW *__restrict__ W, const TV *__restrict__ V, int64_t vstride, int nvec,
+                                         const double2 *__restrict__ h, int64_t n_elem, int r, double sign,
+                                         int64_t t) {
     if (t >= n_elem) return;
     double hh = 0.0;
     const TW w0 = W[t];
     const double2 w = combo_element(make_double2(w0.x, w0.y), V, vstride, nvec, h, t, (int)(t % r), r, sign, hh);
     W[t].x = w.x;
     W[t].y = w.y;
+}
+template <typename TW, typename TV>
+__global__ void k_combo(TW *__restrict__ W, const TV *__restrict__ V, int64_t vstride, int nvec,
+                        const double2 *__restrict__ h, int64_t n_elem, int r, double sign) {
+    combo_at<TW, TV>(W, V, vstride, nvec, h, n_elem, r, sign, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // The last pass of a reorthogonalised Gram-Schmidt step, with the normalisation folded in:
@@ -1432,10 +1481,9 @@ __global__ void k_combo(TW *__restrict__ W, const TV *__restrict__ V, int64_t vs
 // first projection h is of the order of the rounding of that projection, so the difference
 // cancels only when W itself has collapsed into span(V), where GMRES has converged.
 template <typename VT>
-__global__ void k_combo_normalize(VT *__restrict__ W, const VT *__restrict__ V, int64_t vstride,
-                                  int nvec, const double2 *__restrict__ h, int64_t n_elem, int r,
-                                  double2 *__restrict__ n2) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void combo_normalize_at(VT *__restrict__ W, const VT *__restrict__ V, int64_t vstride,
+                                                   int nvec, const double2 *__restrict__ h, int64_t n_elem, int r,
+                                                   double2 *__restrict__ n2, int64_t t) {
     if (t >= n_elem) return;
     const int k = (int)(t % r);
     double hh = 0.0;
@@ -1446,6 +1494,12 @@ __global__ void k_combo_normalize(VT *__restrict__ W, const VT *__restrict__ V, 
     W[t].x = f * w.x;
     W[t].y = f * w.y;
     if (t < r) n2[t] = make_double2(nu2, 0.0);
+}
+template <typename VT>
+__global__ void k_combo_normalize(VT *__restrict__ W, const VT *__restrict__ V, int64_t vstride,
+                                  int nvec, const double2 *__restrict__ h, int64_t n_elem, int r,
+                                  double2 *__restrict__ n2) {
+    combo_normalize_at<VT>(W, V, vstride, nvec, h, n_elem, r, n2, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_axpby(double2 *__restrict__ Y, const double2 *__restrict__ A, const double2 *__restrict__ B,
@@ -1484,14 +1538,136 @@ __global__ void k_scale_by_norm(TY *__restrict__ Y, const double2 *__restrict__ 
 
 // W[t] /= sqrt(n2[k]) in place (a zero norm leaves zeros)
 template <typename VT>
-__global__ void k_normalize(VT *__restrict__ W, const double2 *__restrict__ n2, int64_t n_elem, int r) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void normalize_at(VT *__restrict__ W, const double2 *__restrict__ n2, int64_t n_elem,
+                                             int r, int64_t t) {
     if (t >= n_elem) return;
     const double nn = n2[t % r].x;
     const double f = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
     const VT w = W[t];
     W[t].x = f * w.x;
     W[t].y = f * w.y;
+}
+template <typename VT>
+__global__ void k_normalize(VT *__restrict__ W, const double2 *__restrict__ n2, int64_t n_elem, int r) {
+    normalize_at<VT>(W, n2, n_elem, r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ------------------------------------------------------------------ grouped launches
+// One launch of a stage for all members of a lockstep group (the channels of a batch, whose
+// shapes differ).  A table in device memory gives each member's arguments and the first
+// workgroup of its share of the grid; a workgroup finds its member by binary search over
+// those starts and runs the member's tile through the same body as the member's own launch.
+constexpr int GROUP_MAX = 64;   // members per grouped launch (larger groups take several)
+struct GroupHead {
+    int n;                        // members (>= 1)
+    int start[GROUP_MAX + 1];     // first workgroup of each member; start[n] = the grid
+};
+// the member whose share holds workgroup b (0 <= b < start[n]): the last m with start[m] <= b,
+// so a member with an empty share is never chosen
+__device__ __forceinline__ int group_member(const GroupHead *__restrict__ h, int b) {
+    int lo = 0, hi = h->n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b >= h->start[mid])
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// k_block's arguments; the member's grid is gx * gy tiles (tile b: bx = b % gx, by = b / gx)
+struct BlockMember {
+    const void *X;
+    void *Y;
+    const int *part_ptr, *part_alpha;
+    const void *blocks, *G;
+    const int *gidx;
+    const void *Zsub;
+    const int *ket_item_ptr;
+    double2 *zero_buf;
+    unsigned *amax;
+    double in_scale;
+    int Na, Nq, Np, r, kc, ne_max, in_cell, out_cell, zero_n, gx;
+};
+template <typename TI, typename TO, typename CT, typename GT>
+__global__ __launch_bounds__(256) void k_block_grp(const GroupHead *__restrict__ h,
+                                                   const BlockMember *__restrict__ mb) {
+    const int m = group_member(h, (int)blockIdx.x);
+    const int b = (int)blockIdx.x - h->start[m];
+    const BlockMember &d = mb[m];
+    block_tile<TI, TO, CT, GT>(static_cast<const TI *>(d.X), static_cast<TO *>(d.Y), d.part_ptr, d.part_alpha,
+                               static_cast<const CT *>(d.blocks), static_cast<const GT *>(d.G), d.gidx, d.Na, d.Nq,
+                               d.Np, d.r, d.kc, d.ne_max, static_cast<const TO *>(d.Zsub), d.in_cell, d.out_cell,
+                               d.ket_item_ptr, d.in_scale, d.zero_buf, d.zero_n, d.amax, b % d.gx, b / d.gx);
+}
+
+// k_cellmfma's arguments; the member's grid is nit items times the v-tiles
+struct ProdMember {
+    const int2 *items;
+    const int *cell_bra;
+    const void *B, *S;
+    void *Upart;
+    int Na, R2, nit;
+};
+// The XCD-aware order runs over the whole grouped grid, so each member's items are dealt to
+// all XCDs in chunks of consecutive items, as in its own launch.
+template <typename T, int MT, int NT, int UN>
+__global__ __launch_bounds__(256) void k_cellmfma_grp(const GroupHead *__restrict__ h,
+                                                      const ProdMember *__restrict__ mb) {
+    const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);
+    const int m = group_member(h, g);
+    const int b = g - h->start[m];
+    const ProdMember &d = mb[m];
+    cell_item<T, MT, NT, UN>(d.items, d.cell_bra, static_cast<const T *>(d.B), static_cast<const T *>(d.S),
+                             static_cast<T *>(d.Upart), d.Na, d.R2, b % d.nit, b / d.nit);
+}
+
+// The vector kernels' arguments (k_dots: V, W, vstride, n_elem, r, out = accum, nvec, nb; the
+// combinations: W, V, vstride, nvec, h, n_elem, r, sign, out = n2; k_normalize: W, h = n2)
+struct VecMember {
+    const void *V;
+    void *W;
+    const double2 *h;
+    double2 *out;
+    int64_t vstride, n_elem;
+    double sign;
+    int r, nvec, nb;
+};
+// 256 threads per workgroup: blockDim.x / r column groups, as k_dots' own launch with
+// 256 - 256 % r threads
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dots_grp(const GroupHead *__restrict__ h, const VecMember *__restrict__ mb) {
+    const int m = group_member(h, (int)blockIdx.x);
+    const int b = (int)blockIdx.x - h->start[m];
+    const VecMember &d = mb[m];
+    dots_slice<VT>(static_cast<const VT *>(d.V), d.vstride, static_cast<const VT *>(d.W), d.n_elem, d.r, d.out,
+                   b % d.nb, d.nb, b / d.nb);
+}
+template <typename TW, typename TV>
+__global__ __launch_bounds__(256) void k_combo_grp(const GroupHead *__restrict__ h, const VecMember *__restrict__ mb) {
+    const int m = group_member(h, (int)blockIdx.x);
+    const int b = (int)blockIdx.x - h->start[m];
+    const VecMember &d = mb[m];
+    combo_at<TW, TV>(static_cast<TW *>(d.W), static_cast<const TV *>(d.V), d.vstride, d.nvec, d.h, d.n_elem, d.r,
+                     d.sign, (int64_t)b * 256 + threadIdx.x);
+}
+template <typename VT>
+__global__ __launch_bounds__(256) void k_combo_normalize_grp(const GroupHead *__restrict__ h,
+                                                             const VecMember *__restrict__ mb) {
+    const int m = group_member(h, (int)blockIdx.x);
+    const int b = (int)blockIdx.x - h->start[m];
+    const VecMember &d = mb[m];
+    combo_normalize_at<VT>(static_cast<VT *>(d.W), static_cast<const VT *>(d.V), d.vstride, d.nvec, d.h, d.n_elem,
+                           d.r, d.out, (int64_t)b * 256 + threadIdx.x);
+}
+template <typename VT>
+__global__ __launch_bounds__(256) void k_normalize_grp(const GroupHead *__restrict__ h,
+                                                       const VecMember *__restrict__ mb) {
+    const int m = group_member(h, (int)blockIdx.x);
+    const int b = (int)blockIdx.x - h->start[m];
+    const VecMember &d = mb[m];
+    normalize_at<VT>(static_cast<VT *>(d.W), d.h, d.n_elem, d.r, (int64_t)b * 256 + threadIdx.x);
 }
 
 // out[idx][k] = in[idx][k] * G[gk][idx] + (idx == unit[k])   (e + x .* G for the adjoint)
@@ -1695,6 +1871,251 @@ static void vc_products(const tt_run_params &rp, const double *V_unco, const dou
                &out.c[(size_t)(c - nu) * 4 * Np * Np], 2 * Np);
     });
     out.valid = true;
+}
+
+// ------------------------------------------------------------------ grouped launches (host)
+// Stages of a grouped Arnoldi step (or of a grouped operator application: stages 0-2), in the
+// order they run; the members of one stage share a kernel instance.
+enum GroupPhase { GP_BLOCK_IN = 0, GP_PRODUCT = 1, GP_BLOCK_OUT = 2, GP_DOTS = 3, GP_COMBO = 4, GP_DOTS2 = 5,
+                  GP_FINAL = 6 };
+struct GroupStage {
+    bool f32 = false;
+    int key = 0;          // product instance: MT * 100 + NT * 10 + UN
+    size_t lds = 0;       // dynamic LDS of the kernel (the largest member's)
+    size_t msize = 0;     // bytes of one member record
+    std::vector<char> mem;
+    std::vector<int> nblk;   // workgroups of each member
+    int count() const { return (int)nblk.size(); }
+};
+// What one grouped call launches: stages by (phase, precision, product instance).
+struct GroupPlan {
+    std::map<std::tuple<int, int, int>, GroupStage> st;
+    template <typename M>
+    void add(int phase, bool f32, int key, const M &m, int64_t nblk, size_t lds = 0) {
+        TT_REQUIRE(nblk >= 0 && nblk < (1 << 30), "grouped launch: member grid out of range");
+        if (nblk == 0) return;   // nothing to run (no work items)
+        GroupStage &g = st[std::make_tuple(phase, f32 ? 1 : 0, key)];
+        TT_REQUIRE(g.msize == 0 || g.msize == sizeof(M), "grouped launch: mixed member records");
+        g.f32 = f32;
+        g.key = key;
+        g.msize = sizeof(M);
+        g.lds = std::max(g.lds, lds);
+        const char *b = reinterpret_cast<const char *>(&m);
+        g.mem.insert(g.mem.end(), b, b + sizeof(M));
+        g.nblk.push_back((int)nblk);
+    }
+    bool empty() const { return st.empty(); }
+};
+
+// Device copies of the tables: a ring of slots, each a pinned host staging area and its
+// device copy.  One grouped call fills a slot, sends it with one copy ahead of its launches and
+// records the slot's event after them; a slot is refilled only after that event.
+struct GroupTables {
+    // two grouped calls per lane and step (the step and the one ahead), several steps in flight
+    static constexpr int NSLOT = 64;
+    size_t cap = 0;
+    char *host[NSLOT] = {};
+    char *dev[NSLOT] = {};
+    hipEvent_t ev[NSLOT] = {};
+    bool pending[NSLOT] = {};
+    int slot = 0;
+    size_t used = 0;
+    ~GroupTables() {
+        for (int i = 0; i < NSLOT; ++i) {
+            if (ev[i]) {
+                if (pending[i]) (void)hipEventSynchronize(ev[i]);
+                (void)hipEventDestroy(ev[i]);
+            }
+            if (host[i]) (void)hipHostFree(host[i]);
+            if (dev[i]) (void)hipFree(dev[i]);
+        }
+    }
+    void begin(size_t need) {
+        slot = (slot + 1) % NSLOT;
+        if (pending[slot]) {
+            TT_HIP(hipEventSynchronize(ev[slot]));
+            pending[slot] = false;
+        }
+        if (need > cap) {   // every slot is replaced: wait for all of them
+            for (int i = 0; i < NSLOT; ++i) {
+                if (pending[i]) TT_HIP(hipEventSynchronize(ev[i]));
+                pending[i] = false;
+                if (host[i]) TT_HIP(hipHostFree(host[i]));
+                if (dev[i]) TT_HIP(hipFree(dev[i]));
+                host[i] = dev[i] = nullptr;
+            }
+            cap = std::max<size_t>({need, 2 * cap, (size_t)64 << 10});
+            for (int i = 0; i < NSLOT; ++i) {
+                TT_HIP(hipHostMalloc(reinterpret_cast<void **>(&host[i]), cap, hipHostMallocDefault));
+                TT_HIP(hipMalloc(reinterpret_cast<void **>(&dev[i]), cap));
+                if (!ev[i]) TT_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+            }
+        }
+        used = 0;
+    }
+    // n bytes at the next 256-byte boundary of the slot; returns their device address
+    void *put(const void *src, size_t n) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        TT_REQUIRE(off + n <= cap, "grouped launch tables exceed their slot");
+        std::memcpy(host[slot] + off, src, n);
+        used = off + n;
+        return dev[slot] + off;
+    }
+    void send(hipStream_t s) {
+        if (used) TT_HIP(hipMemcpyAsync(dev[slot], host[slot], used, hipMemcpyHostToDevice, s));
+    }
+    void end(hipStream_t s) {
+        TT_HIP(hipEventRecord(ev[slot], s));
+        pending[slot] = true;
+    }
+};
+void free_group_tables(GroupTables *g) { delete g; }
+
+// the grouped product instance of a stage key (MT * 100 + NT * 10 + UN)
+template <typename T, int MT, int NT>
+static void launch_prod_grp_un(hipStream_t s, int UN, unsigned grid, const GroupHead *h, const ProdMember *m) {
+    if constexpr (sizeof(T) == 8) {
+        if (UN == 1)
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 1>), dim3(grid), dim3(256), 0, s, h, m);
+        else if (UN == 4)
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 4>), dim3(grid), dim3(256), 0, s, h, m);
+        else
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 2>), dim3(grid), dim3(256), 0, s, h, m);
+    } else {
+        if (UN == 2)
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 2>), dim3(grid), dim3(256), 0, s, h, m);
+        else if (UN == 8)
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 8>), dim3(grid), dim3(256), 0, s, h, m);
+        else
+            hipLaunchKernelGGL((k_cellmfma_grp<T, MT, NT, 4>), dim3(grid), dim3(256), 0, s, h, m);
+    }
+}
+template <typename T, int MT>
+static void launch_prod_grp_nt(hipStream_t s, int NT, int UN, unsigned grid, const GroupHead *h, const ProdMember *m) {
+    switch (NT) {
+        case 1: launch_prod_grp_un<T, MT, 1>(s, UN, grid, h, m); break;
+        case 2: launch_prod_grp_un<T, MT, 2>(s, UN, grid, h, m); break;
+        case 3: launch_prod_grp_un<T, MT, 3>(s, UN, grid, h, m); break;
+        default: launch_prod_grp_un<T, MT, 4>(s, UN, grid, h, m); break;
+    }
+}
+template <typename T>
+static void launch_prod_grp(hipStream_t s, int key, unsigned grid, const GroupHead *h, const ProdMember *m) {
+    const int MT = key / 100, NT = key / 10 % 10, UN = key % 10;
+    switch (MT) {
+        case 1: launch_prod_grp_nt<T, 1>(s, NT, UN, grid, h, m); break;
+        case 2: launch_prod_grp_nt<T, 2>(s, NT, UN, grid, h, m); break;
+        case 3: launch_prod_grp_nt<T, 3>(s, NT, UN, grid, h, m); break;
+        default: launch_prod_grp_nt<T, 4>(s, NT, UN, grid, h, m); break;
+    }
+}
+// the product's instance key for Nalpha, R2 and batch depth un, as launch_cellmm chooses it
+template <typename T>
+static int prod_key(int Na, int R2, int un) {
+    const int MT = Na <= 16 ? 1 : Na <= 32 ? 2 : Na <= 48 ? 3 : 4;
+    if (un == 0) un = cell_un<T>();
+    if (sizeof(T) == 8)
+        un = (un == 1 || un == 4) ? un : 2;
+    else
+        un = (un == 2 || un == 8) ? un : 4;
+    return MT * 100 + cell_nt(R2) * 10 + un;
+}
+
+// Launches the plan's stages in order on s, each as one launch per GROUP_MAX members.
+static void run_group_plan(GroupTables &gt, const GroupPlan &gp, hipStream_t s) {
+    if (gp.empty()) return;
+    const size_t hsz = (sizeof(GroupHead) + 255) & ~(size_t)255;
+    size_t need = 0;
+    for (const auto &kv : gp.st)
+        for (int c0 = 0; c0 < kv.second.count(); c0 += GROUP_MAX) {
+            const int n = std::min(GROUP_MAX, kv.second.count() - c0);
+            need += hsz + (((size_t)n * kv.second.msize + 255) & ~(size_t)255);
+        }
+    gt.begin(need);
+    struct L {
+        int phase;
+        const GroupStage *g;
+        const GroupHead *h;
+        const void *m;
+        unsigned grid;
+    };
+    std::vector<L> ls;
+    for (const auto &kv : gp.st) {
+        const GroupStage &g = kv.second;
+        for (int c0 = 0; c0 < g.count(); c0 += GROUP_MAX) {
+            const int n = std::min(GROUP_MAX, g.count() - c0);
+            GroupHead head{};
+            head.n = n;
+            int64_t tot = 0;
+            for (int i = 0; i < n; ++i) {
+                head.start[i] = (int)tot;
+                tot += g.nblk[c0 + i];
+            }
+            TT_REQUIRE(tot > 0 && tot < (1ll << 31), "grouped launch: grid out of range");
+            head.start[n] = (int)tot;
+            const GroupHead *hd = static_cast<const GroupHead *>(gt.put(&head, sizeof(head)));
+            const void *md = gt.put(g.mem.data() + (size_t)c0 * g.msize, (size_t)n * g.msize);
+            ls.push_back(L{std::get<0>(kv.first), &g, hd, md, (unsigned)tot});
+        }
+    }
+    gt.send(s);
+    for (const L &l : ls) {
+        const dim3 grid(l.grid), blk(256);
+        const bool f = l.g->f32;
+        switch (l.phase) {
+            case GP_BLOCK_IN:
+                if (f)
+                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, float2>), grid, blk, l.g->lds, s, l.h,
+                                       static_cast<const BlockMember *>(l.m));
+                else
+                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, l.h,
+                                       static_cast<const BlockMember *>(l.m));
+                break;
+            case GP_BLOCK_OUT:
+                if (f)
+                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, double2>), grid, blk, l.g->lds, s, l.h,
+                                       static_cast<const BlockMember *>(l.m));
+                else
+                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, l.h,
+                                       static_cast<const BlockMember *>(l.m));
+                break;
+            case GP_PRODUCT:
+                if (f)
+                    launch_prod_grp<float>(s, l.g->key, l.grid, l.h, static_cast<const ProdMember *>(l.m));
+                else
+                    launch_prod_grp<double>(s, l.g->key, l.grid, l.h, static_cast<const ProdMember *>(l.m));
+                break;
+            case GP_DOTS:
+            case GP_DOTS2:
+                if (f)
+                    hipLaunchKernelGGL(k_dots_grp<float2>, grid, blk, 256 * sizeof(double2), s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                else
+                    hipLaunchKernelGGL(k_dots_grp<double2>, grid, blk, 256 * sizeof(double2), s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                break;
+            case GP_COMBO:
+                if (f)
+                    hipLaunchKernelGGL((k_combo_grp<float2, float2>), grid, blk, 0, s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                else
+                    hipLaunchKernelGGL((k_combo_grp<double2, double2>), grid, blk, 0, s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                break;
+            case GP_FINAL:   // FP32: normalisation by the norm of the projected vector; FP64: the second pass
+                if (f)
+                    hipLaunchKernelGGL(k_normalize_grp<float2>, grid, blk, 0, s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                else
+                    hipLaunchKernelGGL(k_combo_normalize_grp<double2>, grid, blk, 0, s, l.h,
+                                       static_cast<const VecMember *>(l.m));
+                break;
+            default:
+                fail(TT_ERR_ARG, "grouped launch: unknown stage");
+        }
+    }
+    TT_HIP(hipGetLastError());
+    gt.end(s);
 }
 
 struct RowOperator {
@@ -2254,6 +2675,79 @@ struct RowOperator {
         }
     }
 
+    // Whether this operator's applications can join a grouped launch: the cell-block product
+    // without the split-half form (FP64 vectors with the FP64 product, FP32 with FP32).
+    bool groupable(int r, bool fp32) const { return cells && !(fp32 && choice(true, 2 * r).h); }
+
+    // apply() as member records of a grouped call (nothing is launched here): the first block
+    // op, the product and the second block op of this operator join stages 0-2 of gp, with the
+    // same arguments apply() gives its own launches.  Untimed.
+    template <typename VT>
+    void group_apply(const VT *X, VT *Y, int r, bool use_G, bool fp32, const VT *Zsub, GroupPlan &gp) {
+        constexpr bool v64 = std::is_same<VT, double2>::value;
+        TT_REQUIRE(v64 != fp32, "grouped applications: FP64 vectors take the FP64 product, FP32 the FP32 one");
+        prepare(r, fp32);
+        TT_REQUIRE(groupable(r, fp32), "grouped applications need the cell-block product (not the split-half one)");
+        const int R2 = 2 * r;
+        const CellItems *ci = next_items ? next_items : &items_for(fp32, R2);
+        next_items = nullptr;
+        if (upart.n < (size_t)std::max(ci->n, 1) * Na * R2) {
+            upart.alloc((size_t)std::max(ci->n, 1) * Na * R2);
+            ++layout_gen;
+        }
+        const int un = choice(fp32, R2).un;
+        BlockMember b{};
+        b.part_ptr = c_ptr.p;
+        b.part_alpha = c_alpha.p;
+        b.gidx = gidx.p;
+        b.Na = Na;
+        b.Nq = Nq;
+        b.Np = Np;
+        b.r = r;
+        b.gx = Na * Nq;
+        b.in_scale = 1.0;
+        b.X = X;
+        b.Y = fp32 ? (void *)S32.p : (void *)S.p;
+        b.blocks = fp32 ? (const void *)c_blocks32.p : (const void *)c_blocks.p;
+        b.G = use_G ? (fp32 ? (const void *)G32.p : (const void *)G.p) : nullptr;
+        b.ne_max = c_npart_max;
+        b.in_cell = 0;
+        b.out_cell = 1;
+        b.zero_buf = zero_next;
+        b.zero_n = zero_next_n;
+        zero_next = nullptr;
+        zero_next_n = 0;
+        BlockGeom g1 = fp32 ? block_geom<float>(r, c_npart_max, Np) : block_geom<double>(r, c_npart_max, Np);
+        b.kc = g1.kc;
+        gp.add(GP_BLOCK_IN, fp32, 0, b, (int64_t)b.gx * g1.gy, g1.lds);
+        ProdMember pm{ci->items.p, cv.partner, fp32 ? (const void *)cv.blk32 : (const void *)cv.blk,
+                      fp32 ? (const void *)S32.p : (const void *)S.p, upart.p, Na, R2, ci->n};
+        const int key = fp32 ? prod_key<float>(Na, R2, un) : prod_key<double>(Na, R2, un);
+        const int ny = (R2 + 16 * cell_nt(R2) - 1) / (16 * cell_nt(R2));
+        TT_REQUIRE(Na >= 1 && Na <= CELL_NA_MAX, "cell-block product needs 1 <= Nalpha <= 64");
+        gp.add(GP_PRODUCT, fp32, key, pm, (int64_t)ci->n * ny);
+        BlockMember o = b;
+        o.X = upart.p;
+        o.Y = Y;
+        o.part_ptr = vc_ptr.p;
+        o.part_alpha = vc_alpha.p;
+        o.blocks = fp32 ? (const void *)vc_blocks32.p : (const void *)vc_blocks.p;
+        o.G = nullptr;
+        o.ne_max = vc_npart_max;
+        o.Zsub = Zsub;
+        o.in_cell = 1;
+        o.out_cell = 0;
+        o.ket_item_ptr = ci->ket_item_ptr.p;
+        o.in_scale = 2.0;
+        o.zero_buf = nullptr;
+        o.zero_n = 0;
+        BlockGeom g3 = fp32 ? block_geom<float>(r, vc_npart_max, Np) : block_geom<double>(r, vc_npart_max, Np);
+        o.kc = g3.kc;
+        gp.add(GP_BLOCK_OUT, fp32, 0, o, (int64_t)o.gx * g3.gy, g3.lds);
+        ++n_apply;
+        if (fp32) ++n_apply32;
+    }
+
     // Ucell = 2 P^T (C_op X) for the row operator, 2 P (VC_op X) for the column operator: the
     // first block op (no G) and the cell product, as a full cell-major panel [(q,p)][alpha][k].
     void stage_product(const double2 *X, double2 *Ucell, int r) {
@@ -2340,10 +2834,18 @@ struct KrylovBase {
     virtual ~KrylovBase() = default;
     bool in_cycle = false;
     int jmax = 0, used = 0;
+    int launched = 0;    // steps of the cycle enqueued
+    bool hold = false;   // the next step is not launched ahead (predicted to be past convergence)
     std::vector<hipEvent_t> step_ev;
     virtual void advance(int j) = 0;       // enqueue step j (and one ahead)
     virtual void take_column(int j) = 0;   // after step_ev[j]: Givens rotations, convergence
     virtual void finish() = 0;             // enqueue x += V z (the x of begin_cycle)
+    // grouped steps: step j's launches as member records of gp, then (after gp has been
+    // launched) the Hessenberg column's copy and step_ev[j]
+    virtual bool fp32_basis() const = 0;
+    virtual hipStream_t stream() const = 0;
+    virtual void add_step(int j, GroupPlan &gp) = 0;
+    virtual void post_step(int j) = 0;
 };
 
 template <typename VT>
@@ -2371,8 +2873,7 @@ struct Krylov : KrylovBase {
     std::vector<cdouble> H, cs, sn, g;
     std::vector<double> ref, prev_rel;  // residual norms are relative to ref[k]
     double tol = 0.0;
-    int launched = 0;
-    bool hold = false, conv = false;
+    bool conv = false;
     double est = 0.0;                   // largest relative residual estimate at the end of the cycle
 
     Krylov(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
@@ -2465,6 +2966,35 @@ struct Krylov : KrylovBase {
         hipLaunchKernelGGL(k_combo_normalize<VT>, dim3(nblk(ne, 256)), dim3(256), 0, op.s, vj1, V.p, ne, j + 1, h2, ne,
                            r, n2);
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, h1, sizeof(double2) * slot_len, hipMemcpyDeviceToHost, op.s));
+    }
+
+    // enqueue_step(j) as member records of a grouped call: the same kernels with the same
+    // arguments, each stage launched once for all members of the group
+    bool fp32_basis() const override { return fp32; }
+    hipStream_t stream() const override { return op.s; }
+    void add_step(int j, GroupPlan &gp) override {
+        VT *vj = V.p + (size_t)j * ne, *vj1 = V.p + (size_t)(j + 1) * ne;
+        double2 *h1 = Hd.p + (size_t)j * slot_len, *h2 = h1 + (m + 1) * r, *n2 = h2 + (m + 1) * r;
+        op.zero_next = h1;
+        op.zero_next_n = slot_len;
+        op.group_apply(vj, vj1, r, true, fp32, vj, gp);
+        const int64_t nbv = nblk(ne, 256);
+        gp.add(GP_DOTS, fp32, 0, VecMember{V.p, vj1, nullptr, h1, ne, ne, 0.0, r, j + 1, nb_dots},
+               (int64_t)nb_dots * (j + 1));
+        gp.add(GP_COMBO, fp32, 0, VecMember{V.p, vj1, h1, nullptr, ne, ne, -1.0, r, j + 1, 0}, nbv);
+        if (fp32) {
+            gp.add(GP_DOTS2, fp32, 0, VecMember{vj1, vj1, nullptr, n2, ne, ne, 0.0, r, 1, nb_dots}, nb_dots);
+            gp.add(GP_FINAL, fp32, 0, VecMember{nullptr, vj1, n2, nullptr, 0, ne, 0.0, r, 0, 0}, nbv);
+        } else {
+            gp.add(GP_DOTS2, fp32, 0, VecMember{V.p, vj1, nullptr, h2, ne, ne, 0.0, r, j + 2, nb_dots},
+                   (int64_t)nb_dots * (j + 2));
+            gp.add(GP_FINAL, fp32, 0, VecMember{V.p, vj1, h2, n2, ne, ne, 0.0, r, j + 1, 0}, nbv);
+        }
+    }
+    void post_step(int j) override {
+        TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, Hd.p + (size_t)j * slot_len, sizeof(double2) * slot_len,
+                              hipMemcpyDeviceToHost, op.s));
+        TT_HIP(hipEventRecord(step_ev[j], op.s));
     }
 
     // Launch Arnoldi step j (a replay of its graph, captured on first use) and record step_ev[j].
@@ -2616,6 +3146,59 @@ static void lockstep_cycle(const std::vector<KrylovBase *> &ks) {
                 any = true;
             }
         if (!any) break;
+        for (auto *k : ks)
+            if (k->in_cycle) TT_HIP(hipEventSynchronize(k->step_ev[j]));
+        for (auto *k : ks) {
+            if (!k->in_cycle) continue;
+            k->take_column(j);
+            if (!k->in_cycle) k->finish();
+        }
+    }
+    for (auto *k : ks)
+        if (k->in_cycle) k->finish();
+}
+
+// Step j of the Krylov objects ks as grouped calls, one per (stream, basis precision): each
+// stage of the step is one launch for the objects of that stream.
+static void grouped_steps(GroupTables &gt, const std::vector<KrylovBase *> &ks, int j) {
+    std::vector<hipStream_t> streams;
+    for (auto *k : ks)
+        if (std::find(streams.begin(), streams.end(), k->stream()) == streams.end()) streams.push_back(k->stream());
+    for (hipStream_t s : streams)
+        for (int f = 0; f < 2; ++f) {
+            GroupPlan gp;
+            std::vector<KrylovBase *> mem;
+            for (auto *k : ks)
+                if (k->stream() == s && (int)k->fp32_basis() == f) {
+                    k->add_step(j, gp);
+                    mem.push_back(k);
+                }
+            if (mem.empty()) continue;
+            run_group_plan(gt, gp, s);
+            for (auto *k : mem) k->post_step(j);
+        }
+}
+
+// lockstep_cycle with grouped launches: the objects' step j (and the step ahead, for those not
+// held back) is one grouped call per stream instead of one launch sequence per object.
+static void lockstep_cycle_grouped(const std::vector<KrylovBase *> &ks, GroupTables &gt) {
+    int jmax = 0;
+    for (auto *k : ks) jmax = std::max(jmax, k->in_cycle ? k->jmax : 0);
+    for (int j = 0; j < jmax; ++j) {
+        std::vector<KrylovBase *> now, ahead;
+        for (auto *k : ks)
+            if (k->in_cycle && k->launched <= j) now.push_back(k);
+        grouped_steps(gt, now, j);
+        for (auto *k : now) k->launched = j + 1;
+        bool any = false;
+        for (auto *k : ks)
+            if (k->in_cycle) {
+                any = true;
+                if (!k->hold && k->launched == j + 1 && k->launched < k->jmax) ahead.push_back(k);
+            }
+        if (!any) break;
+        grouped_steps(gt, ahead, j + 1);
+        for (auto *k : ahead) k->launched = j + 2;
         for (auto *k : ks)
             if (k->in_cycle) TT_HIP(hipEventSynchronize(k->step_ev[j]));
         for (auto *k : ks) {
@@ -3064,8 +3647,10 @@ struct KMember {
 // one cycle per member (a restart cycle of FP64 GMRES, or the inner FP32 solve of a refinement
 // step) followed by the true FP64 residual.  G must be in each operator; the solutions stay in
 // the plans' x and their on-shell elements are fetched (read_onshell) before returning.
+// With gt, each stage of an Arnoldi step, and of the FP64 residual application, is one grouped
+// launch for the members of one stream (tables in gt).
 constexpr int GRAPH_MIN_MEMBERS = 4;
-static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
+static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, GroupTables *gt = nullptr) {
     if (ms.empty()) return;
     const int m = o.restart > 0 ? o.restart : 40;
     const size_t n = ms.size();
@@ -3073,7 +3658,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
     // where the host's launch rate limits the lanes (measured on the 18-channel batch: +5-8%);
     // plain launches for smaller groups, where a graph launch costs more than its kernels' own
     // launches (one channel: -9%)
-    const bool graphs = o.use_graphs > 0 || (o.use_graphs < 0 && n >= (size_t)GRAPH_MIN_MEMBERS);
+    const bool graphs = !gt && (o.use_graphs > 0 || (o.use_graphs < 0 && n >= (size_t)GRAPH_MIN_MEMBERS));
     // the members' streams (one, or the context's lanes): every host wait covers all of them
     std::vector<hipStream_t> streams;
     for (const auto &mb : ms)
@@ -3131,10 +3716,19 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
         if (outer == 0) {
             for (size_t i : act) beta[i] = ms[i].bnorm;
         } else {
+            if (gt)   // the members' FP64 applications M x as one grouped call per stream
+                for (hipStream_t st : streams) {
+                    GroupPlan gp;
+                    for (size_t i : act)
+                        if (ms[i].pl->op.s == st)
+                            ms[i].pl->op.group_apply(ms[i].pl->x.p, ms[i].pl->rres.p, ms[i].L.r, true, false,
+                                                     (const double2 *)ms[i].pl->x.p, gp);
+                    run_group_plan(*gt, gp, st);
+                }
             for (size_t i : act) {   // R = b - M x (FP64) and its column norms
                 SolvePlan &pl = *ms[i].pl;
                 const int64_t ne = pl.op.dim * ms[i].L.r;
-                pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);
+                if (!gt) pl.op.apply(pl.x.p, pl.rres.p, ms[i].L.r, true, false, true, pl.x.p);
                 hipLaunchKernelGGL(k_axpby, dim3(nblk(ne, 256)), dim3(256), 0, pl.op.s, pl.rres.p, pl.b.p, pl.rres.p,
                                    1.0, -1.0, ne);
                 enqueue_norms(i, pl.rres.p);
@@ -3162,7 +3756,10 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
             }
         }
         if (ks.empty()) break;
-        lockstep_cycle(ks);
+        if (gt)
+            lockstep_cycle_grouped(ks, *gt);
+        else
+            lockstep_cycle(ks);
         for (size_t i : act)
             if (!ms[i].done) ms[i].iters += ms[i].mixed ? k32[i]->used : k64[i]->used;
     }
@@ -3178,18 +3775,28 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o) {
 // process, the streams share them; measured on the 18-channel batch with plain launches, 8 lanes
 // gave 2.7% more than 4 (16 the same as 8, and 8 lanes on 8 queues less than 4 on 4); with the
 // replayed step graphs, 6 lanes gave 1.5-3% more than 5, 7 or 8 (alternating runs on one box).
-static int lanes_wanted() {
-    static const int n = [] {
-        const char *e = std::getenv("TT_LANES");
-        const int v = e ? std::atoi(e) : 0;
-        return v >= 1 && v <= 16 ? v : 6;
-    }();
-    return n;
+static int lanes_wanted() {   // read at every batch call
+    const char *e = std::getenv("TT_LANES");
+    const int v = e ? std::atoi(e) : 0;
+    return v >= 1 && v <= 16 ? v : 6;
+}
+// TT_GROUPED=1: every stage of the steps is one grouped launch for the members of each lane
+// (solve_lockstep with tables; TT_LANES=1 puts all members in one group), where all of them
+// take the cell-block product and the split-half form is off.
+static bool grouped_wanted() {
+    const char *e = std::getenv("TT_GROUPED");
+    return e && std::atoi(e) == 1;
 }
 static void solve_on_lanes(tt_ctx *ctx, std::vector<KMember> &ms, const tt_solve_opts &o) {
+    GroupTables *gt = nullptr;
+    if (grouped_wanted() && ms.size() > 1 && cell_h_mode() == 0 &&
+        std::all_of(ms.begin(), ms.end(), [](const KMember &mb) { return mb.pl->op.cells; })) {
+        if (!ctx->groups) ctx->groups = new GroupTables;
+        gt = ctx->groups;
+    }
     const int L = std::min<int>((int)ms.size(), lanes_wanted());
     if (L <= 1) {
-        solve_lockstep(ms, o);
+        solve_lockstep(ms, o, gt);
         return;
     }
     while ((int)ctx->lanes.size() < L) {
@@ -3213,7 +3820,7 @@ static void solve_on_lanes(tt_ctx *ctx, std::vector<KMember> &ms, const tt_solve
     TT_HIP(hipEventRecord(ready, ctx->stream));
     for (int l = 0; l < L; ++l) TT_HIP(hipStreamWaitEvent(ctx->lanes[l], ready, 0));
     for (size_t i = 0; i < ms.size(); ++i) ms[i].pl->op.s = ctx->lanes[i % L];
-    solve_lockstep(ms, o);
+    solve_lockstep(ms, o, gt);
 }
 
 // M[c][r0 + k] (column-major, leading dim dim) = delta - A[r0 + k][c] G[c] from the rows
@@ -3392,6 +3999,8 @@ void tt_ctx_destroy(tt_ctx *c) {
     for (hipStream_t st : c->lanes) (void)hipStreamSynchronize(st);
     free_plan_cache(c->plans);
     c->plans = nullptr;
+    free_group_tables(c->groups);
+    c->groups = nullptr;
     for (hipStream_t st : c->lanes) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -217,6 +217,9 @@ void swp_derivative(const tt_run_params &rp, const tt_fwp_states &fwp, const tt_
 // Solve plans cached per context (operator set-up, Krylov workspaces, captured graphs).
 struct PlanCache;
 void free_plan_cache(PlanCache *c);
+// Device tables of grouped launches (one launch per stage for all channels of a lockstep group).
+struct GroupTables;
+void free_group_tables(GroupTables *g);
 
 }  // namespace tt
 
@@ -247,4 +250,5 @@ struct tt_ctx {
     // extra streams (default priority) over which a lockstep group's channels are spread, so
     // that independent channels' kernels overlap; created on first use
     std::vector<hipStream_t> lanes;
+    tt::GroupTables *groups = nullptr;   // created on first grouped solve
 };
