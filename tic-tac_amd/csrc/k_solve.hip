@@ -805,10 +805,17 @@ __device__ __forceinline__ void cell_item(const int2 *__restrict__ items, const 
         }
 }
 
+// FP32 instances of at most 12 tiles with batches of up to 2 steps are held to 128 registers,
+// four waves per SIMD (instead of three at 74 + 72 for the 4 x 3 instance): with no spills,
+// the Nalpha = 63, R2 = 36 product alone took 102 instead of 115-119 us on one box
 template <typename T, int MT, int NT, int UN>
-__global__ __launch_bounds__(256) void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra,
-                                                  const T *__restrict__ B, const T *__restrict__ S,
-                                                  T *__restrict__ Upart, int Na, int R2) {
+constexpr int cell_waves_per_eu() {
+    return sizeof(T) == 4 && UN <= 2 && MT * NT <= 12 ? 4 : 1;
+}
+template <typename T, int MT, int NT, int UN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(cell_waves_per_eu<T, MT, NT, UN>())))
+void k_cellmfma(const int2 *__restrict__ items, const int *__restrict__ cell_bra, const T *__restrict__ B,
+                const T *__restrict__ S, T *__restrict__ Upart, int Na, int R2) {
     cell_item<T, MT, NT, UN>(items, cell_bra, B, S, Upart, Na, R2, xcd_order((int)blockIdx.x, (int)gridDim.x),
                              (int)blockIdx.y);
 }
@@ -1613,8 +1620,8 @@ struct ProdMember {
 // The XCD-aware order runs over the whole grouped grid, so each member's items are dealt to
 // all XCDs in chunks of consecutive items, as in its own launch.
 template <typename T, int MT, int NT, int UN>
-__global__ __launch_bounds__(256) void k_cellmfma_grp(const GroupHead *__restrict__ h,
-                                                      const ProdMember *__restrict__ mb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(cell_waves_per_eu<T, MT, NT, UN>())))
+void k_cellmfma_grp(const GroupHead *__restrict__ h, const ProdMember *__restrict__ mb) {
     const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);
     const int m = group_member(h, g);
     const int b = g - h->start[m];
