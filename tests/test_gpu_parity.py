@@ -724,6 +724,18 @@ def test_grouped_batch_matches_lane_batch(ctx, grouped_launches):
     assert np.abs(U0 - Uo).max() <= 1e-10 * np.abs(Uo).max()
 
 
+def test_grouped_gradient_matches(ctx, grad_case, grouped_launches):
+    """TT_GROUPED=1 with one lane: the gradient's row and column solves (two operators of
+    opposite orientation in one lockstep group) as grouped launches give the same U and dU/dθ
+    as the default lanes, to 1e-10 of their largest elements."""
+    c = grad_case
+    os.environ["TT_GROUPED"] = "1"
+    os.environ["TT_LANES"] = "1"
+    U, dU, _ = c.prob.gradient(ctx, 0, c.P, c.theta, c.T, basis=c.basis)
+    assert np.abs(U - c.U).max() <= 1e-10 * np.abs(c.U).max()
+    assert np.abs(dU - c.dU).max() <= 1e-10 * np.abs(c.dU).max()
+
+
 @pytest.fixture
 def split_half_product():
     """Forces the split-half FP16 product for the FP32 applications of a fresh context (the
