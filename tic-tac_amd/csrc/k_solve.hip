@@ -710,7 +710,10 @@ __device__ __forceinline__ void cell_item(const int2 *__restrict__ items, const 
                                           const T *__restrict__ B, const T *__restrict__ S, T *__restrict__ Upart,
                                           int Na, int R2, int item, int vt) {
     typedef typename Mfma16<T>::acc_t acc_t;
-    __shared__ acc_t red[3][MT * NT][64];
+    // cross-wave sum of the accumulators in one round (FP32) or in two through half the LDS
+    // (FP64, whose one-round buffer of 72 KB at MT x NT = 12 allows two workgroups per CU)
+    constexpr bool two_rounds = sizeof(T) == 8;
+    __shared__ acc_t red[two_rounds ? 2 : 3][MT * NT][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kl = lane >> 4, il = lane & 15;
     const int v0 = vt * 16 * NT;
@@ -781,21 +784,45 @@ __device__ __forceinline__ void cell_item(const int2 *__restrict__ items, const 
         mma_batch(base + stride, a1, b1);
         base += 2 * stride;
     }
-    // waves 1-3 leave all their tiles in LDS at once (one barrier), wave 0 adds them in a fixed
-    // order ((w0 + w1) + w2) + w3 and stores
+    // the four waves' tiles are added in a fixed order.  One round (FP32): waves 1-3 leave all
+    // their tiles at once, wave 0 adds ((w0 + w1) + w2) + w3.  Two rounds (FP64): waves 2 and 3
+    // leave their tiles, waves 0 and 1 add them, wave 1 leaves its sum and wave 0 adds
+    // (w0 + w2) + (w1 + w3).  Wave 0 stores.
+    if constexpr (two_rounds) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int n = 0; n < NT; ++n)
-            if (wave > 0) red[wave - 1][m * NT + n][lane] = acc[m][n];
-    __syncthreads();
+            for (int n = 0; n < NT; ++n)
+                if (wave >= 2) red[wave - 2][m * NT + n][lane] = acc[m][n];
+        __syncthreads();
+        if (wave < 2)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) acc[m][n] += red[wave][m * NT + n][lane];
+        __syncthreads();
+        if (wave == 1)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) red[0][m * NT + n][lane] = acc[m][n];
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+                if (wave > 0) red[wave - 1][m * NT + n][lane] = acc[m][n];
+        __syncthreads();
+    }
     if (wave != 0) return;
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             const int tt = m * NT + n;
-            const acc_t t = ((acc[m][n] + red[0][tt][lane]) + red[1][tt][lane]) + red[2][tt][lane];
+            acc_t t = acc[m][n] + red[0][tt][lane];
+            if constexpr (!two_rounds) t = (t + red[1][tt][lane]) + red[2][tt][lane];
             const int v = v0 + 16 * n + il;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
