@@ -305,7 +305,8 @@ tt_status tt_solve_faddeev_equations(tt_ctx *ctx, const tt_run_params *rp, doubl
  * share each A-application).  G is computed on the device for each item (E_com + E_bound,
  * os->E_com required).  With GMRES or MIXED all items' Krylov solves advance in lockstep,
  * spread over the context's stream lanes (TT_LANES, default 6; with TT_GROUPED=1 each stage of
- * a step is one grouped launch for the items of a lane); Pade and dense items are solved
+ * a step is one grouped launch for the items of a lane, the lane's step captured as one graph
+ * when step graphs are on); Pade and dense items are solved
  * one after another.  Each item goes through its channel's cached plan (operator, workspace,
  * captured step graphs), so a repeated batch over the same channels only recomputes G and
  * solves.  No reference counterpart: it replaces

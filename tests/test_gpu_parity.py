@@ -690,8 +690,9 @@ def grouped_launches():
 def test_grouped_batch_matches_lane_batch(ctx, grouped_launches):
     """TT_GROUPED=1: each stage of the lockstep steps (the block ops, the cell product per
     kernel instance, the Gram-Schmidt kernels) and of the FP64 residual applications is one
-    launch for all channels of a lane: all channels of the batch with TT_LANES=1, two or one
-    per lane with three lanes.  The channels 2J = 1..7 of both parities at Np = Nq = 8
+    launch for all channels of a lane: all channels of the batch with TT_LANES=1, two or three
+    per lane with three lanes; the steps captured as graphs and replayed (three batches in a
+    row, with single solves and host synchronisations between), and as plain launches.  The channels 2J = 1..7 of both parities at Np = Nq = 8
     (N_alpha 27 to 63: the two-, three- and four-M-tile product instances) give the six-lane
     batch's U to the solve tolerance (1e-10 of max|U|: the dot products sum by device
     atomics), FP64 and mixed, twice in a row with single solves in between; the FP64 result of
@@ -709,13 +710,14 @@ def test_grouped_batch_matches_lane_batch(ctx, grouped_launches):
         os.environ["TT_GROUPED"] = "1"
         for lanes in ("1", "3"):
             os.environ["TT_LANES"] = lanes
-            for rep in range(2):
-                out = prob.solve_batch(ctx, chans, Ps, method=method)
-                for (U, st, status), Ur, ch in zip(out, ref, chans):
-                    assert status == 0 and st.residual <= 1e-12, (method, lanes, rep, ch.Nalpha)
-                    assert np.abs(U - Ur).max() <= 1e-10 * np.abs(Ur).max(), (method, lanes, rep, ch.Nalpha)
-                chans[rep].solve(ctx, Ps[rep], None, method=method, device_resolvent=True)
-                ctx.synchronize()
+            for graphs in (None, False):   # captured grouped steps (the default here), plain
+                for rep in range(3):
+                    out = prob.solve_batch(ctx, chans, Ps, method=method, graphs=graphs)
+                    for (U, st, status), Ur, ch in zip(out, ref, chans):
+                        assert status == 0 and st.residual <= 1e-12, (method, lanes, graphs, rep, ch.Nalpha)
+                        assert np.abs(U - Ur).max() <= 1e-10 * np.abs(Ur).max(), (method, lanes, graphs, rep)
+                    chans[rep].solve(ctx, Ps[rep], None, method=method, device_resolvent=True)
+                    ctx.synchronize()
     os.environ["TT_LANES"] = "1"
     U0 = prob.solve_batch(ctx, chans, Ps, method=tt.SOLVE_GMRES)[0][0]
     c = _setup(ctx, rp, pot, T, channel=0)

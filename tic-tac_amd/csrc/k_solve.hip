@@ -1955,6 +1955,7 @@ struct GroupTables {
     int slot = 0;
     size_t used = 0;
     ~GroupTables() {
+        clear_graphs();
         for (int i = 0; i < NSLOT; ++i) {
             if (ev[i]) {
                 if (pending[i]) (void)hipEventSynchronize(ev[i]);
@@ -2001,6 +2002,28 @@ struct GroupTables {
     void end(hipStream_t s) {
         TT_HIP(hipEventRecord(ev[slot], s));
         pending[slot] = true;
+    }
+    // Captured grouped steps, keyed by (stream, basis precision, step, the members' Krylov
+    // objects), each with its tables in a device buffer of its own.  A graph is kept while every
+    // member's operator layout is the one it was captured with (gens) and no Krylov object has
+    // been made or freed since (kgen: the members' buffers are then unchanged).
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        void *tab = nullptr;
+        std::vector<uint64_t> gens;
+    };
+    std::map<std::vector<uintptr_t>, StepGraph> graphs;
+    uint64_t graphs_kgen = 0;
+    static void drop(StepGraph &g) {
+        if (g.exec || g.tab) (void)hipDeviceSynchronize();   // its last replay may still run
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.tab) (void)hipFree(g.tab);
+        g.exec = nullptr;
+        g.tab = nullptr;
+    }
+    void clear_graphs() {
+        for (auto &kv : graphs) drop(kv.second);
+        graphs.clear();
     }
 };
 void free_group_tables(GroupTables *g) { delete g; }
@@ -2055,25 +2078,23 @@ static int prod_key(int Na, int R2, int un) {
     return MT * 100 + cell_nt(R2) * 10 + un;
 }
 
-// Launches the plan's stages in order on s, each as one launch per GROUP_MAX members.
-static void run_group_plan(GroupTables &gt, const GroupPlan &gp, hipStream_t s) {
-    if (gp.empty()) return;
-    const size_t hsz = (sizeof(GroupHead) + 255) & ~(size_t)255;
-    size_t need = 0;
-    for (const auto &kv : gp.st)
-        for (int c0 = 0; c0 < kv.second.count(); c0 += GROUP_MAX) {
-            const int n = std::min(GROUP_MAX, kv.second.count() - c0);
-            need += hsz + (((size_t)n * kv.second.msize + 255) & ~(size_t)255);
-        }
-    gt.begin(need);
-    struct L {
-        int phase;
-        const GroupStage *g;
-        const GroupHead *h;
-        const void *m;
-        unsigned grid;
+// The plan's launches: each stage in order, one launch per GROUP_MAX members, with the byte
+// offsets of its head and member records in the plan's packed tables (256-byte aligned).
+struct GroupLaunch {
+    int phase;
+    const GroupStage *g;
+    size_t head_off, mem_off;
+    unsigned grid;
+};
+static size_t pack_group_plan(const GroupPlan &gp, std::vector<char> &bytes, std::vector<GroupLaunch> &ls) {
+    bytes.clear();
+    ls.clear();
+    auto put = [&](const void *src, size_t n) {
+        const size_t off = (bytes.size() + 255) & ~(size_t)255;
+        bytes.resize(off + n);
+        std::memcpy(bytes.data() + off, src, n);
+        return off;
     };
-    std::vector<L> ls;
     for (const auto &kv : gp.st) {
         const GroupStage &g = kv.second;
         for (int c0 = 0; c0 < g.count(); c0 += GROUP_MAX) {
@@ -2087,68 +2108,85 @@ static void run_group_plan(GroupTables &gt, const GroupPlan &gp, hipStream_t s) 
             }
             TT_REQUIRE(tot > 0 && tot < (1ll << 31), "grouped launch: grid out of range");
             head.start[n] = (int)tot;
-            const GroupHead *hd = static_cast<const GroupHead *>(gt.put(&head, sizeof(head)));
-            const void *md = gt.put(g.mem.data() + (size_t)c0 * g.msize, (size_t)n * g.msize);
-            ls.push_back(L{std::get<0>(kv.first), &g, hd, md, (unsigned)tot});
+            const size_t ho = put(&head, sizeof(head));
+            const size_t mo = put(g.mem.data() + (size_t)c0 * g.msize, (size_t)n * g.msize);
+            ls.push_back(GroupLaunch{std::get<0>(kv.first), &g, ho, mo, (unsigned)tot});
         }
     }
-    gt.send(s);
-    for (const L &l : ls) {
+    return bytes.size();
+}
+
+// The launches on s, their tables at dev (the packed bytes' device copy).
+static void launch_group_plan(const std::vector<GroupLaunch> &ls, const char *dev, hipStream_t s) {
+    for (const GroupLaunch &l : ls) {
         const dim3 grid(l.grid), blk(256);
         const bool f = l.g->f32;
+        const GroupHead *h = reinterpret_cast<const GroupHead *>(dev + l.head_off);
+        const void *m = dev + l.mem_off;
         switch (l.phase) {
             case GP_BLOCK_IN:
                 if (f)
-                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, float2>), grid, blk, l.g->lds, s, l.h,
-                                       static_cast<const BlockMember *>(l.m));
+                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, float2>), grid, blk, l.g->lds, s, h,
+                                       static_cast<const BlockMember *>(m));
                 else
-                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, l.h,
-                                       static_cast<const BlockMember *>(l.m));
+                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, h,
+                                       static_cast<const BlockMember *>(m));
                 break;
             case GP_BLOCK_OUT:
                 if (f)
-                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, double2>), grid, blk, l.g->lds, s, l.h,
-                                       static_cast<const BlockMember *>(l.m));
+                    hipLaunchKernelGGL((k_block_grp<float2, float2, float, double2>), grid, blk, l.g->lds, s, h,
+                                       static_cast<const BlockMember *>(m));
                 else
-                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, l.h,
-                                       static_cast<const BlockMember *>(l.m));
+                    hipLaunchKernelGGL((k_block_grp<double2, double2, double, double2>), grid, blk, l.g->lds, s, h,
+                                       static_cast<const BlockMember *>(m));
                 break;
             case GP_PRODUCT:
                 if (f)
-                    launch_prod_grp<float>(s, l.g->key, l.grid, l.h, static_cast<const ProdMember *>(l.m));
+                    launch_prod_grp<float>(s, l.g->key, l.grid, h, static_cast<const ProdMember *>(m));
                 else
-                    launch_prod_grp<double>(s, l.g->key, l.grid, l.h, static_cast<const ProdMember *>(l.m));
+                    launch_prod_grp<double>(s, l.g->key, l.grid, h, static_cast<const ProdMember *>(m));
                 break;
             case GP_DOTS:
             case GP_DOTS2:
                 if (f)
-                    hipLaunchKernelGGL(k_dots_grp<float2>, grid, blk, 256 * sizeof(double2), s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL(k_dots_grp<float2>, grid, blk, 256 * sizeof(double2), s, h,
+                                       static_cast<const VecMember *>(m));
                 else
-                    hipLaunchKernelGGL(k_dots_grp<double2>, grid, blk, 256 * sizeof(double2), s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL(k_dots_grp<double2>, grid, blk, 256 * sizeof(double2), s, h,
+                                       static_cast<const VecMember *>(m));
                 break;
             case GP_COMBO:
                 if (f)
-                    hipLaunchKernelGGL((k_combo_grp<float2, float2>), grid, blk, 0, s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL((k_combo_grp<float2, float2>), grid, blk, 0, s, h,
+                                       static_cast<const VecMember *>(m));
                 else
-                    hipLaunchKernelGGL((k_combo_grp<double2, double2>), grid, blk, 0, s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL((k_combo_grp<double2, double2>), grid, blk, 0, s, h,
+                                       static_cast<const VecMember *>(m));
                 break;
             case GP_FINAL:   // FP32: normalisation by the norm of the projected vector; FP64: the second pass
                 if (f)
-                    hipLaunchKernelGGL(k_normalize_grp<float2>, grid, blk, 0, s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL(k_normalize_grp<float2>, grid, blk, 0, s, h, static_cast<const VecMember *>(m));
                 else
-                    hipLaunchKernelGGL(k_combo_normalize_grp<double2>, grid, blk, 0, s, l.h,
-                                       static_cast<const VecMember *>(l.m));
+                    hipLaunchKernelGGL(k_combo_normalize_grp<double2>, grid, blk, 0, s, h,
+                                       static_cast<const VecMember *>(m));
                 break;
             default:
                 fail(TT_ERR_ARG, "grouped launch: unknown stage");
         }
     }
     TT_HIP(hipGetLastError());
+}
+
+// The plan's stages on s, their tables sent from a slot of gt's ring.
+static void run_group_plan(GroupTables &gt, const GroupPlan &gp, hipStream_t s) {
+    if (gp.empty()) return;
+    std::vector<char> bytes;
+    std::vector<GroupLaunch> ls;
+    const size_t n = pack_group_plan(gp, bytes, ls);
+    gt.begin(n);
+    const char *dev = static_cast<const char *>(gt.put(bytes.data(), n));
+    gt.send(s);
+    launch_group_plan(ls, dev, s);
     gt.end(s);
 }
 
@@ -2880,7 +2918,16 @@ struct KrylovBase {
     virtual hipStream_t stream() const = 0;
     virtual void add_step(int j, GroupPlan &gp) = 0;
     virtual void post_step(int j) = 0;
+    // for captured grouped steps: the operator's allocations and layout generation, the
+    // application count of a replayed step, and the two halves of post_step
+    virtual void prepare_step() = 0;
+    virtual uint64_t layout_gen() const = 0;
+    virtual void count_step() = 0;
+    virtual void copy_column(int j) = 0;
+    virtual void record_step(int j) = 0;
 };
+// Bumped whenever a Krylov object is made or freed (captured grouped steps address their buffers).
+static std::atomic<uint64_t> g_krylov_gen{1};
 
 template <typename VT>
 struct Krylov : KrylovBase {
@@ -2911,6 +2958,7 @@ struct Krylov : KrylovBase {
     double est = 0.0;                   // largest relative residual estimate at the end of the cycle
 
     Krylov(RowOperator &o, int r_, int m_, bool graphs_) : op(o), r(r_), m(m_), use_graphs(graphs_) {
+        ++g_krylov_gen;
         TT_REQUIRE(r <= 256, "at most 256 simultaneous systems");
         ne = o.dim * r;
         slot_len = (2 * (m + 1) + 1) * r;
@@ -2933,6 +2981,7 @@ struct Krylov : KrylovBase {
         prev_rel.resize(r);
     }
     ~Krylov() {
+        ++g_krylov_gen;
         for (auto &gr : graphs) {
             if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
         }
@@ -3026,10 +3075,20 @@ struct Krylov : KrylovBase {
         }
     }
     void post_step(int j) override {
+        copy_column(j);
+        record_step(j);
+    }
+    void prepare_step() override { op.prepare(r, fp32); }
+    uint64_t layout_gen() const override { return op.layout_gen; }
+    void count_step() override {
+        ++op.n_apply;
+        if (fp32) ++op.n_apply32;
+    }
+    void copy_column(int j) override {
         TT_HIP(hipMemcpyAsync(Hh + (size_t)j * slot_len, Hd.p + (size_t)j * slot_len, sizeof(double2) * slot_len,
                               hipMemcpyDeviceToHost, op.s));
-        TT_HIP(hipEventRecord(step_ev[j], op.s));
     }
+    void record_step(int j) override { TT_HIP(hipEventRecord(step_ev[j], op.s)); }
 
     // Launch Arnoldi step j (a replay of its graph, captured on first use) and record step_ev[j].
     // The first step of a cycle always runs as plain launches: the product's launch time is
@@ -3193,21 +3252,87 @@ static void lockstep_cycle(const std::vector<KrylovBase *> &ks) {
 }
 
 // Step j of the Krylov objects ks as grouped calls, one per (stream, basis precision): each
-// stage of the step is one launch for the objects of that stream.
-static void grouped_steps(GroupTables &gt, const std::vector<KrylovBase *> &ks, int j) {
+// stage of the step is one launch for the objects of that stream.  With graphs, each such call
+// is captured once (its launches and the members' Hessenberg copies, its tables sent once to a
+// buffer of its own) and replayed afterwards.
+constexpr size_t GROUP_GRAPHS_MAX = 4096;
+static void grouped_step_graph(GroupTables &gt, const std::vector<KrylovBase *> &mem, int j, bool f32,
+                               hipStream_t s) {
+    if (gt.graphs_kgen != g_krylov_gen.load()) {
+        gt.clear_graphs();
+        gt.graphs_kgen = g_krylov_gen.load();
+    }
+    for (auto *k : mem) k->prepare_step();   // allocations first: a capture may not allocate
+    std::vector<uintptr_t> key{(uintptr_t)s, (uintptr_t)f32, (uintptr_t)j};
+    std::vector<uint64_t> gens;
+    for (auto *k : mem) {
+        key.push_back((uintptr_t)k);
+        gens.push_back(k->layout_gen());
+    }
+    auto it = gt.graphs.find(key);
+    if (it != gt.graphs.end() && it->second.gens == gens) {
+        for (auto *k : mem) k->count_step();
+    } else {
+        if (it != gt.graphs.end()) {
+            GroupTables::drop(it->second);
+            gt.graphs.erase(it);
+        }
+        if (gt.graphs.size() >= GROUP_GRAPHS_MAX) gt.clear_graphs();
+        GroupPlan gp;
+        for (auto *k : mem) k->add_step(j, gp);
+        gens.clear();
+        for (auto *k : mem) gens.push_back(k->layout_gen());
+        std::vector<char> bytes;
+        std::vector<GroupLaunch> ls;
+        const size_t n = pack_group_plan(gp, bytes, ls);
+        GroupTables::StepGraph g;
+        g.gens = gens;
+        TT_HIP(hipMalloc(&g.tab, std::max<size_t>(n, 1)));
+        try {
+            TT_HIP(hipMemcpyAsync(g.tab, bytes.data(), n, hipMemcpyHostToDevice, s));
+            TT_HIP(hipStreamSynchronize(s));   // bytes is pageable and local
+            TT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                launch_group_plan(ls, static_cast<const char *>(g.tab), s);
+                for (auto *k : mem) k->copy_column(j);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            TT_HIP(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                g.exec = nullptr;
+                fail(TT_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+        } catch (...) {
+            GroupTables::drop(g);
+            throw;
+        }
+        it = gt.graphs.emplace(key, g).first;
+    }
+    TT_HIP(hipGraphLaunch(it->second.exec, s));
+    for (auto *k : mem) k->record_step(j);
+}
+static void grouped_steps(GroupTables &gt, const std::vector<KrylovBase *> &ks, int j, bool graphs) {
     std::vector<hipStream_t> streams;
     for (auto *k : ks)
         if (std::find(streams.begin(), streams.end(), k->stream()) == streams.end()) streams.push_back(k->stream());
     for (hipStream_t s : streams)
         for (int f = 0; f < 2; ++f) {
-            GroupPlan gp;
             std::vector<KrylovBase *> mem;
             for (auto *k : ks)
-                if (k->stream() == s && (int)k->fp32_basis() == f) {
-                    k->add_step(j, gp);
-                    mem.push_back(k);
-                }
+                if (k->stream() == s && (int)k->fp32_basis() == f) mem.push_back(k);
             if (mem.empty()) continue;
+            if (graphs) {
+                grouped_step_graph(gt, mem, j, f == 1, s);
+                continue;
+            }
+            GroupPlan gp;
+            for (auto *k : mem) k->add_step(j, gp);
             run_group_plan(gt, gp, s);
             for (auto *k : mem) k->post_step(j);
         }
@@ -3215,14 +3340,14 @@ static void grouped_steps(GroupTables &gt, const std::vector<KrylovBase *> &ks, 
 
 // lockstep_cycle with grouped launches: the objects' step j (and the step ahead, for those not
 // held back) is one grouped call per stream instead of one launch sequence per object.
-static void lockstep_cycle_grouped(const std::vector<KrylovBase *> &ks, GroupTables &gt) {
+static void lockstep_cycle_grouped(const std::vector<KrylovBase *> &ks, GroupTables &gt, bool graphs) {
     int jmax = 0;
     for (auto *k : ks) jmax = std::max(jmax, k->in_cycle ? k->jmax : 0);
     for (int j = 0; j < jmax; ++j) {
         std::vector<KrylovBase *> now, ahead;
         for (auto *k : ks)
             if (k->in_cycle && k->launched <= j) now.push_back(k);
-        grouped_steps(gt, now, j);
+        grouped_steps(gt, now, j, graphs);
         for (auto *k : now) k->launched = j + 1;
         bool any = false;
         for (auto *k : ks)
@@ -3231,7 +3356,7 @@ static void lockstep_cycle_grouped(const std::vector<KrylovBase *> &ks, GroupTab
                 if (!k->hold && k->launched == j + 1 && k->launched < k->jmax) ahead.push_back(k);
             }
         if (!any) break;
-        grouped_steps(gt, ahead, j + 1);
+        grouped_steps(gt, ahead, j + 1, graphs);
         for (auto *k : ahead) k->launched = j + 2;
         for (auto *k : ks)
             if (k->in_cycle) TT_HIP(hipEventSynchronize(k->step_ev[j]));
@@ -3692,7 +3817,9 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, Gro
     // where the host's launch rate limits the lanes (measured on the 18-channel batch: +5-8%);
     // plain launches for smaller groups, where a graph launch costs more than its kernels' own
     // launches (one channel: -9%)
-    const bool graphs = !gt && (o.use_graphs > 0 || (o.use_graphs < 0 && n >= (size_t)GRAPH_MIN_MEMBERS));
+    const bool want_graphs = o.use_graphs > 0 || (o.use_graphs < 0 && n >= (size_t)GRAPH_MIN_MEMBERS);
+    const bool graphs = !gt && want_graphs;   // the members' own step graphs
+    const bool group_graphs = gt && want_graphs;   // captured grouped steps
     // the members' streams (one, or the context's lanes): every host wait covers all of them
     std::vector<hipStream_t> streams;
     for (const auto &mb : ms)
@@ -3791,7 +3918,7 @@ static void solve_lockstep(std::vector<KMember> &ms, const tt_solve_opts &o, Gro
         }
         if (ks.empty()) break;
         if (gt)
-            lockstep_cycle_grouped(ks, *gt);
+            lockstep_cycle_grouped(ks, *gt, group_graphs);
         else
             lockstep_cycle(ks);
         for (size_t i : act)
