@@ -383,7 +383,7 @@ __device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 
     for (int i = 0; i < BP; ++i) {
         const int p = p0 + i;
         if (p >= Np) continue;
-        const int64_t dst = (out_cell ? ((int64_t)q * Np + p) * Na + a : ((int64_t)a * Nq + q) * Np + p) * r + k;
+        const int dst = (out_cell ? (q * Np + p) * Na + a : (a * Nq + q) * Np + p) * r + k;
         CT vr = sr[i], vi = si[i];
         if (Zsub) {  // Y = Z - (result): the operator y - (y .* G) A of the Krylov solves
             vr = (CT)Zsub[dst].x - vr;
@@ -396,7 +396,8 @@ __device__ __forceinline__ void block_rows(const CT *__restrict__ Bs, const CT2 
 }
 
 // The tile (bx, by) of the op: bx = a * Nq + q, by = the k-chunk (k_block's grid; a grouped
-// launch passes each member's own tile indices).
+// launch passes each member's own tile indices).  Element indexes are 32-bit: the launches
+// check that dim * r, items * Nalpha * r and n_E * dim stay below 2^31.
 template <typename TI, typename TO, typename CT, typename GT>
 __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restrict__ Y,
                                            const int *__restrict__ part_ptr, const int *__restrict__ part_alpha,
@@ -410,7 +411,7 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
     __shared__ int gk[256];   // gidx of this chunk's columns (kc <= 256)
     if (zero_buf && bx == 0 && by == 0)   // a buffer a later kernel accumulates into
         for (int i = threadIdx.x; i < zero_n; i += blockDim.x) zero_buf[i] = make_double2(0.0, 0.0);
-    const int64_t dim = (int64_t)Na * Nq * Np;
+    const int dim = Na * Nq * Np;
     const int a = bx / Nq, q = bx % Nq;
     const int k0 = by * kc, nk = min(kc, r - k0);
     const int e0 = part_ptr[a], ne = part_ptr[a + 1] - e0;
@@ -461,7 +462,7 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
             const int idx = min(base + u * (int)blockDim.x, total - 1);   // past total: loaded, not stored
             const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
             const int ae = s_alpha[e];
-            const int64_t src = ((int64_t)ae * Nq + q) * Np + pp;  // (alpha, q, p) index
+            const int src = (ae * Nq + q) * Np + pp;  // (alpha, q, p) index
             aes[u] = ae;
             kks[u] = kk;
             if (ket_item_ptr) {
@@ -469,12 +470,12 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
                 i0s[u] = i0;
                 i1s[u] = i1;
 #pragma unroll
-                for (int w = 0; w < 2; ++w) v[u][w] = X[((int64_t)max(0, min(i0 + w, i1 - 1)) * Na + ae) * r + k];
+                for (int w = 0; w < 2; ++w) v[u][w] = X[(max(0, min(i0 + w, i1 - 1)) * Na + ae) * r + k];
             } else {
-                const int64_t srcm = in_cell ? ((int64_t)q * Np + pp) * Na + ae : src;  // memory index of X
+                const int srcm = in_cell ? (q * Np + pp) * Na + ae : src;  // memory index of X
                 v[u][0] = X[srcm * r + k];
             }
-            if (G) gv[u] = G[(int64_t)max(0, gk[kk]) * dim + src];
+            if (G) gv[u] = G[max(0, gk[kk]) * dim + src];
         }
 #pragma unroll
         for (int u = 0; u < EU; ++u) {
@@ -492,7 +493,7 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
                         xi += (double)v[u][w].y;
                     }
                 for (int it = i0 + 2; it < i1; ++it) {
-                    const TI x = X[((int64_t)it * Na + aes[u]) * r + k0 + kks[u]];
+                    const TI x = X[(it * Na + aes[u]) * r + k0 + kks[u]];
                     xr += (double)x.x;
                     xi += (double)x.y;
                 }
@@ -585,6 +586,7 @@ static void launch_block(hipStream_t s, const TI *X, TO *Y, const int *part_ptr,
                          int Nq, int Np,
                          int r, const TO *Zsub, int in_cell, int out_cell, const int *ket_item_ptr,
                          double in_scale, double2 *zero_buf = nullptr, int zero_n = 0, unsigned *amax = nullptr) {
+    TT_REQUIRE((int64_t)Na * Nq * Np * r < INT32_MAX, "block op: dim * r exceeds its 32-bit element indexes");
     const BlockGeom bg = block_geom<CT>(r, ne_max, Np);
     const dim3 grid((unsigned)(Na * Nq), (unsigned)bg.gy), block(256);
     hipLaunchKernelGGL((k_block<TI, TO, CT, GT>), grid, block, bg.lds, s, X, Y, part_ptr, part_alpha, blocks, G, gidx,
@@ -2433,6 +2435,9 @@ struct RowOperator {
                 ++layout_gen;
             }
         }
+        // the block ops index X, Y, the item panels and G with 32-bit element indexes
+        TT_REQUIRE((int64_t)dim * r < INT32_MAX && (int64_t)upart.n < INT32_MAX && (int64_t)G.n < INT32_MAX,
+                   "block op: a vector, the item panels or G exceed its 32-bit element indexes");
     }
 
     static int tune_key(bool fp32, int R2) { return (fp32 ? (1 << 20) + ((cell_h_mode() + 1) << 16) : 0) + R2; }
