@@ -453,6 +453,10 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
     // ket's further item panels are added after, in item order.  (EU = 4 was slower: 2858-2933
     // against 2953-2985 channel-solves/s on the batch, its registers cutting the occupancy.)
     constexpr int EU = 2;
+    // idx -> (e, pp, kk) by reciprocal multiplication: floor((x + 1/2) / d) in single precision is
+    // exact for x < 2^21 (the fractional part stays 1/(2d) away from an integer, far above the
+    // rounding), and total = ne * Np * nk is far below that
+    const float rtile = 1.0f / (float)tile, rnk = 1.0f / (float)nk;
     for (int base = threadIdx.x; base < total; base += EU * blockDim.x) {
         TI v[EU][2];
         GT gv[EU];
@@ -460,7 +464,8 @@ __device__ __forceinline__ void block_tile(const TI *__restrict__ X, TO *__restr
 #pragma unroll
         for (int u = 0; u < EU; ++u) {
             const int idx = min(base + u * (int)blockDim.x, total - 1);   // past total: loaded, not stored
-            const int e = idx / tile, rem = idx - e * tile, pp = rem / nk, kk = rem - (rem / nk) * nk, k = k0 + kk;
+            const int e = (int)(((float)idx + 0.5f) * rtile), rem = idx - e * tile;
+            const int pp = (int)(((float)rem + 0.5f) * rnk), kk = rem - pp * nk, k = k0 + kk;
             const int ae = s_alpha[e];
             const int src = (ae * Nq + q) * Np + pp;  // (alpha, q, p) index
             aes[u] = ae;
