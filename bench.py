@@ -160,11 +160,13 @@ def product_roofline(stats, shapes, kernel, fp32=False, traffic=None, wall_s=Non
     (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None.  mfma: the same launches
     against the MFMA roof of their precision, at 4 real flops per nonzero and complex
     right-hand side (a real value times a complex element: two fused multiply-adds).
-    With lanes > 1 the channels' streams run concurrently, so a launch shares the device with
-    the other lanes' kernels and its event time is its duration under that sharing, plus the time
-    it waits behind another lane's kernels where two lanes share one of HIP's hardware queues;
-    traced_us (traced_product_us) is the kernels' own mean duration in the rocprofv3 trace of
-    the same workload, and frac_kernel_trace the fraction with it.  With
+    The sampled cell-product launches carry events that the runtime sets at the kernel's own
+    start and end (hipExtLaunchKernel).  With lanes > 1 the channels' streams run concurrently,
+    and the sampled launches (the first step of each cycle) start when every lane starts its
+    cycle, so they share the device with the most other kernels: their mean is above the mean
+    over all launches.  traced_us (traced_product_us) is the kernels' mean duration over all
+    their launches in the rocprofv3 trace of the same workload, and frac_kernel_trace the
+    fraction with it.  With
     wall_s, aggregate_achieved = the s8d bytes of every product launch of the timed steps
     (sampled or not) over their wall time: a lower bound on the products' HBM rate.
     stats[i] belongs to the channel of shapes[i % len(shapes)] (nnz, dim, r)."""

@@ -11,6 +11,7 @@
 //   U = 2 P123^T S         gather SpMM over the reference's own CSC arrays (CSC of P = CSR of P^T)
 //   Y = VC_op^T U          block op
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -1190,33 +1191,40 @@ struct CellItems {
     }
 };
 
+// With ev0/ev1 the launch records the kernel's own start and end in them (hipExtLaunchKernel:
+// the dispatch's timestamps, which leave out the time the launch waits in its queue behind
+// other streams' kernels).
 template <typename T, int MT, int NT, int UN>
 static void launch_cellmfma_un(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
-                               T *Upart, int R2) {
+                               T *Upart, int R2, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     const dim3 grid((unsigned)ci.n, (unsigned)((R2 + 16 * NT - 1) / (16 * NT))), block(256);
-    hipLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ci.items, P->partner, blk, S, Upart, P->Na,
-                       R2);
+    if (ev0)
+        hipExtLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ev0, ev1, 0, ci.items, P->partner, blk,
+                              S, Upart, P->Na, R2);
+    else
+        hipLaunchKernelGGL((k_cellmfma<T, MT, NT, UN>), grid, block, 0, s, ci.items, P->partner, blk, S, Upart, P->Na,
+                           R2);
 }
 
 // un: batch depth (one of cell_un_candidates<T>(); 0 = the default)
 template <typename T, int MT, int NT>
 static void launch_cellmfma(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
-                            T *Upart, int R2, int un) {
+                            T *Upart, int R2, int un, hipEvent_t ev0, hipEvent_t ev1) {
     if (un == 0) un = cell_un<T>();
     if constexpr (sizeof(T) == 8) {
         if (un == 1)
-            launch_cellmfma_un<T, MT, NT, 1>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 1>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
         else if (un == 4)
-            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
         else
-            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
     } else {
         if (un == 2)
-            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 2>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
         else if (un == 8)
-            launch_cellmfma_un<T, MT, NT, 8>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 8>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
         else
-            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2);
+            launch_cellmfma_un<T, MT, NT, 4>(s, P, ci, blk, S, Upart, R2, ev0, ev1);
     }
 }
 
@@ -1232,19 +1240,19 @@ static int cell_nt(int R2) {
 
 template <typename T, int MT>
 static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S,
-                               T *Upart, int R2, int un) {
+                               T *Upart, int R2, int un, hipEvent_t ev0, hipEvent_t ev1) {
     switch (cell_nt(R2)) {
         case 1:
-            launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma<T, MT, 1>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
             break;
         case 2:
-            launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma<T, MT, 2>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
             break;
         case 3:
-            launch_cellmfma<T, MT, 3>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma<T, MT, 3>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
             break;
         default:
-            launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma<T, MT, 4>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
             break;
     }
 }
@@ -1253,17 +1261,20 @@ static void launch_cellmfma_nt(hipStream_t s, const CellView *P, const ItemsRef 
 // U = scale * (sum of each ket's item panels), the full product in cell-major order.
 template <typename T>
 static void launch_cellmm(hipStream_t s, const CellView *P, const ItemsRef &ci, const T *blk, const T *S, T *U,
-                          T *Upart, int R2, T scale, int un = 0) {
+                          T *Upart, int R2, T scale, int un = 0, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     TT_REQUIRE(P->Na >= 1 && P->Na <= CELL_NA_MAX, "cell-block product needs 1 <= Nalpha <= 64 (use the CSC gather)");
     if (ci.n > 0) {
         if (P->Na <= 16)
-            launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma_nt<T, 1>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
         else if (P->Na <= 32)
-            launch_cellmfma_nt<T, 2>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma_nt<T, 2>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
         else if (P->Na <= 48)
-            launch_cellmfma_nt<T, 3>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma_nt<T, 3>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
         else
-            launch_cellmfma_nt<T, 4>(s, P, ci, blk, S, Upart, R2, un);
+            launch_cellmfma_nt<T, 4>(s, P, ci, blk, S, Upart, R2, un, ev0, ev1);
+    } else if (ev0) {   // nothing to launch: the interval is empty
+        TT_HIP(hipEventRecord(ev0, s));
+        TT_HIP(hipEventRecord(ev1, s));
     }
     if (U) {
         const int64_t nout = (int64_t)P->NqNp * P->Na * R2;
@@ -2701,14 +2712,15 @@ struct RowOperator {
                 launch_block<double2, double2>(s, X, S.p, c_ptr.p, c_alpha.p, c_blocks.p, c_npart_max,
                                                use_G ? G.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
                                                1.0, zb, zn);
-                if (timed) record(e1);
-                if (cells)
+                if (cells)   // timed: e1 and e2 hold the kernel's own start and end
                     launch_cellmm<double>(s, &cv, ci->ref(), cv.blk, reinterpret_cast<const double *>(S.p), nullptr,
-                                          upart.p, R2, 2.0, un);
-                else
+                                          upart.p, R2, 2.0, un, timed ? e1 : nullptr, timed ? e2 : nullptr);
+                else {
+                    if (timed) record(e1);
                     launch_spmm<double>(s, P, P->val, reinterpret_cast<const double *>(S.p),
                                         reinterpret_cast<double *>(U.p), dim, R2, 2.0);
-                if (timed) record(e2);
+                    if (timed) record(e2);
+                }
                 const double2 *Uin = cells ? reinterpret_cast<const double2 *>(upart.p) : U.p;
                 launch_block<double2, double2>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr,
                                                gidx.p, Na, Nq, Np, r, Zsub, cm, 0, kip, in_scale);
@@ -2729,17 +2741,20 @@ struct RowOperator {
                 launch_block<VT, float2, float, float2>(s, X, S32.p, c_ptr.p, c_alpha.p, c_blocks32.p, c_npart_max,
                                                 use_G ? G32.p : nullptr, gidx.p, Na, Nq, Np, r, nullptr, 0, cm, nullptr,
                                                 1.0, zb, zn, amax);
-            if (timed) record(e1);
-            if (h)
-                launch_cellh(s, &cv, ci->ref(), Ah, Ah_scale, reinterpret_cast<const float *>(S32.p), Sth.p,
-                             reinterpret_cast<float *>(hsc.p), amax, reinterpret_cast<float *>(upart.p), R2, un);
-            else if (cells)
+            if (h || !cells) {
+                if (timed) record(e1);
+                if (h)
+                    launch_cellh(s, &cv, ci->ref(), Ah, Ah_scale, reinterpret_cast<const float *>(S32.p), Sth.p,
+                                 reinterpret_cast<float *>(hsc.p), amax, reinterpret_cast<float *>(upart.p), R2, un);
+                else
+                    launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
+                                       reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
+                if (timed) record(e2);
+            } else {   // timed: e1 and e2 hold the kernel's own start and end
                 launch_cellmm<float>(s, &cv, ci->ref(), cv.blk32, reinterpret_cast<const float *>(S32.p), nullptr,
-                                     reinterpret_cast<float *>(upart.p), R2, 2.0f, un);
-            else
-                launch_spmm<float>(s, P, P->val32, reinterpret_cast<const float *>(S32.p),
-                                   reinterpret_cast<float *>(U32.p), dim, R2, 2.0f);
-            if (timed) record(e2);
+                                     reinterpret_cast<float *>(upart.p), R2, 2.0f, un, timed ? e1 : nullptr,
+                                     timed ? e2 : nullptr);
+            }
             const float2 *Uin = cells ? reinterpret_cast<const float2 *>(upart.p) : U32.p;
             if constexpr (v64)
                 launch_block<float2, VT>(s, Uin, Y, vc_ptr.p, vc_alpha.p, vc_blocks.p, vc_npart_max, nullptr, gidx.p,
