@@ -2,7 +2,8 @@
 //
 // Replaces fill_P123_arrays (CPP/make_permutation_matrix.cpp:1297-1403) and the
 // column-major sort + CSC conversion of CPP/main.cpp:226-244:
-//   k_mask      bin-overlap test per (q',p',q,p) cell        (lines 160-253)
+//   k_mask_bra  bin-overlap test per (q',p',q,p) cell        (lines 160-253)
+//               (k_mask, the per-cell form, where the bra-bin tables exceed 64 KB of LDS)
 //   k_angular   angular basis F_t(phi, x) per bra bin        (CPP/auxiliary.cpp:950-1067)
 //   k_quad      H[cell][t] = sum_{phi,x} F_t W_cell           (CPP/auxiliary.cpp:660-746)
 //               W = w_x w_phi cos sin (kmax^2-kmin^2)/(2 zeta1 zeta2) / norm, staged through LDS
@@ -75,6 +76,55 @@ __global__ void k_mask(unsigned char *__restrict__ mask, const double *__restric
         }
     }
     mask[t] = hit ? 1 : 0;
+}
+
+// k_mask with the work that depends on the bra bin only done once per bra bin: one workgroup per
+// (q', p'), whose k interval per phi node and pi1 / pi2 per (phi, x) node sit in LDS (the same
+// expressions as k_mask, so the same bits), its threads over the (q, p) kets.  Dynamic LDS:
+// 4 Nphi + 2 Nphi Nx doubles.
+__global__ __launch_bounds__(256) void k_mask_bra(unsigned char *__restrict__ mask, const double *__restrict__ pe,
+                                                  const double *__restrict__ qe, const double *__restrict__ x,
+                                                  const double *__restrict__ phi, int Np, int Nq, int Nx, int Nphi) {
+    extern __shared__ double lds[];
+    double *sps = lds, *cps = sps + Nphi, *kmn = cps + Nphi, *kmx = kmn + Nphi;
+    double *z1s = kmx + Nphi, *z2s = z1s + Nphi * Nx;   // [i][j]
+    const int bra = blockIdx.x, qp = bra / Np, pp = bra - qp * Np;
+    const double ppl = pe[pp], ppu = pe[pp + 1], qpl = qe[qp], qpu = qe[qp + 1];
+    const double *ph = phi + (int64_t)bra * Nphi;
+    for (int i = threadIdx.x; i < Nphi; i += blockDim.x) {
+        double sp, cp;
+        sincos(ph[i], &sp, &cp);
+        sps[i] = sp;
+        cps[i] = cp;
+        kmn[i] = fmax(ppl / sp, qpl / cp);
+        kmx[i] = fmin(ppu / sp, qpu / cp);
+    }
+    __syncthreads();
+    for (int ij = threadIdx.x; ij < Nphi * Nx; ij += blockDim.x) {
+        const int i = ij / Nx, j = ij - i * Nx;
+        z1s[ij] = pi1t(sps[i], cps[i], x[j]);
+        z2s[ij] = pi2t(sps[i], cps[i], x[j]);
+    }
+    __syncthreads();
+    const int nket = Nq * Np;
+    for (int c = threadIdx.x; c < nket; c += blockDim.x) {
+        const int q = c / Np, p = c - q * Np;
+        const double pl = pe[p], pu = pe[p + 1], ql = qe[q], qu = qe[q + 1];
+        bool hit = false;
+        for (int i = 0; i < Nphi && !hit; ++i) {
+            const double kmin = kmn[i], kmax = kmx[i];
+            if (!(kmin < kmax)) continue;
+            for (int j = 0; j < Nx; ++j) {
+                const double z1 = z1s[i * Nx + j], z2 = z2s[i * Nx + j];
+                const double hl = fmax(pl / z1, ql / z2), hu = fmin(pu / z1, qu / z2);
+                if (hl < hu && fmax(kmin, hl) < fmin(kmax, hu)) {
+                    hit = true;
+                    break;
+                }
+            }
+        }
+        mask[(int64_t)bra * nket + c] = hit ? 1 : 0;
+    }
 }
 
 // F[b][t][k] for the bra bins rb0 + b, k = iphi*Nx + ix (points fastest, so a wave's stores are
@@ -430,8 +480,13 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
     // (a1) overlap mask
     const int64_t ncell_all = (int64_t)NqNp * NqNp;
     DBuf<unsigned char> d_mask(ncell_all);
-    hipLaunchKernelGGL(k_mask, dim3(nb(ncell_all, 256)), dim3(256), 0, s, d_mask.p, d_pe.p, d_qe.p, d_x.p, d_phi.p, Np,
-                       Nq, Nx, Nphi);
+    const size_t mask_lds = sizeof(double) * ((size_t)4 * Nphi + (size_t)2 * Nphi * Nx);
+    if (mask_lds <= 64 * 1024)
+        hipLaunchKernelGGL(k_mask_bra, dim3(NqNp), dim3(256), mask_lds, s, d_mask.p, d_pe.p, d_qe.p, d_x.p, d_phi.p,
+                           Np, Nq, Nx, Nphi);
+    else   // angular grids too fine for the bra-bin tables: the per-cell form
+        hipLaunchKernelGGL(k_mask, dim3(nb(ncell_all, 256)), dim3(256), 0, s, d_mask.p, d_pe.p, d_qe.p, d_x.p, d_phi.p,
+                           Np, Nq, Nx, Nphi);
     TT_HIP(hipGetLastError());
     std::vector<unsigned char> mask(ncell_all);
     TT_HIP(hipMemcpyAsync(mask.data(), d_mask.p, ncell_all, hipMemcpyDeviceToHost, s));
