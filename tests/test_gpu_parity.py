@@ -84,6 +84,22 @@ def test_p123_c1_matches_oracle_structure_and_values(c1):
     assert np.abs(vo - c1.val).max() <= 1e-13 * np.abs(vo).max()
 
 
+@pytest.mark.parametrize("n_ang", [48, 64])
+def test_p123_mask_forms_match_oracle(ctx, n_ang):
+    """Nphi = Nx = 48 builds the overlap mask from per-bra-bin tables in LDS (38 KB); 64 would
+    need 68 KB and takes the per-cell form.  Both give the oracle's mask, structure and values."""
+    rp, pot, T = synthetic_config(N=8, J_2N_max=1, Nphi=n_ang, Nx=n_ang)
+    prob = tt.Problem(rp, pot)
+    ch = prob.channel(0, T)
+    P = ch.build_p123(ctx)
+    val, row, ptr = P.to_host()
+    ent, mask = pm.p123_rows(oracle_pw(ch), prob.p_edges, prob.q_edges, rp.Nphi, rp.Nx)
+    vo, ro, po = pm.to_csc(ent, ch.dim)
+    assert P.stats.mask_density == pytest.approx(mask.mean(), abs=0)
+    assert np.array_equal(po, ptr) and np.array_equal(ro, row)
+    assert np.abs(vo - val).max() <= 1e-13 * np.abs(vo).max()
+
+
 def test_apply_c1(ctx, c1):
     rng = np.random.default_rng(0)
     X = rng.normal(size=(3, c1.ch.dim)) + 1j * rng.normal(size=(3, c1.ch.dim))
