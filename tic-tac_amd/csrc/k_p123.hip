@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_mask_bra(unsigned char *__restrict__ ma
 // so that a wave's reads of one table entry are conflict-free: the chunk's tables Y2[i][kk],
 // Y3[i][kk], and Y1[i][ix] of all Nx x nodes.
 constexpr int ANG_THREADS = 1024;
+template <bool PAIR>
 __global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F, int rb0, int kc,
                                                          const double *__restrict__ phi, const double *__restrict__ x,
                                                          const double *__restrict__ Y1tab, int n1, int lmax2, int n2,
@@ -174,6 +175,46 @@ __global__ __launch_bounds__(ANG_THREADS) void k_angular(double *__restrict__ F,
     // their products are added (in term order, as a plain loop would)
     const int lane = threadIdx.x & 63, nh = kc >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: term data by scalar loads
+    if constexpr (PAIR) {
+        // Two adjacent points per lane (even Nx, chunks of 128 points), one tuple per wave over the
+        // whole chunk: each table read fetches both points' entries (16 bytes per lane, the pair
+        // never straddles a Y1 row since k and Nx are even), so the LDS instructions and the scalar
+        // term loads per point are halved.  Each point's sum runs over the terms in the same order
+        // and form as below, so F is the same to the bit.
+        const int kk = 2 * lane, k = k0 + kk;
+        const double *y1 = Y1 + min(k, K - 2) % Nx;
+        const double *y2 = Y2 + kk, *y3 = Y3 + kk;
+        for (int t = wave; t < T; t += blockDim.x >> 6) {
+            const int j0 = term_ptr[t], j1 = term_ptr[t + 1];
+            double acc0 = 0.0, acc1 = 0.0;
+            int j = j0;
+            for (; j + 4 <= j1; j += 4) {
+                double2 a[4], bb[4], d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int4 q = tidx[j + u];
+                    a[u] = *reinterpret_cast<const double2 *>(y1 + q.x);
+                    bb[u] = *reinterpret_cast<const double2 *>(y2 + q.y);
+                    d[u] = *reinterpret_cast<const double2 *>(y3 + q.z);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc0 += coef[j + u] * a[u].x * bb[u].x * d[u].x;
+                    acc1 += coef[j + u] * a[u].y * bb[u].y * d[u].y;
+                }
+            }
+            for (; j < j1; ++j) {
+                const int4 q = tidx[j];
+                const double2 a = *reinterpret_cast<const double2 *>(y1 + q.x);
+                const double2 bb = *reinterpret_cast<const double2 *>(y2 + q.y);
+                const double2 d = *reinterpret_cast<const double2 *>(y3 + q.z);
+                acc0 += coef[j] * a.x * bb.x * d.x;
+                acc1 += coef[j] * a.y * bb.y * d.y;
+            }
+            if (k < K) *reinterpret_cast<double2 *>(F + ((int64_t)b * T + t) * K + k) = make_double2(acc0, acc1);
+        }
+        return;
+    }
     const int kk = (wave % nh) * 64 + lane, k = k0 + kk;
     const double *y1 = Y1 + min(k, K - 1) % Nx;
     const double *y2 = Y2 + kk, *y3 = Y3 + kk;
@@ -547,11 +588,13 @@ extern "C" tt_status tt_fill_P123_arrays(tt_ctx *ctx, const tt_run_params *rp, c
         hipLaunchKernelGGL(k_qpoints, dim3(nb((int64_t)NqNp * K, 256)), dim3(256), 0, s, d_Q.p, NqNp, K, Nx, Nphi, Np,
                            d_pe.p, d_qe.p, d_x.p, d_wx.p, d_phi.p, d_wphi.p);
         const int tpb = QUAD_THREADS / 4;   // tuples per k_quad block
+        // two points per lane where the pairs stay inside a Y1 row and the chunk is one wave's
+        const bool ang_pair = Nx % 2 == 0 && kc == 128;
         for (int rb0 = 0; rb0 < NqNp; rb0 += batch) {
             const int nrb = std::min(batch, NqNp - rb0);
-            hipLaunchKernelGGL(k_angular, dim3(nb(K, kc), nrb), dim3(ANG_THREADS), ang_lds(kc), s, d_F.p, rb0, kc,
-                               d_phi.p, d_x.p, d_Y1.p, n1, ang.lmax2, n2, ang.lmax3, n3, K, Nx, Nphi, T, d_tptr.p,
-                               d_coef.p, d_tidx.p);
+            hipLaunchKernelGGL(ang_pair ? k_angular<true> : k_angular<false>, dim3(nb(K, kc), nrb), dim3(ANG_THREADS),
+                               ang_lds(kc), s, d_F.p, rb0, kc, d_phi.p, d_x.p, d_Y1.p, n1, ang.lmax2, n2, ang.lmax3,
+                               n3, K, Nx, Nphi, T, d_tptr.p, d_coef.p, d_tidx.p);
             hipLaunchKernelGGL(k_quad, dim3(nb(T, tpb), nrb), dim3(QUAD_THREADS), 0, s, d_H.p, d_F.p, rb0, T, K, Np,
                                d_rbptr.p, d_rbcells.p, d_cket.p, d_pe.p, d_qe.p, d_Q.p);
         }
