@@ -84,10 +84,11 @@ def test_p123_c1_matches_oracle_structure_and_values(c1):
     assert np.abs(vo - c1.val).max() <= 1e-13 * np.abs(vo).max()
 
 
-@pytest.mark.parametrize("n_ang", [48, 64])
+@pytest.mark.parametrize("n_ang", [47, 48, 64])
 def test_p123_mask_forms_match_oracle(ctx, n_ang):
     """Nphi = Nx = 48 builds the overlap mask from per-bra-bin tables in LDS (38 KB); 64 would
-    need 68 KB and takes the per-cell form.  Both give the oracle's mask, structure and values."""
+    need 68 KB and takes the per-cell form.  The angular basis takes two points per lane at even
+    Nx and one at odd Nx (47).  All give the oracle's mask, structure and values."""
     rp, pot, T = synthetic_config(N=8, J_2N_max=1, Nphi=n_ang, Nx=n_ang)
     prob = tt.Problem(rp, pot)
     ch = prob.channel(0, T)
